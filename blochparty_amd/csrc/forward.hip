@@ -1,0 +1,183 @@
+// Forward kernels: the event loop with the coil contraction on the matrix cores, and the deterministic cross-block
+// signal sum. Replaces the reference's bloch_chunk_mma (src/blochparty/engines/gpu/cuda/kernels.cu.in:2-185) and
+// its float atomicAdd flush (cuda/common.cuh:77-117).
+//
+// Launch: one lane per isochromat, workgroups of 256 lanes (4 waves of 64). Every lane walks the same segments
+// [s0, s1), so segment records are uniform loads (scalar cache) and the control flow never diverges.
+//
+// Coil contraction, per wave: each ADC sample leaves Mxy of the wave's 64 spins; eight samples are staged in LDS as
+// the columns of B (16 x 64: Re Mxy of samples 0..7, then Im), and D = A B on v_mfma_f32_16x16x4f32 (full float32
+// products, float32 accumulation) with A (16 x 64) the wave's sensitivities of one tile of 8 coils, rows Re S_0..7,
+// Im S_0..7, held in registers in the operand order (16 floats per lane per tile, loaded once). With the K index of
+// step u at lane group g being spin 16 g + u, lane l supplies A[l % 16][16 (l / 16) + u] and B[16 (l / 16) + u][l % 16]
+// and receives D[4 (l / 16) + r][l % 16], r = 0..3. The complex sums are then
+//     P_c(t) = D[c][t] - D[8 + c][8 + t],   Q_c(t) = D[c][8 + t] + D[8 + c][t]      (signal before demodulation),
+// one cross-lane read per value. The four waves' sums are added in LDS in a fixed order and written as this
+// workgroup's partial row; hb_signal_reduce adds the workgroups in float64, in order. No atomics: the signal is the
+// same bit for bit on every run.
+#include "bloch_device.h"
+
+namespace hb {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kStgStride = 68;  // floats per staged column row: 64 lanes + 4 (spreads the b128 reads over banks)
+constexpr int kStgFloats = 16 * kStgStride;
+constexpr int kMaxTiles = 16;   // 128 coils: LDS = 4 x 16 x 68 + 4 x 8 x 256 floats = 49 KB
+
+__host__ __device__ constexpr int lds_floats(int nt) { return kWaves * kStgFloats + kWaves * 8 * 16 * nt; }
+
+// The coil sums of the nb (<= 8) staged samples of every wave, added over the workgroup, into the partial rows
+// [row, row + nb) of this workgroup. Every lane of the workgroup calls it (it has barriers).
+template <int NT>
+__device__ __forceinline__ void flush(float* stg, float* wred, const float (&A)[NT > 0 ? NT : 1][16],
+                                      const float* __restrict__ sa, int nt, int nb, int row, int rows,
+                                      float* __restrict__ partial) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, jj = lane & 15;
+    const int vals = 16 * nt;
+    __syncthreads();
+    float b[16];
+    {
+        const f4* src = reinterpret_cast<const f4*>(stg + jj * kStgStride + 16 * g);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const f4 x = src[v];
+            b[4 * v] = x[0], b[4 * v + 1] = x[1], b[4 * v + 2] = x[2], b[4 * v + 3] = x[3];
+        }
+    }
+    const int partner = (lane < 32) ? lane + ((jj < 8) ? 40 : 24) : lane;
+    float* mine = wred + wave * 8 * vals;
+    for (int t = 0; t < nt; ++t) {
+        f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (NT > 0) {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A[t][u], b[u], acc, 0, 0, 0);
+        } else {
+            const float* at = sa + t * 1024;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(at[u * 64], b[u], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float other = __shfl(acc[r], partner, 64);
+            const float v = (jj < 8) ? acc[r] - other : acc[r] + other;
+            if (lane < 32) mine[(jj & 7) * vals + (8 * t + 4 * g + r) * 2 + (jj >> 3)] = v;
+        }
+    }
+    __syncthreads();
+    float* out = partial + ((size_t)blockIdx.x * rows + row) * vals;
+    const int n_out = (row + nb <= rows ? nb : (rows > row ? rows - row : 0)) * vals;  // never past the buffer
+    for (int v = tid; v < n_out; v += kBlock) {
+        float s = wred[v];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) s += wred[w * 8 * vals + v];
+        out[v] = s;
+    }
+    __syncthreads();
+}
+
+template <int NT>
+__global__ __launch_bounds__(kBlock) void forward_kernel(hb_case cs, int s0, int s1, float* __restrict__ partial,
+                                                         int rows) {
+    extern __shared__ float lds[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = blockIdx.x * kBlock + tid;
+    const int np = cs.n_pad;
+    const int nt = NT > 0 ? NT : cs.n_tiles;
+    float* stg = lds + wave * kStgFloats;
+    float* wred = lds + kWaves * kStgFloats;
+    const Spin p = load_spin(cs.spins, cs.pos_lo, np, j);
+    State m;
+    m.mx = cs.state[j], m.my = cs.state[np + j], m.mz = cs.state[2 * np + j];
+    m.lx = cs.state[3 * np + j], m.ly = cs.state[4 * np + j], m.lz = cs.state[5 * np + j];
+    const float* sa = cs.sens_a + (size_t)(blockIdx.x * kWaves + wave) * nt * 1024 + lane;
+    float A[NT > 0 ? NT : 1][16];
+    if constexpr (NT > 0) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int u = 0; u < 16; ++u) A[t][u] = sa[t * 1024 + u * 64];
+    }
+    Cache k = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    int nb = 0, row = 0;
+    for (int s = s0; s < s1; ++s) {
+        const hb_segment q = cs.segs[s];
+        const int f = q.flags;
+        step(q, f, p, m, k);
+        if (f & HB_ADC) {
+            stg[nb * kStgStride + lane] = m.mx + m.lx;
+            stg[(8 + nb) * kStgStride + lane] = m.my + m.ly;
+            if (++nb == 8) {
+                flush<NT>(stg, wred, A, sa, nt, 8, row, rows, partial);
+                row += 8;
+                nb = 0;
+            }
+        }
+        if (f & HB_SPOIL) m.mx = m.my = m.lx = m.ly = 0.0f;
+    }
+    if (nb > 0) flush<NT>(stg, wred, A, sa, nt, nb, row, rows, partial);
+    cs.state[j] = m.mx, cs.state[np + j] = m.my, cs.state[2 * np + j] = m.mz;
+    cs.state[3 * np + j] = m.lx, cs.state[4 * np + j] = m.ly, cs.state[5 * np + j] = m.lz;
+}
+
+// sig[row0 + r][c] = exp(-i psi) sum_b partial[b][r][c], float64, blocks in order.
+__global__ __launch_bounds__(256) void reduce_kernel(const float* __restrict__ partial, int n_blocks, int rows,
+                                                      int vals, int n_coils, const double* __restrict__ psi, int row0,
+                                                      double* __restrict__ sig) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= rows * n_coils) return;
+    const int r = idx / n_coils, c = idx - r * n_coils;
+    const size_t stride = (size_t)rows * vals;
+    const float* src = partial + (size_t)r * vals + 2 * c;
+    double re = 0.0, im = 0.0;
+    for (int b = 0; b < n_blocks; ++b) {
+        const float2 v = *reinterpret_cast<const float2*>(src + b * stride);
+        re += (double)v.x;
+        im += (double)v.y;
+    }
+    double sp, cp;
+    sincos(psi[row0 + r], &sp, &cp);
+    double* out = sig + ((size_t)(row0 + r) * n_coils + c) * 2;
+    out[0] = re * cp + im * sp;
+    out[1] = im * cp - re * sp;
+}
+
+}  // namespace hb
+
+extern "C" int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* partial, int32_t rows, void* stream) {
+    using namespace hb;
+    if (!cs || cs->n_pad <= 0 || cs->n_pad % kBlock || cs->n < 0 || cs->n > cs->n_pad) return hipErrorInvalidValue;
+    if (cs->n_tiles < 1 || cs->n_tiles > kMaxTiles || 8 * cs->n_tiles < cs->n_coils) return hipErrorInvalidValue;
+    if (s0 < 0 || s1 < s0 || s1 > cs->n_segments || rows < 0) return hipErrorInvalidValue;
+    if (s1 == s0) return hipSuccess;
+    const dim3 grid(cs->n_pad / kBlock), block(kBlock);
+    const size_t lds = sizeof(float) * lds_floats(cs->n_tiles);
+    hipStream_t st = (hipStream_t)stream;
+    if (cs->n_tiles == 1)
+        hipLaunchKernelGGL(forward_kernel<1>, grid, block, lds, st, *cs, s0, s1, partial, rows);
+    else if (cs->n_tiles == 2)
+        hipLaunchKernelGGL(forward_kernel<2>, grid, block, lds, st, *cs, s0, s1, partial, rows);
+    else
+        hipLaunchKernelGGL(forward_kernel<0>, grid, block, lds, st, *cs, s0, s1, partial, rows);
+    return hipGetLastError();
+}
+
+extern "C" int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t rows, int32_t n_tiles,
+                                int32_t n_coils, const double* psi, int32_t row0, double* sig, void* stream) {
+    using namespace hb;
+    if (rows < 0 || n_blocks < 1 || n_tiles < 1 || n_tiles > kMaxTiles || n_coils < 1 || 8 * n_tiles < n_coils ||
+        row0 < 0)
+        return hipErrorInvalidValue;
+    if (rows == 0) return hipSuccess;
+    const int total = rows * n_coils;
+    hipLaunchKernelGGL(reduce_kernel, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, partial,
+                       n_blocks, rows, 16 * n_tiles, n_coils, psi, row0, sig);
+    return hipGetLastError();
+}
+
+extern "C" const char* hb_error_string(int code) { return hipGetErrorString((hipError_t)code); }
+
+extern "C" int hb_abi_version(void) { return 2; }

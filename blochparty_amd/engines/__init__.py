@@ -1,0 +1,123 @@
+"""``simulate(..., engine="gpu")``: the drop-in entry point (reference: ``src/blochparty/engines/__init__.py:22-144``
+and ``engines/gpu/__init__.py:54-96``).
+
+The signature, the option names, the engine resolution and the error types are the reference's. This package holds
+one engine, ``"hip"`` (hand-written HIP kernels for gfx950); ``"gpu"`` resolves to it when a ROCm GPU is visible
+and raises ``RuntimeError`` otherwise, as the reference does without a GPU. The reference's CPU engines (``"core"``,
+``"chunked"``) and its Metal / CUDA backends are not part of this package: naming them raises ``ValueError``.
+Results come back float32 / complex64 on the CPU; with inputs that require grad, gradients flow through the
+hand-written adjoint (``engines/adjoint.py``).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+
+import torch
+from torch import Tensor
+
+from ..phantom import Phantom
+from ..result import SimResult
+from ..sequence import Sequence
+from . import hip
+from .capabilities import check, features
+
+ENGINES = ("gpu", "hip")
+GPU_OPTIONS = ("chunk", "group", "mma", "readouts", "nufft_width", "nufft_terms", "pulses")
+_OTHER_ENGINES = ("core", "chunked", "metal", "cuda")
+
+
+def available(engine: str = "gpu") -> bool:
+    """Whether ``engine`` runs here."""
+    if engine in ("gpu", "hip"):
+        return hip.available()
+    if engine in _OTHER_ENGINES:
+        return False
+    raise ValueError(f"engine must be one of {ENGINES}, not {engine!r}")
+
+
+def resolve(engine: str) -> str:
+    """``"gpu"`` is ``"hip"`` where a ROCm GPU and the built library are (RuntimeError when not)."""
+    if engine in _OTHER_ENGINES:
+        raise ValueError(f"engine={engine!r} is not part of blochparty_amd (its engines: {ENGINES})")
+    if engine not in ENGINES:
+        raise ValueError(f"engine must be one of {ENGINES}, not {engine!r}")
+    if not hip.available():
+        raise RuntimeError("engine='gpu': no GPU engine here (the HIP engine needs a ROCm GPU and the built "
+                           "library, python -m blochparty_amd.build)")
+    return "hip"
+
+
+def requires_grad(*objs) -> bool:
+    """Whether any tensor in ``objs`` (tensors, dataclasses such as Phantom and Sequence, tuples) requires grad
+    (reference ``engines/gpu/__init__.py:39-51``)."""
+    for o in objs:
+        if isinstance(o, Tensor):
+            if o.requires_grad:
+                return True
+        elif dataclasses.is_dataclass(o) and not isinstance(o, type):
+            if requires_grad(*(getattr(o, f.name) for f in dataclasses.fields(o))):
+                return True
+        elif isinstance(o, tuple) and requires_grad(*o):
+            return True
+    return False
+
+
+def simulate(
+    phantom: Phantom,
+    seq: Sequence,
+    coils: Tensor | None = None,
+    *,
+    m_init: Tensor | None = None,
+    checkpoint_every: int | None = None,
+    merge_free_runs: bool = True,
+    scanner=None,
+    seed: int = 0,
+    maps=None,
+    engine: str = "gpu",
+    **options,
+) -> SimResult:
+    """Run ``seq`` on ``phantom`` on the GPU.
+
+    coils: (N, C) complex receive sensitivities (default one uniform coil). m_init: (N, 3) starting magnetization
+    (default equilibrium). checkpoint_every: the adjoint stores the state every this many segments (a multiple of 64;
+    default about sqrt(segments)). merge_free_runs must stay True (the core's option). scanner / maps: refused
+    (NotImplementedError) unless None. seed: only meaningful for diffusion, which is refused.
+    options: ``chunk`` (segments per dispatch, default 32768), ``group`` (256), ``mma`` (the matrix cores always
+    contract the coils here; any value is accepted), ``readouts`` ("direct"), ``nufft_width`` / ``nufft_terms``
+    (unused with "direct"), ``pulses`` ("step"). Other keywords raise TypeError.
+    """
+    name = resolve(engine)
+    if unknown := sorted(set(options) - set(GPU_OPTIONS)):
+        raise TypeError(f"simulate(engine={engine!r}) got unexpected keyword argument(s) {', '.join(unknown)}")
+    if not merge_free_runs:
+        raise ValueError(f"merge_free_runs=False is the core's segment-by-segment stepping, not engine={engine!r}'s")
+    chunk = int(options.get("chunk", 32768))
+    group = int(options.get("group", 256))
+    readouts = options.get("readouts", "direct")
+    pulses = options.get("pulses", "step")
+    if group != 256:
+        raise ValueError("the HIP kernels run workgroups of 256")
+    if readouts not in ("direct", "nufft"):
+        raise ValueError(f"readouts must be 'direct' or 'nufft', not {readouts!r}")
+    if readouts == "nufft":
+        raise NotImplementedError("hip engine: readouts='nufft' is not implemented; use readouts='direct'")
+    if pulses not in ("step", "reuse"):
+        raise ValueError(f"pulses must be 'step' or 'reuse', not {pulses!r}")
+    if pulses == "reuse":
+        raise NotImplementedError("hip engine: pulses='reuse' is not implemented; use pulses='step'")
+    if chunk < 1:
+        raise ValueError("chunk must be positive")
+    grad = torch.is_grad_enabled() and requires_grad(phantom, seq, coils, m_init)
+    check(name, features(phantom, seq, m_init, scanner, maps), grad=grad)
+    if grad:
+        from .adjoint import simulate_grad
+
+        return simulate_grad(phantom, seq, coils, m_init, checkpoint_every)
+    from .driver import forward
+
+    with torch.no_grad():
+        return forward(phantom, seq, coils, m_init, chunk=chunk)
+
+
+__all__ = ["ENGINES", "GPU_OPTIONS", "available", "resolve", "simulate"]

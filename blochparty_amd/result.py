@@ -1,0 +1,20 @@
+"""What a simulation returns (reference: ``src/blochparty/result.py:10-15``)."""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+from torch import Tensor
+
+
+@dataclass
+class SimResult:
+    """``signal``: (C, n_adc) complex k-space in ADC order; ``m``: (N, 3) magnetization after the last segment.
+
+    ``m_bound`` / ``m_pools`` keep the reference's field names; the GPU engine here never fills them (bound and
+    free exchanging pools are refused by its capability check)."""
+
+    signal: Tensor
+    m: Tensor
+    m_bound: Tensor | None = None
+    m_pools: Tensor | None = None

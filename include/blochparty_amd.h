@@ -1,0 +1,75 @@
+/*
+ * blochparty_amd: C ABI of the MI355X (gfx950) engine behind blochparty's simulate(..., engine="gpu").
+ *
+ * Every entry point takes plain device pointers, sizes and a hipStream_t (passed as void*), enqueues its kernels on
+ * that stream, allocates nothing and returns a hipError_t as int (0 = success; hb_error_string() names the code).
+ * The caller owns every buffer. Buffers are float32 structure-of-arrays over n_pad spins (n_pad a multiple of 256;
+ * padding spins carry zero PD, zero state and zero sensitivities, so they add nothing).
+ *
+ * The reference binds its GPU kernels from Python through NVRTC + the CUDA driver API
+ * (src/blochparty/engines/gpu/cuda/rtc.py:116-126: every argument a device pointer, the launch on PyTorch's current
+ * stream, errors raised as RuntimeError). Each entry point below names the kernel or host routine it replaces.
+ */
+#ifndef BLOCHPARTY_AMD_H
+#define BLOCHPARTY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Segment flags (the reference's flag bits, src/blochparty/engines/gpu/driver.py:375-382, plus two cache hints). */
+#define HB_RF 1           /* an RF segment (Strang step)                                                   */
+#define HB_ADC 2          /* sample the signal at the end of the segment                                  */
+#define HB_SPOIL 4        /* zero the transverse magnetization at the end of the segment                  */
+#define HB_SAME_FREE 8    /* free segment equal (dt, gradient, b0_offset) to the previous free one here    */
+#define HB_SAME_RF_DT 16  /* RF segment with the previous RF segment's dt (reuse its relaxation factors)    */
+
+/* One segment of the event list, 96 bytes (the reference's segs row + flags, driver.py:357-384). psi is the
+ * receiver phase wrapped to [0, 2 pi) in float64 before the cast (driver.py:357). The float64 block carries the
+ * duration, the gradient moment g dt and the b0_offset area, from which free segments take their phase in float64
+ * (k . r reduced mod 1 cycle per segment). */
+typedef struct hb_segment {
+    float dt, rf_x, rf_y, g_x, g_y, g_z, b0, psi;
+    double dt64, m_x, m_y, m_z, b0_area;
+    int32_t flags, pad[5];
+} hb_segment;
+
+/* Device buffers of one case (driver.py:320-433). spins: (8, n_pad) px py pz pd r1 r2 db0 b1; pos_lo: (3, n_pad)
+ * the float64 positions' remainders (pos = hi + lo) for the float64 k.r phase of free segments; sens_a: coil
+ * sensitivities in the matrix-core operand order, (n_pad / 64, n_tiles, 16, 64) floats, n_tiles = ceil(C / 8), rows
+ * of a tile Re S_0..7 then Im S_0..7. state: (6, n_pad) mx my mz and their two-sum low parts (the reference's M and
+ * L buffers). */
+typedef struct hb_case {
+    const float* spins;
+    const float* pos_lo;
+    const float* sens_a;
+    const hb_segment* segs;
+    float* state;
+    int32_t n, n_pad, n_coils, n_tiles, n_segments;
+} hb_case;
+
+/* Forward event loop over segments [s0, s1) (replaces bloch_chunk_mma, cuda/kernels.cu.in:2-185, one launch per
+ * dispatch of driver.py:658-698). Every ADC sample's coil sums per workgroup of 256 spins go to
+ * partial[(block * rows + r) * 16 * n_tiles + 2 c + {0: re, 1: im}], r the sample's index among this dispatch's
+ * samples (< rows), block = spin / 256; no atomics. */
+int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* partial, int32_t rows, void* stream);
+
+/* Deterministic cross-block sum (replaces the float atomicAdd of cuda/common.cuh:77-89): for r < rows,
+ * sig[((row0 + r) * n_coils + c) * 2 + {re, im}] = exp(-i psi[row0 + r]) * sum over blocks, in float64, blocks in
+ * order. psi: (n_adc,) float64. */
+int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t rows, int32_t n_tiles, int32_t n_coils,
+                     const double* psi, int32_t row0, double* sig, void* stream);
+
+/* Name of a return code. */
+const char* hb_error_string(int code);
+
+/* ABI version (bumped on any change of the structs or signatures above). */
+int hb_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

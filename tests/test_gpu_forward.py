@@ -1,0 +1,115 @@
+"""The HIP forward path against the FP64 oracle (mirrors the reference's tests/engines/test_gpu.py)."""
+
+import math
+
+import pytest
+import torch
+
+import blochparty_amd as bp
+from cases import busy_sequence, c1_case, random_case, random_phantom, rel
+from oracle import core
+
+pytestmark = pytest.mark.gpu
+
+
+def test_c1_gre_2d_64_matches_oracle(gpu):
+    ph, seq, coils = c1_case()
+    print("case C1 2D GRE 64^2, 200 dummies, 8 coils", flush=True)
+    ref_s, ref_m = core.simulate(ph, seq, coils)
+    out = bp.simulate(ph, seq, coils, engine="gpu")
+    assert out.signal.shape == (8, 4096) and out.signal.dtype == torch.complex64
+    assert out.m.shape == (len(ph), 3) and out.m.dtype == torch.float32
+    e_s, e_m = rel(out.signal, ref_s), rel(out.m, ref_m)
+    print(f"C1 rel signal {e_s:.3e} m {e_m:.3e}", flush=True)
+    assert e_s < 1e-5 and e_m < 1e-5
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_random_rf_gradients_off_resonance(gpu, seed):
+    ph, seq = random_case(seed)
+    ref_s, ref_m = core.simulate(ph, seq)
+    out = bp.simulate(ph, seq)
+    assert rel(out.m, ref_m) < 1e-5
+    assert rel(out.signal, ref_s) < 1e-5
+
+
+def test_hard_pulse_fid_closed_form(gpu):
+    ph = bp.Phantom(pos=torch.zeros(1, 3, dtype=torch.float64), pd=torch.ones(1, dtype=torch.float64),
+                    t1=torch.full((1,), 1e9, dtype=torch.float64), t2=torch.full((1,), 0.05, dtype=torch.float64),
+                    db0=torch.full((1,), 37.0, dtype=torch.float64))
+    seq = bp.SequenceBuilder().rf(10e-9, flip_deg=90.0).readout(43, 1e-3).build()
+    s = bp.simulate(ph, seq).signal[0]  # 43 samples: 5 flushes of 8 and a partial one
+    t = (torch.arange(43, dtype=torch.float64) + 0.5) * 1e-3 + 5e-9
+    expected = 1j * torch.exp(-t / 0.05) * torch.exp(-2j * math.pi * 37.0 * t)
+    assert (s.to(torch.complex128) - expected).abs().max() < 1e-5
+
+
+@pytest.mark.parametrize("n_coils", [1, 13, 50])
+def test_coil_counts_and_partial_flushes(gpu, n_coils):
+    ph = bp.shepp_logan(8, ndim=3)
+    seq = bp.gre_3d(8, n_dummy=3)
+    coils = torch.randn(len(ph), n_coils, dtype=torch.complex128, generator=torch.Generator().manual_seed(0))
+    out = bp.simulate(ph, seq, coils).signal
+    ref, _ = core.simulate(ph, seq, coils)
+    assert rel(out, ref) < 1e-5
+
+
+def test_spins_not_multiple_of_block_and_m_init(gpu):
+    g = torch.Generator().manual_seed(3)
+    n = 1000
+    ph = bp.Phantom(pos=(torch.rand(n, 3, generator=g, dtype=torch.float64) - 0.5) * 0.1,
+                    pd=torch.rand(n, generator=g, dtype=torch.float64), t1=torch.full((n,), 0.8, dtype=torch.float64),
+                    t2=torch.full((n,), 0.07, dtype=torch.float64), db0=torch.zeros(n, dtype=torch.float64))
+    m0 = torch.randn(n, 3, generator=g, dtype=torch.float64)
+    seq = bp.gre_2d(16, n_dummy=2)
+    ref_s, ref_m = core.simulate(ph, seq, m_init=m0)
+    out = bp.simulate(ph, seq, m_init=m0)
+    assert rel(out.signal, ref_s) < 1e-5 and rel(out.m, ref_m) < 1e-5
+
+
+def test_gre_b1_b0_offset_rf_spoiling_ramps(gpu):
+    ph = bp.shepp_logan(24)
+    n = len(ph)
+    ph = bp.Phantom(ph.pos, ph.pd, ph.t1, ph.t2, torch.linspace(-50, 50, n, dtype=torch.float64),
+                    b1=torch.linspace(0.8, 1.2, n, dtype=torch.float64))
+    seq = bp.gre_2d(24, n_dummy=10, spoiling="rf", ramps=True)
+    seq.b0_offset = 3.0 * torch.sin(torch.arange(len(seq), dtype=torch.float64) * 1e-3)
+    coils = bp.birdcage_sensitivities(ph.pos, 8)
+    ref_s, ref_m = core.simulate(ph, seq, coils)
+    out = bp.simulate(ph, seq, coils)
+    assert rel(out.signal, ref_s) < 1e-5
+    assert rel(out.m, ref_m) < 1e-5
+
+
+def test_busy_sequence_and_chunk_splits_agree(gpu):
+    ph = random_phantom(300, seed=4)
+    seq = busy_sequence()
+    coils = bp.birdcage_sensitivities(ph.pos, 4)
+    ref_s, ref_m = core.simulate(ph, seq, coils)
+    base = bp.simulate(ph, seq, coils)
+    assert rel(base.signal, ref_s) < 1e-5 and rel(base.m, ref_m) < 1e-5
+    for chunk in (7, 301):
+        out = bp.simulate(ph, seq, coils, chunk=chunk)
+        assert rel(out.signal, ref_s) < 1e-5 and rel(out.m, ref_m) < 1e-5
+
+
+def test_signal_is_bitwise_deterministic(gpu):
+    ph, seq, coils = c1_case(32, n_dummy=5)
+    a = bp.simulate(ph, seq, coils).signal
+    b = bp.simulate(ph, seq, coils).signal
+    assert torch.equal(a, b)
+
+
+def test_sinc_slab_3d_subset_matches_oracle(gpu):
+    """The 3D workload's sequence (2 ms sinc slab, 200 substeps on a 10 us raster) on a small grid."""
+    maps = bp.three_class_maps(8, 3)
+    ph = bp.phantom_from_maps(maps["pd"], maps["t1"], maps["t2"], 0.22, maps["db0"], spins_per_voxel=(1, 2, 2))
+    seq = bp.gre_cartesian((8, 8, 8), flip_deg=10.0, tr=8e-3, slice_thickness=0.22, rf_duration=2e-3,
+                           system=bp.System(rf_raster=10e-6))
+    assert sum(seq.is_rf) == 200 * 64
+    coils = bp.birdcage_sensitivities(ph.pos, 8)
+    ref_s, ref_m = core.simulate(ph, seq, coils)
+    out = bp.simulate(ph, seq, coils)
+    e = rel(out.signal, ref_s)
+    print(f"sinc slab 8^3 x4 rel signal {e:.3e} m {rel(out.m, ref_m):.3e}", flush=True)
+    assert e < 1e-5 and rel(out.m, ref_m) < 1e-5
