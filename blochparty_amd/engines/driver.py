@@ -201,9 +201,10 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
     )
 
 
-def run_forward(case: DeviceCase, lib=None, keep: bool = False) -> list[torch.Tensor] | None:
+def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> list[torch.Tensor] | None:
     """All dispatches on the current stream from ``case.state0``: the state ends in ``case.state``, the signal in
-    ``case.sig``. ``keep``: also return the state before every dispatch (the adjoint's checkpoints)."""
+    ``case.sig``. ``keep``: also return the state before every dispatch (the adjoint's checkpoints). ``timer``:
+    called as timer("begin") / timer("end") around every forward-kernel launch (the benchmark records events)."""
     lib = lib or hip.HIP.lib()
     case.state.copy_(case.state0)
     st = case.struct()
@@ -212,7 +213,11 @@ def run_forward(case: DeviceCase, lib=None, keep: bool = False) -> list[torch.Te
     for s0, s1, r0, nr in zip(case.starts, case.ends, case.row_starts, case.rows):
         if keep:
             ckpts.append(case.state.clone())
+        if timer:
+            timer("begin")
         lib.forward(hip.ctypes.byref(st), s0, s1, case.partial.data_ptr(), nr)
+        if timer:
+            timer("end")
         if nr:
             lib.signal_reduce(case.partial.data_ptr(), n_blocks, nr, case.n_tiles, case.n_coils,
                               case.psi_rows.data_ptr(), r0, case.sig.data_ptr())
