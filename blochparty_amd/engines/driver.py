@@ -8,7 +8,7 @@ the HIP kernels' layouts:
 - segments as 96-byte records: float32 dt, rf, g, b0_offset, psi (wrapped mod 2 pi in float64 first,
   ``driver.py:357``), float64 dt, moment g dt and b0_offset dt, flags (RF 1, ADC 2, spoil 4, ``driver.py:375-382``,
   and two cache hints);
-- the state as float32 hi + lo, split exactly from float64 (``driver.py:386-433``).
+- the state as float32 hi + lo, split from float64 to ~48 bits (``driver.py:386-433``).
 
 The signal is summed per workgroup into partial rows and reduced across workgroups in float64 (``hb_signal_reduce``),
 so the result does not depend on scheduling. ``prepare_case`` / ``run_forward`` split the upload from the timed GPU

@@ -1,0 +1,126 @@
+"""Host side of the GPU engine on the CPU: the C-ABI library loads and exports what the header declares, the
+matrix-core operand layout and the dispatch plan are right, and the API refuses what it should. No GPU calls."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import blochparty_amd as bp
+from blochparty_amd.engines import capabilities, driver, hip
+from cases import busy_sequence, random_phantom
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    out = set()
+    for h in (ROOT / "include").glob("*.h"):
+        out |= set(re.findall(r"\b(hb_\w+)\s*\(", h.read_text()))
+    return out
+
+
+def test_library_exports_every_declared_symbol():
+    if not hip.LIB_PATH.exists():
+        pytest.skip("library not built (run python -m blochparty_amd.build)")
+    lib = ctypes.CDLL(str(hip.LIB_PATH))
+    names = declared_symbols()
+    assert {"hb_forward", "hb_signal_reduce", "hb_error_string", "hb_abi_version"} <= names
+    for name in names:
+        assert hasattr(lib, name), name
+    assert hip.load().hb_abi_version() == hip.ABI_VERSION
+    assert lib.hb_error_string  # callable without a device
+
+
+def test_segment_record_layout():
+    assert driver.SEG_DTYPE.itemsize == 96
+    assert driver.SEG_DTYPE.fields["d"][1] == 32 and driver.SEG_DTYPE.fields["flags"][1] == 72
+
+
+def mfma_16x16x4(a_lanes, b_lanes):
+    """v_mfma_f32_16x16x4f32 on one wave: lane l gives A[l % 16][l // 16] and B[l // 16][l % 16]; returns D (16, 16)."""
+    A = np.zeros((16, 4))
+    B = np.zeros((4, 16))
+    for lane in range(64):
+        A[lane % 16, lane // 16] = a_lanes[lane]
+        B[lane // 16, lane % 16] = b_lanes[lane]
+    return A @ B
+
+
+@pytest.mark.parametrize("n, C", [(200, 13), (64, 8), (100, 1)])
+def test_sensitivity_operand_order_gives_the_coil_sums(n, C):
+    """The kernel's flush, emulated lane by lane: staged Mxy columns, the operand from sens_operand, the P / Q
+    combination across lanes; must equal sum_spins S_c Mxy."""
+    nt = -(-C // 8)
+    n_pad = 256
+    g = torch.Generator().manual_seed(0)
+    coils = torch.randn(n, C, dtype=torch.complex128, generator=g)
+    A = driver.sens_operand(coils, n, n_pad, nt).double().numpy()
+    M = np.zeros((n_pad, 8), complex)
+    M[:n] = torch.randn(n, 8, dtype=torch.complex128, generator=g).numpy()
+    out = np.zeros((8, 8 * nt), complex)
+    for w in range(n_pad // 64):
+        stg = np.concatenate([M[64 * w: 64 * w + 64].real.T, M[64 * w: 64 * w + 64].imag.T])  # (16 cols, 64)
+        for t in range(nt):
+            D = sum(mfma_16x16x4(A[w, t, u], [stg[lane % 16, 16 * (lane // 16) + u] for lane in range(64)])
+                    for u in range(16))
+            acc = np.array([[D[4 * (lane // 16) + r, lane % 16] for r in range(4)] for lane in range(64)])
+            for lane in range(32):
+                jj, gg = lane % 16, lane // 16
+                partner = lane + (40 if jj < 8 else 24)
+                for r in range(4):
+                    v = acc[lane, r] - acc[partner, r] if jj < 8 else acc[lane, r] + acc[partner, r]
+                    out[jj & 7, 8 * t + 4 * gg + r] += v if jj < 8 else 1j * v
+    ref = M[:n].T @ np.concatenate([coils.numpy(), np.zeros((n, 8 * nt - C))], 1)
+    assert np.abs(out - ref).max() < 1e-5 * np.abs(ref).max()
+
+
+def test_dispatch_plan_and_cache_hints():
+    seq = busy_sequence()
+    starts = driver.dispatch_starts(seq, 7, 8)
+    ends = starts[1:] + [len(seq)]
+    adc = np.asarray(seq.adc)
+    assert starts[0] == 0 and all(e > s for s, e in zip(starts, ends))
+    assert all(adc[s:e].sum() <= 8 for s, e in zip(starts, ends))
+    rec = driver.segment_table(seq, starts)
+    fl = rec["flags"]
+    for s in starts:  # no cache hint may reach across a dispatch start
+        nxt_free = next((i for i in range(s, len(seq)) if not seq.is_rf[i]), None)
+        nxt_rf = next((i for i in range(s, len(seq)) if seq.is_rf[i]), None)
+        if nxt_free is not None and all(i >= s for i in [nxt_free]):
+            assert not (fl[nxt_free] & driver.SAME_FREE) or any(
+                not seq.is_rf[i] for i in range(s, nxt_free)), s
+        if nxt_rf is not None:
+            assert not (fl[nxt_rf] & driver.SAME_RF_DT) or any(seq.is_rf[i] for i in range(s, nxt_rf))
+    psi = rec["f"][:, 7]
+    assert (psi >= 0).all() and (psi < 2 * np.pi + 1e-6).all()
+
+
+def test_case_packing_splits_float64_to_48_bits():
+    ph = random_phantom(300, seed=1)
+    m0 = torch.randn(300, 3, dtype=torch.float64)
+    case = driver.prepare_case(ph, busy_sequence(), None, m0, device="cpu")
+    st = case.state.double()
+    assert ((st[:3, :300] + st[3:, :300]) - m0.T).abs().max() < 2.0**-46 * m0.abs().max()
+    pos = case.spins[:3, :300].double() + case.pos_lo[:, :300].double()
+    assert (pos - ph.pos.T).abs().max() < 2.0**-46 * ph.pos.abs().max()
+    assert torch.all(case.spins[3, 300:] == 0) and torch.all(case.state[:, 300:] == 0)
+
+
+def test_api_refusals_without_gpu():
+    ph = random_phantom(10)
+    seq = busy_sequence()
+    with pytest.raises(ValueError):
+        bp.simulate(ph, seq, engine="core")
+    with pytest.raises(ValueError):
+        bp.simulate(ph, seq, engine="tpu")
+    if not bp.available():
+        with pytest.raises(RuntimeError):
+            bp.simulate(ph, seq, engine="gpu")
+    with pytest.raises(NotImplementedError):
+        capabilities.check("hip", capabilities.features(bp.Phantom(ph.pos, ph.pd, ph.t1, ph.t2, ph.db0,
+                                                                   diffusion=torch.ones(10)), seq))
+    capabilities.check("hip", frozenset({"b1", "m_init"}), grad=True)
