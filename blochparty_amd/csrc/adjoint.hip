@@ -1,0 +1,235 @@
+// Adjoint kernels: the reverse sweep of one checkpoint interval and the in-order sum of the per-segment gradients.
+// Replace the reference's bloch_replay + bloch_adjoint_c{NC} (src/blochparty/engines/gpu/cuda/replay.cu.in:3-33,
+// adjoint.cu.in:5-121) and seg_reduce (cuda/seg_reduce.cu:3-11).
+//
+// One launch per interval [s0, s1) of the forward's checkpoints, last interval first; one lane per isochromat:
+//   1. replay the interval from its checkpoint, the state before every 8th segment into HBM scratch (r8);
+//   2. for every block of 8 segments, last first: replay the block from its r8 state with the 9 states it passes
+//      through held in registers (the loop is unrolled, so the indices are static), then sweep it in reverse:
+//      unspoil, add the ADC sample's source term sum_c h_c conj(S_c) and the coil gradients h_c conj(Mxy), reverse
+//      the step (adjoint_device.h).
+// Per-spin gradients are summed in registers and added to float64 accumulators at every segment index that is a
+// multiple of 64; per-segment gradients are summed over each wave (butterfly), then over the workgroup's 4 waves in
+// LDS in a fixed order, one partial per workgroup and segment, and over workgroups in order by seg_reduce. Nothing
+// depends on the interval length (a multiple of 64) or on scheduling: the gradients are the same bit for bit for
+// any checkpoint_every and every run.
+#include "adjoint_device.h"
+
+namespace hb {
+
+constexpr int kBlockA = 256;
+constexpr int kWavesA = kBlockA / 64;
+
+struct Stash {
+    float x, y, z;
+};
+struct StashXY {
+    float x, y;
+};
+
+// Effective flags: cache hints honoured only while the cache holds the factors they refer to.
+__device__ __forceinline__ int live_flags(int f, bool free_ok, bool rf_ok, int same_free, int same_rf) {
+    if (!free_ok) f &= ~same_free;
+    if (!rf_ok) f &= ~same_rf;
+    return f;
+}
+
+template <int NC, bool SEG>
+__global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint ad, int s0, int s1) {
+    __shared__ float segbuf[kWavesA][8][8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = blockIdx.x * kBlockA + tid;
+    const int np = cs.n_pad;
+    const Spin p = load_spin(cs.spins, cs.pos_lo, np, j);
+
+    // 1. replay the interval, r8[b] = state before segment s0 + 8 b (low parts folded)
+    {
+        State m;
+        m.mx = ad.ckpt[j], m.my = ad.ckpt[np + j], m.mz = ad.ckpt[2 * np + j];
+        m.lx = ad.ckpt[3 * np + j], m.ly = ad.ckpt[4 * np + j], m.lz = ad.ckpt[5 * np + j];
+        Cache k = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        bool free_ok = false, rf_ok = false;
+        for (int s = s0; s < s1; ++s) {
+            if (((s - s0) & 7) == 0) {
+                float* r = ad.r8 + (size_t)((s - s0) >> 3) * 3 * np;
+                r[j] = m.mx + m.lx, r[np + j] = m.my + m.ly, r[2 * np + j] = m.mz + m.lz;
+            }
+            const hb_segment q = cs.segs[s];
+            const int f = live_flags(q.flags, free_ok, rf_ok, HB_SAME_FREE, HB_SAME_RF_DT);
+            step(q, f, p, m, k);
+            if (f & HB_RF) rf_ok = true; else free_ok = true;
+            if (f & HB_SPOIL) m.mx = m.my = m.lx = m.ly = 0.0f;
+        }
+    }
+
+    // 2. reverse sweep
+    float lx = ad.lam[j], ly = ad.lam[np + j], lz = ad.lam[2 * np + j];
+    float sr[NC > 0 ? NC : 1], si[NC > 0 ? NC : 1], gr[NC > 0 ? NC : 1], gi[NC > 0 ? NC : 1];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        sr[c] = ad.sens[(2 * c) * (size_t)np + j];
+        si[c] = ad.sens[(2 * c + 1) * (size_t)np + j];
+        gr[c] = gi[c] = 0.0f;
+    }
+    Acc acc;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) acc.v[a] = 0.0f;
+    RevCache rc = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    bool rfree_ok = false, rrf_ok = false;  // reverse caches (hints: same as the next segment of the kind)
+    const int n8 = (s1 - s0 + 7) >> 3;
+    for (int b = n8 - 1; b >= 0; --b) {
+        const int sb = s0 + 8 * b;
+        const int nseg = min(8, s1 - sb);
+        // replay the block with its states in registers: before[i] before segment sb + i, after[i] after its step
+        Stash before[8];
+        StashXY after[8];
+        {
+            const float* r = ad.r8 + (size_t)b * 3 * np;
+            State m;
+            m.mx = r[j], m.my = r[np + j], m.mz = r[2 * np + j];
+            m.lx = m.ly = m.lz = 0.0f;
+            Cache k = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            bool free_ok = false, rf_ok = false;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (i < nseg) {
+                    before[i] = {m.mx + m.lx, m.my + m.ly, m.mz + m.lz};
+                    const hb_segment q = cs.segs[sb + i];
+                    const int f = live_flags(q.flags, free_ok, rf_ok, HB_SAME_FREE, HB_SAME_RF_DT);
+                    step(q, f, p, m, k);
+                    if (f & HB_RF) rf_ok = true; else free_ok = true;
+                    after[i] = {m.mx + m.lx, m.my + m.ly};
+                    if (f & HB_SPOIL) m.mx = m.my = m.lx = m.ly = 0.0f;
+                }
+            }
+        }
+        float sg[N_SEG_GRADS];
+#pragma unroll
+        for (int i = 7; i >= 0; --i) {
+            if (i < nseg) {
+                const int s = sb + i;
+                const hb_segment q = cs.segs[s];
+                const int f = live_flags(q.flags, rfree_ok, rrf_ok, HB_NEXT_FREE, HB_NEXT_RF_DT);
+                if (f & HB_SPOIL) lx = ly = 0.0f;
+                if (NC > 0 && (f & HB_ADC)) {
+                    const float* hrow = ad.h + (size_t)q.pad[0] * 2 * NC;
+                    const float ax = after[i].x, ay = after[i].y;
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        const float hr = hrow[2 * c], hi = hrow[2 * c + 1];
+                        lx += hr * sr[c] + hi * si[c];
+                        ly += hi * sr[c] - hr * si[c];
+                        gr[c] += hr * ax + hi * ay;
+                        gi[c] += hi * ax - hr * ay;
+                    }
+                }
+#pragma unroll
+                for (int g = 0; g < N_SEG_GRADS; ++g) sg[g] = 0.0f;
+                // the reverse hints name the next segment of the kind: map them onto the step's bits
+                const int fr = (f & (HB_RF | HB_ADC | HB_SPOIL)) | ((f & HB_NEXT_FREE) ? HB_SAME_FREE : 0) |
+                               ((f & HB_NEXT_RF_DT) ? HB_SAME_RF_DT : 0);
+                reverse_step<SEG>(q, fr, p, before[i].x, before[i].y, before[i].z, lx, ly, lz, acc, sg, rc);
+                if (f & HB_RF) rrf_ok = true; else rfree_ok = true;
+                if (SEG) {
+#pragma unroll
+                    for (int g = 0; g < N_SEG_GRADS; ++g) {
+                        float v = sg[g];
+#pragma unroll
+                        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+                        if (lane == 0) segbuf[wave][i][g] = v;
+                    }
+                }
+            }
+        }
+        if (SEG) {
+            __syncthreads();
+            if (tid < 8 * nseg) {
+                const int i = tid >> 3, g = tid & 7;
+                float v = 0.0f;
+                if (g < N_SEG_GRADS) {
+                    v = segbuf[0][i][g];
+#pragma unroll
+                    for (int w = 1; w < kWavesA; ++w) v += segbuf[w][i][g];
+                }
+                ad.gseg[((size_t)blockIdx.x * ad.k_max + (sb - s0 + i)) * 8 + g] = v;
+            }
+            __syncthreads();
+        }
+        if ((sb & 63) == 0) {  // every 64 segments: into the float64 accumulators
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                ad.gs[(size_t)a * np + j] += (double)acc.v[a];
+                acc.v[a] = 0.0f;
+            }
+            if (NC > 0 && ad.coil_grads) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    ad.gc[(size_t)(2 * c) * np + j] += (double)gr[c];
+                    ad.gc[(size_t)(2 * c + 1) * np + j] += (double)gi[c];
+                    gr[c] = gi[c] = 0.0f;
+                }
+            }
+        }
+    }
+    // an interval that does not start at a multiple of 64 leaves a remainder (only possible for the first one)
+#pragma unroll
+    for (int a = 0; a < 8; ++a) ad.gs[(size_t)a * np + j] += (double)acc.v[a];
+    if (NC > 0 && ad.coil_grads) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            ad.gc[(size_t)(2 * c) * np + j] += (double)gr[c];
+            ad.gc[(size_t)(2 * c + 1) * np + j] += (double)gi[c];
+        }
+    }
+    ad.lam[j] = lx, ad.lam[np + j] = ly, ad.lam[2 * np + j] = lz;
+}
+
+// seg[s0 + k][g] = sum over workgroups of gseg[b][k][g], in order, float64.
+__global__ __launch_bounds__(256) void seg_reduce_kernel(const float* __restrict__ gseg, int n_blocks, int k_max,
+                                                         int s0, int len, double* __restrict__ seg) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 8 * len) return;
+    const int k = t >> 3, g = t & 7;
+    double s = 0.0;
+    for (int b = 0; b < n_blocks; ++b) s += (double)gseg[((size_t)b * k_max + k) * 8 + g];
+    seg[(size_t)(s0 + k) * 8 + g] = s;
+}
+
+template <int NC>
+static hipError_t launch_adjoint(const hb_case* cs, const hb_adjoint* ad, int s0, int s1, hipStream_t st) {
+    const dim3 grid(cs->n_pad / kBlockA), block(kBlockA);
+    if (ad->seg_grads)
+        hipLaunchKernelGGL((adjoint_kernel<NC, true>), grid, block, 0, st, *cs, *ad, s0, s1);
+    else
+        hipLaunchKernelGGL((adjoint_kernel<NC, false>), grid, block, 0, st, *cs, *ad, s0, s1);
+    return hipGetLastError();
+}
+
+}  // namespace hb
+
+extern "C" int hb_adjoint_interval(const hb_case* cs, const hb_adjoint* ad, int32_t s0, int32_t s1, void* stream) {
+    using namespace hb;
+    if (!cs || !ad || cs->n_pad <= 0 || cs->n_pad % kBlockA) return hipErrorInvalidValue;
+    if (s0 < 0 || s1 < s0 || s1 > cs->n_segments || s1 - s0 > ad->k_max) return hipErrorInvalidValue;
+    if (s1 == s0) return hipSuccess;
+    hipStream_t st = (hipStream_t)stream;
+    switch (ad->nc) {
+        case 0: return launch_adjoint<0>(cs, ad, s0, s1, st);
+        case 1: return launch_adjoint<1>(cs, ad, s0, s1, st);
+        case 2: return launch_adjoint<2>(cs, ad, s0, s1, st);
+        case 4: return launch_adjoint<4>(cs, ad, s0, s1, st);
+        case 8: return launch_adjoint<8>(cs, ad, s0, s1, st);
+        case 16: return launch_adjoint<16>(cs, ad, s0, s1, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+extern "C" int hb_seg_reduce(const float* gseg, int32_t n_blocks, int32_t k_max, int32_t s0, int32_t len,
+                             double* seg, void* stream) {
+    using namespace hb;
+    if (n_blocks < 1 || len < 0 || len > k_max || s0 < 0) return hipErrorInvalidValue;
+    if (len == 0) return hipSuccess;
+    hipLaunchKernelGGL(seg_reduce_kernel, dim3((8 * len + 255) / 256), dim3(256), 0, (hipStream_t)stream, gseg,
+                       n_blocks, k_max, s0, len, seg);
+    return hipGetLastError();
+}

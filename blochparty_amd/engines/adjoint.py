@@ -1,0 +1,168 @@
+"""The GPU engine's gradients: a ``torch.autograd.Function`` around the forward kernels with checkpoints and the
+hand-written adjoint kernels (reference: ``src/blochparty/engines/gpu/adjoint.py:23-323``).
+
+The contract is the reference's ``_GPUSimulate``: inputs (pos, pd, t1, t2, db0, coils as real pairs (N, C, 2), dt, rf,
+grad, adc_phase, m0, b1, b0_offset); outputs (signal (n_adc, C, 2), m (N, 3)), float32 on the CPU; first-order
+gradients for every input that requires them, each in its input's dtype and device; ``checkpoint_every`` a multiple
+of 64 (default about sqrt(segments)), else ``ValueError``.
+
+Forward: the dispatches are exactly the checkpoint intervals, the state before each is kept on the device (hi + lo,
+24 bytes per spin per interval). Backward: dL/dsignal turned by exp(+i psi) (the source of the sweep), then one
+``hb_adjoint_interval`` per interval, last first, and ``hb_seg_reduce`` for the per-segment gradients. Parameter
+gradients map as the reference's (adjoint.py:222-261): t1 = 1 / r1 (-dL/dr1 / t1^2), t2 likewise, d/dpsi from the
+signal itself (dsignal/dpsi = -i signal), dL/dm0 = lambda at the start.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import NamedTuple
+
+import numpy as np
+import torch
+from torch.autograd.function import once_differentiable
+
+from ..phantom import Phantom
+from ..result import SimResult
+from ..sequence import Sequence
+from . import driver, hip
+
+ADJOINT_COILS = (1, 2, 4, 8, 16)  # coils held in registers by the adjoint kernel (padded up)
+
+
+def default_checkpoint_every(n_segments: int) -> int:
+    """About sqrt(segments), a multiple of 64."""
+    return 64 * max(1, math.ceil(math.sqrt(max(n_segments, 1)) / 64))
+
+
+class Opts(NamedTuple):
+    is_rf: list
+    adc: list
+    spoil: list
+    every: int
+
+
+class GPUSimulate(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, opts: Opts, pos, pd, t1, t2, db0, coils_ri, dt, rf, grad, adc_phase, m0, b1, b0_offset):
+        ctx.set_materialize_grads(False)
+        d = lambda t: None if t is None else t.detach()
+        ph = Phantom(d(pos), d(pd), d(t1), d(t2), d(db0), b1=d(b1))
+        sq = Sequence(d(dt), d(rf), d(grad), d(adc_phase), opts.is_rf, opts.adc, opts.spoil, {}, d(b0_offset))
+        coils = torch.complex(coils_ri[..., 0].detach(), coils_ri[..., 1].detach())
+        case = driver.prepare_case(ph, sq, coils, d(m0), chunk=opts.every, aligned=True)
+        ckpts = driver.run_forward(case, keep=True)
+        hip.HIP.synchronize()
+        res = driver.read_result(case)
+        case.partial = None  # the forward's work buffer is not needed any more
+        ctx.opts, ctx.case, ctx.ckpts = opts, case, ckpts
+        ctx.sens_re, ctx.sens_im = coils.real.T.float().cpu(), coils.imag.T.float().cpu()  # (C, N)
+        ctx.like = [None if t is None else (t.dtype, t.device)
+                    for t in (pos, pd, t1, t2, db0, coils_ri, dt, rf, grad, adc_phase, m0, b1, b0_offset)]
+        sig = torch.view_as_real(res.signal.T.contiguous())  # (n_adc, C, 2)
+        ctx.save_for_backward(t1, t2, adc_phase, sig)
+        return sig, res.m
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, g_sig, g_m):
+        opts, case, ckpts = ctx.opts, ctx.case, ctx.ckpts
+        t1, t2, adc_phase, sig = ctx.saved_tensors
+        # pos, pd, t1, t2, db0, coils, dt, rf, grad, adc_phase, m0, b1, b0_offset
+        need = list(ctx.needs_input_grad[1:])
+        if g_sig is None:
+            need[5] = need[9] = False
+        if (g_sig is None and g_m is None) or not any(need):
+            return (None,) * 14
+        dev = case.spins.device
+        n, n_pad, C, S, every = case.n, case.n_pad, case.n_coils, len(opts.is_rf), opts.every
+        n_blocks = n_pad // driver.BLOCK
+        adc_idx = torch.tensor(opts.adc).nonzero()[:, 0]
+        nc = 0
+        if g_sig is not None and case.n_adc:
+            nc = next((c for c in ADJOINT_COILS if c >= C), None)
+            if nc is None:
+                raise NotImplementedError(f"hip engine: gradients with more than {ADJOINT_COILS[-1]} coils")
+        h = torch.zeros(max(case.n_adc, 1), max(nc, 1), 2, dtype=torch.float64)
+        if nc:
+            psi = adc_phase.detach().double().cpu()[adc_idx][:, None]
+            gr, gi = g_sig[..., 0].double().cpu(), g_sig[..., 1].double().cpu()
+            h[:, :C, 0] = gr * torch.cos(psi) - gi * torch.sin(psi)
+            h[:, :C, 1] = gr * torch.sin(psi) + gi * torch.cos(psi)
+        sens = torch.zeros(2 * max(nc, 1), n_pad, dtype=torch.float32)  # rows Re S_0, Im S_0, Re S_1, ...
+        if nc:
+            sens[0 : 2 * C : 2, :n] = ctx.sens_re
+            sens[1 : 2 * C : 2, :n] = ctx.sens_im
+        lam = torch.zeros(3, n_pad, dtype=torch.float32)
+        if g_m is not None:
+            lam[:, :n] = g_m.detach().float().cpu().T
+        need_seg = any(need[i] for i in (6, 7, 8, 12))
+        need_coil = bool(nc) and need[5]
+        gs = torch.zeros(8, n_pad, dtype=torch.float64, device=dev)
+        gc = torch.zeros(2 * max(nc, 1), n_pad, dtype=torch.float64, device=dev)
+        gseg = torch.zeros(n_blocks * every * 8 if need_seg else 1, dtype=torch.float32, device=dev)
+        seg = torch.zeros(S * 8 if need_seg else 1, dtype=torch.float64, device=dev)
+        r8 = torch.empty((every + 7) // 8 * 3 * n_pad, dtype=torch.float32, device=dev)
+        lam_d = lam.to(dev).contiguous()
+        h_d = h.float().to(dev).contiguous()
+        sens_d = sens.to(dev).contiguous()
+        lib = hip.HIP.lib()
+        cs = case.struct()
+        for c in reversed(range(len(ckpts))):
+            s0, s1 = case.starts[c], case.ends[c]
+            ad = hip.HbAdjoint(ckpts[c].data_ptr(), r8.data_ptr(), lam_d.data_ptr(), gs.data_ptr(), gc.data_ptr(),
+                               gseg.data_ptr(), h_d.data_ptr(), sens_d.data_ptr(), nc, int(need_seg), int(need_coil),
+                               every)
+            lib.adjoint_interval(ctypes.byref(cs), ctypes.byref(ad), s0, s1)
+            if need_seg:
+                lib.seg_reduce(gseg.data_ptr(), n_blocks, every, s0, s1 - s0, seg.data_ptr())
+        hip.HIP.synchronize()
+        GS = gs.cpu()[:, :n]
+        lam = lam_d.cpu().double()[:, :n]
+        out = [None] * 13
+        if need[0]:
+            out[0] = GS[0:3].T.contiguous()
+        if need[1]:
+            out[1] = GS[4]
+        if need[2]:
+            out[2] = -GS[5] / t1.detach().double().cpu() ** 2
+        if need[3]:
+            out[3] = -GS[6] / t2.detach().double().cpu() ** 2
+        if need[4]:
+            out[4] = GS[3]
+        if need_coil:
+            G = gc.cpu()[: 2 * C, :n]
+            out[5] = torch.stack([G[0::2].T, G[1::2].T], -1)
+        if need_seg:
+            sgr = seg.cpu().view(S, 8)
+            out[6], out[7], out[8], out[12] = sgr[:, 0], sgr[:, 1:3], sgr[:, 3:6], sgr[:, 6]
+        if need[9]:
+            g = torch.zeros(S, dtype=torch.float64)
+            sg_, gsig = sig.double(), g_sig.double().cpu()
+            g[adc_idx] = (gsig[..., 0] * sg_[..., 1] - gsig[..., 1] * sg_[..., 0]).sum(-1)
+            out[9] = g
+        if need[10]:
+            out[10] = lam.T.contiguous()
+        if need[11]:
+            out[11] = GS[7]
+        return (None, *[None if g is None or not need[i] else g.to(dtype=ctx.like[i][0], device=ctx.like[i][1])
+                        for i, g in enumerate(out)])
+
+
+def simulate_grad(phantom: Phantom, seq: Sequence, coils, m_init, checkpoint_every) -> SimResult:
+    """simulate(engine="gpu") with gradients: the inputs as tensors of ``GPUSimulate`` (adjoint.py:264-323)."""
+    n, pd = len(phantom), phantom.pd
+    if coils is None:
+        coils = torch.ones(n, 1, dtype=torch.complex64 if pd.dtype == torch.float32 else torch.complex128)
+    if m_init is None:
+        m0 = torch.stack([torch.zeros_like(pd), torch.zeros_like(pd), pd], -1)
+    else:
+        m0 = m_init[:, :3]
+    every = checkpoint_every or default_checkpoint_every(len(seq))
+    if every % 64:
+        raise ValueError(f"checkpoint_every must be a multiple of 64, got {every}")
+    opts = Opts(list(seq.is_rf), list(seq.adc), list(seq.spoil), every)
+    sig, m = GPUSimulate.apply(opts, phantom.pos, pd, phantom.t1, phantom.t2, phantom.db0, torch.view_as_real(coils),
+                               seq.dt, seq.rf, seq.grad, seq.adc_phase, m0, phantom.b1, seq.b0_offset)
+    return SimResult(torch.view_as_complex(sig.contiguous()).T, m)

@@ -29,7 +29,7 @@ from ..sequence import Sequence
 from . import hip
 
 BLOCK = 256
-RF, ADC, SPOIL, SAME_FREE, SAME_RF_DT = 1, 2, 4, 8, 16
+RF, ADC, SPOIL, SAME_FREE, SAME_RF_DT, NEXT_FREE, NEXT_RF_DT = 1, 2, 4, 8, 16, 32, 64
 SEG_DTYPE = np.dtype([("f", "<f4", (8,)), ("d", "<f8", (5,)), ("flags", "<i4"), ("pad", "<i4", (5,))])
 assert SEG_DTYPE.itemsize == 96
 PARTIAL_BYTES = 1 << 30  # workgroup partial sums of one dispatch at most (one buffer, reused)
@@ -62,13 +62,15 @@ def segment_table(seq: Sequence, starts: list[int]) -> np.ndarray:
     first[np.asarray(starts, dtype=np.int64)] = True
     chunk_id = np.cumsum(first) - 1
     key = np.stack([dt, g[:, 0], g[:, 1], g[:, 2], b0], 1)
-    for kind, bit, cols in ((~is_rf, SAME_FREE, slice(0, 5)), (is_rf, SAME_RF_DT, slice(0, 1))):
+    for kind, bit, nxt, cols in ((~is_rf, SAME_FREE, NEXT_FREE, slice(0, 5)), (is_rf, SAME_RF_DT, NEXT_RF_DT, slice(0, 1))):
         idx = np.flatnonzero(kind)
         if len(idx) > 1:
             prev, cur = idx[:-1], idx[1:]
             same = _close(key[cur, cols], key[prev, cols]).all(1) & (chunk_id[cur] == chunk_id[prev])
-            flags[cur[same]] |= bit
+            flags[cur[same]] |= bit  # forward order: equal to the previous one of the kind
+            flags[prev[same]] |= nxt  # reverse order: equal to the next one of the kind
     rec["flags"] = flags
+    rec["pad"][:, 0] = np.where(adc, np.cumsum(adc) - 1, -1)  # ADC row
     return rec
 
 
@@ -145,8 +147,10 @@ def check_inputs(phantom: Phantom, coils) -> None:
 
 
 def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, chunk: int = DEFAULT_CHUNK,
-                 device=None, partial_bytes: int = PARTIAL_BYTES) -> DeviceCase:
-    """Upload a case: the buffers of the module docstring, the dispatch plan and the work buffers."""
+                 device=None, partial_bytes: int = PARTIAL_BYTES, aligned: bool = False) -> DeviceCase:
+    """Upload a case: the buffers of the module docstring, the dispatch plan and the work buffers. ``aligned``:
+    dispatches are exactly [c chunk, (c + 1) chunk) (the adjoint's checkpoint intervals), the partial buffer sized
+    for the largest one."""
     check_inputs(phantom, coils)
     dev = device if device is not None else hip.HIP.device
     n = len(phantom)
@@ -177,7 +181,7 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
     n_blocks = n_pad // BLOCK
     vals = 16 * n_tiles
     rows_cap = max(8, (partial_bytes // (4 * vals * n_blocks)) // 8 * 8)
-    starts = dispatch_starts(seq, int(chunk), rows_cap)
+    starts = list(range(0, len(seq), int(chunk))) if aligned else dispatch_starts(seq, int(chunk), rows_cap)
     ends = starts[1:] + [len(seq)]
     adc_cum = np.concatenate([[0], np.cumsum(np.asarray(seq.adc, dtype=np.int64))])
     row_starts = [int(adc_cum[s]) for s in starts]

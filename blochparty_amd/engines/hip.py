@@ -15,7 +15,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent.parent / "libblochparty_amd.so"
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_COILS = 128  # 16 tiles of 8 coils (the forward kernel's LDS budget)
 
 
@@ -36,6 +36,25 @@ class HbCase(ctypes.Structure):
     ]
 
 
+class HbAdjoint(ctypes.Structure):
+    """``hb_adjoint`` of the header."""
+
+    _fields_ = [
+        ("ckpt", ctypes.c_void_p),
+        ("r8", ctypes.c_void_p),
+        ("lam", ctypes.c_void_p),
+        ("gs", ctypes.c_void_p),
+        ("gc", ctypes.c_void_p),
+        ("gseg", ctypes.c_void_p),
+        ("h", ctypes.c_void_p),
+        ("sens", ctypes.c_void_p),
+        ("nc", ctypes.c_int32),
+        ("seg_grads", ctypes.c_int32),
+        ("coil_grads", ctypes.c_int32),
+        ("k_max", ctypes.c_int32),
+    ]
+
+
 _LIB: ctypes.CDLL | None = None
 
 # (name, argtypes): every entry point returns int (a hipError_t)
@@ -43,6 +62,8 @@ _P, _I, _V = ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p
 SIGNATURES = {
     "hb_forward": [ctypes.POINTER(HbCase), _I, _I, _P, _I, _V],
     "hb_signal_reduce": [_P, _I, _I, _I, _I, _P, _I, _P, _V],
+    "hb_adjoint_interval": [ctypes.POINTER(HbCase), ctypes.POINTER(HbAdjoint), _I, _I, _V],
+    "hb_seg_reduce": [_P, _I, _I, _I, _I, _P, _V],
 }
 
 

@@ -25,11 +25,13 @@ extern "C" {
 #define HB_SPOIL 4        /* zero the transverse magnetization at the end of the segment                  */
 #define HB_SAME_FREE 8    /* free segment equal (dt, gradient, b0_offset) to the previous free one here    */
 #define HB_SAME_RF_DT 16  /* RF segment with the previous RF segment's dt (reuse its relaxation factors)    */
+#define HB_NEXT_FREE 32   /* free segment equal to the next free one in the interval (reverse sweep cache)   */
+#define HB_NEXT_RF_DT 64  /* RF segment with the next RF segment's dt in the interval (reverse sweep cache)   */
 
-/* One segment of the event list, 96 bytes (the reference's segs row + flags, driver.py:357-384). psi is the
+/* One segment of the event list, 96 bytes (the reference's segs row + flags + rows, driver.py:357-384). psi is the
  * receiver phase wrapped to [0, 2 pi) in float64 before the cast (driver.py:357). The float64 block carries the
  * duration, the gradient moment g dt and the b0_offset area, from which free segments take their phase in float64
- * (k . r reduced mod 1 cycle per segment). */
+ * (k . r reduced mod 1 cycle per segment). pad[0] is the ADC row of an ADC segment (the reference's adc_row). */
 typedef struct hb_segment {
     float dt, rf_x, rf_y, g_x, g_y, g_z, b0, psi;
     double dt64, m_x, m_y, m_z, b0_area;
@@ -61,6 +63,33 @@ int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* partial, int32_
  * order. psi: (n_adc,) float64. */
 int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t rows, int32_t n_tiles, int32_t n_coils,
                      const double* psi, int32_t row0, double* sig, void* stream);
+
+/* One interval of the adjoint (replaces bloch_replay + bloch_adjoint_c{NC}, cuda/replay.cu.in:3-33,
+ * cuda/adjoint.cu.in:5-121; one call per interval of adjoint.py:179-220, last interval first). ckpt: (6, n_pad)
+ * the state before the interval (the forward's, hi + lo); r8: scratch of (ceil(k_max / 8), 3, n_pad) floats;
+ * lam: (3, n_pad) dL/dm after the interval in, before it out; gs: (8, n_pad) float64 accumulators of dL/d(px, py, pz,
+ * db0, pd, r1, r2, b1); gc: (2 nc, n_pad) float64 accumulators of dL/d(Re S_c, Im S_c) (when coil_grads); gseg:
+ * (n_pad / 256, k_max, 8) per-workgroup sums of dL/d(dt, rf_x, rf_y, g_x, g_y, g_z, b0_offset, -) of the interval's
+ * segments (when seg_grads); h: (n_adc, nc, 2) dL/d(signal) turned by exp(+i psi); sens: (2 nc, n_pad) Re S_c,
+ * Im S_c. nc in {0, 1, 2, 4, 8, 16} (coils padded with zeros). The interval length is at most k_max. */
+typedef struct hb_adjoint {
+    const float* ckpt;
+    float* r8;
+    float* lam;
+    double* gs;
+    double* gc;
+    float* gseg;
+    const float* h;
+    const float* sens;
+    int32_t nc, seg_grads, coil_grads, k_max;
+} hb_adjoint;
+
+int hb_adjoint_interval(const hb_case* cs, const hb_adjoint* ad, int32_t s0, int32_t s1, void* stream);
+
+/* In-order sum of an interval's per-workgroup segment gradients (replaces seg_reduce, cuda/seg_reduce.cu:3-11):
+ * seg[(s0 + k) * 8 + g] = sum over b < n_blocks of gseg[(b * k_max + k) * 8 + g], float64, for k < len. */
+int hb_seg_reduce(const float* gseg, int32_t n_blocks, int32_t k_max, int32_t s0, int32_t len, double* seg,
+                  void* stream);
 
 /* Name of a return code. */
 const char* hb_error_string(int code);
