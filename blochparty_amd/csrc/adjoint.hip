@@ -41,24 +41,23 @@ __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint
     const int j = blockIdx.x * kBlockA + tid;
     const int np = cs.n_pad;
     const Spin p = load_spin(cs.spins, cs.pos_lo, np, j);
+    const float b1sq = p.b1 * p.b1;
 
     // 1. replay the interval, r8[b] = state before segment s0 + 8 b (low parts folded)
     {
-        State m;
-        m.mx = ad.ckpt[j], m.my = ad.ckpt[np + j], m.mz = ad.ckpt[2 * np + j];
-        m.lx = ad.ckpt[3 * np + j], m.ly = ad.ckpt[4 * np + j], m.lz = ad.ckpt[5 * np + j];
+        State m = load_state(ad.ckpt, np, j);
         Cache k = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         bool free_ok = false, rf_ok = false;
         for (int s = s0; s < s1; ++s) {
             if (((s - s0) & 7) == 0) {
                 float* r = ad.r8 + (size_t)((s - s0) >> 3) * 3 * np;
-                r[j] = m.mx + m.lx, r[np + j] = m.my + m.ly, r[2 * np + j] = m.mz + m.lz;
+                r[j] = m.mx, r[np + j] = m.my, r[2 * np + j] = m.mz + m.lz;
             }
             const hb_segment q = cs.segs[s];
             const int f = live_flags(q.flags, free_ok, rf_ok, HB_SAME_FREE, HB_SAME_RF_DT);
-            step(q, f, p, m, k);
+            step(q, f, p, b1sq, m, k);
             if (f & HB_RF) rf_ok = true; else free_ok = true;
-            if (f & HB_SPOIL) m.mx = m.my = m.lx = m.ly = 0.0f;
+            if (f & HB_SPOIL) m.mx = m.my = 0.0f;
         }
     }
 
@@ -86,20 +85,19 @@ __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint
         {
             const float* r = ad.r8 + (size_t)b * 3 * np;
             State m;
-            m.mx = r[j], m.my = r[np + j], m.mz = r[2 * np + j];
-            m.lx = m.ly = m.lz = 0.0f;
+            m.mx = r[j], m.my = r[np + j], m.mz = r[2 * np + j], m.lz = 0.0f;
             Cache k = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             bool free_ok = false, rf_ok = false;
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 if (i < nseg) {
-                    before[i] = {m.mx + m.lx, m.my + m.ly, m.mz + m.lz};
+                    before[i] = {m.mx, m.my, m.mz + m.lz};
                     const hb_segment q = cs.segs[sb + i];
                     const int f = live_flags(q.flags, free_ok, rf_ok, HB_SAME_FREE, HB_SAME_RF_DT);
-                    step(q, f, p, m, k);
+                    step(q, f, p, b1sq, m, k);
                     if (f & HB_RF) rf_ok = true; else free_ok = true;
-                    after[i] = {m.mx + m.lx, m.my + m.ly};
-                    if (f & HB_SPOIL) m.mx = m.my = m.lx = m.ly = 0.0f;
+                    after[i] = {m.mx, m.my};
+                    if (f & HB_SPOIL) m.mx = m.my = 0.0f;
                 }
             }
         }

@@ -30,19 +30,8 @@ struct RevCache {
 };
 
 __device__ __forceinline__ void rot_factors(float p2, float& c, float& s1, float& s2, float& f1, float& f2) {
-    if (p2 < 1e-6f) {
-        c = 1.0f - 0.5f * p2;
-        s1 = 1.0f - p2 * (1.0f / 6.0f);
-        s2 = 0.5f - p2 * (1.0f / 24.0f);
-    } else {
-        const float phi = sqrtf(p2);
-        float sh, ch;
-        sincosf(0.5f * phi, &sh, &ch);
-        const float sg = sh / phi;
-        c = 1.0f - 2.0f * sh * sh;
-        s1 = 2.0f * ch * sg;
-        s2 = 2.0f * sg * sg;
-    }
+    rot_ab(p2, s1, s2);
+    c = 1.0f - s2 * p2;
     if (p2 < 1.0f) {
         f1 = -1.0f / 3.0f + p2 * (1.0f / 30.0f + p2 * (-1.0f / 840.0f + p2 * (1.0f / 45360.0f + p2 * (-1.0f / 3991680.0f))));
         f2 = -1.0f / 12.0f + p2 * (1.0f / 180.0f + p2 * (-1.0f / 6720.0f + p2 * (1.0f / 453600.0f + p2 * (-1.0f / 47900160.0f))));

@@ -1,12 +1,13 @@
 // Device-side physics shared by the forward, replay and adjoint kernels: one isochromat per lane, state in
-// registers, the segment record read through the scalar cache (every lane of a wave steps the same segment).
+// registers; every lane of a wave steps the same segment, so control flow never diverges.
 //
 // Physics follows the reference's core (src/blochparty/engines/core.py:71-150, 268-319): RF segments by Strang
-// splitting (relax dt/2, rotate about w = -2 pi dt (b1 rf_x, b1 rf_y, bz), relax dt/2) with the increments added by
-// two-sum into a low part, as the reference's GPU step (shaders/common.metal:18-23, 83-103); free segments exactly
-// (precession about z, relaxation). The rotation is written in half-angle (Cayley-Klein / spinor) form: with
-// phi = |w|, h = phi / 2, the increment is a (w x m) + b w x (w x m), a = sin(phi)/phi = 2 cos(h) sin(h)/phi,
-// b = (1 - cos phi)/phi^2 = 2 (sin(h)/phi)^2, from a single sincos of h.
+// splitting (relax dt/2, rotate about w = -2 pi dt (b1 rf_x, b1 rf_y, bz), relax dt/2), Mz's increments added by
+// two-sum into a low part (as the reference's GPU step keeps low parts, shaders/common.metal:18-23, 83-103); free
+// segments exactly (precession about z, relaxation). A rotation by w is applied as the increment
+// a (w x m) + b w x (w x m) with a = sin(phi)/phi and b = (1 - cos phi)/phi^2, phi = |w|: even functions of phi, so
+// polynomials in phi^2 for the small angles of shaped-pulse substeps, and the half-angle (Cayley-Klein) form
+// a = 2 cos(h) sin(h)/phi, b = 2 (sin(h)/phi)^2, h = phi / 2, from one sincos, for larger ones.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -69,56 +70,72 @@ struct Cache {
     float h1, h2;         // RF: expm1 of the relaxation over dt / 2
 };
 
-// m += a (w x m) + b w x (w x m), each component by two-sum.
-__device__ __forceinline__ void rotate_inc(float wx, float wy, float wz, float& mx, float& my, float& mz, float& lx,
-                                           float& ly, float& lz) {
-    const float phi2 = wx * wx + wy * wy + wz * wz;
-    float a, b;
-    if (phi2 < 1e-6f) {  // phi < 1e-3: series to phi^4 / 120
-        a = 1.0f - phi2 * (1.0f / 6.0f);
-        b = 0.5f - phi2 * (1.0f / 24.0f);
+// a = sin(phi)/phi and b = (1 - cos(phi))/phi^2 of a rotation by phi, from p2 = phi^2. Below phi = 1/2 their
+// Taylor series to p2^4 (the next terms are under 3e-11): no square root, sincos or division on the hot path.
+__device__ __forceinline__ void rot_ab(float p2, float& a, float& b) {
+    if (p2 < 0.25f) {
+        a = 1.0f + p2 * (-1.0f / 6.0f + p2 * (1.0f / 120.0f + p2 * (-1.0f / 5040.0f + p2 * (1.0f / 362880.0f))));
+        b = 0.5f + p2 * (-1.0f / 24.0f + p2 * (1.0f / 720.0f + p2 * (-1.0f / 40320.0f + p2 * (1.0f / 3628800.0f))));
     } else {
-        const float phi = sqrtf(phi2);
+        const float phi = sqrtf(p2);
         float sh, ch;
         sincosf(0.5f * phi, &sh, &ch);
         const float sg = sh / phi;
         a = 2.0f * ch * sg;
         b = 2.0f * sg * sg;
     }
-    const float cx = wy * mz - wz * my, cy = wz * mx - wx * mz, cz = wx * my - wy * mx;
-    const float dx = wy * cz - wz * cy, dy = wz * cx - wx * cz, dz = wx * cy - wy * cx;
-    two_sum(mx, lx, a * cx + b * dx);
-    two_sum(my, ly, a * cy + b * dy);
-    two_sum(mz, lz, a * cz + b * dz);
 }
 
 struct State {
-    float mx, my, mz, lx, ly, lz;
+    float mx, my, mz, lz;  // lz: the two-sum low part of mz
 };
 
+// The state from a (6, n_pad) buffer (mx my mz and their low parts): the transverse low parts are folded in.
+__device__ __forceinline__ State load_state(const float* __restrict__ st, int n_pad, int j) {
+    State m;
+    m.mx = st[j] + st[3 * n_pad + j];
+    m.my = st[n_pad + j] + st[4 * n_pad + j];
+    m.mz = st[2 * n_pad + j];
+    m.lz = st[5 * n_pad + j];
+    return m;
+}
+
+__device__ __forceinline__ void store_state(float* __restrict__ st, int n_pad, int j, const State& m) {
+    st[j] = m.mx, st[n_pad + j] = m.my, st[2 * n_pad + j] = m.mz;
+    st[3 * n_pad + j] = 0.0f, st[4 * n_pad + j] = 0.0f, st[5 * n_pad + j] = m.lz;
+}
+
 // One segment (not including the spoil).
-__device__ __forceinline__ void step(const hb_segment& q, int f, const Spin& p, State& m, Cache& k) {
+//
+// RF: the three Strang increments (relax dt/2, rotate, relax dt/2) are formed from the running state in float32 and
+// summed, then added to the state; Mz's sum goes through two-sum. The rotation vector is (b1 w_rf, w_g . r + w db0 +
+// w_b0) from the record's host-formed factors, and phi^2 = b1^2 |w_rf|^2 + wz^2.
+// Free: Mxy is rotated by the cached cos / sin and decayed as n + n e2 (e2 = expm1: a float32 factor 1 + e2 would
+// lose e2's low bits, and the same factor repeats over a readout, so its rounding would add up).
+// Mz keeps a low part through both: near equilibrium a segment adds an increment far below one ulp of Mz, and
+// rounding Mz to float32 after every segment biases the steady state (2e-5 of the signal of a 264-TR spoiled GRE,
+// against 5e-7 with two-sum); the transverse components are rotated and spoiled, so their rounding does not build up
+// (a 5,000-substep CW saturation stays within 1e-6 of float64 without it).
+__device__ __forceinline__ void step(const hb_segment& q, int f, const Spin& p, float b1sq, State& m, Cache& k) {
     if (f & HB_RF) {
         if (!(f & HB_SAME_RF_DT)) {
             k.h1 = expm1f(-0.5f * q.dt * p.r1);
             k.h2 = expm1f(-0.5f * q.dt * p.r2);
         }
-        two_sum(m.mx, m.lx, m.mx * k.h2);
-        two_sum(m.my, m.ly, m.my * k.h2);
-        two_sum(m.mz, m.lz, (m.mz - p.pd) * k.h1);
-        const float w = -HB_TWO_PI * q.dt;
-        const float wb = w * p.b1;
-        rotate_inc(wb * q.rf_x, wb * q.rf_y, w * offset(p, q), m.mx, m.my, m.mz, m.lx, m.ly, m.lz);
-        two_sum(m.mx, m.lx, m.mx * k.h2);
-        two_sum(m.my, m.ly, m.my * k.h2);
-        two_sum(m.mz, m.lz, (m.mz - p.pd) * k.h1);
+        const float dx1 = m.mx * k.h2, dy1 = m.my * k.h2, dz1 = (m.mz - p.pd) * k.h1;
+        const float x1 = m.mx + dx1, y1 = m.my + dy1, z1 = m.mz + dz1;
+        const float wx = p.b1 * q.w_rf_x, wy = p.b1 * q.w_rf_y;
+        const float wz = p.px * q.w_g_x + p.py * q.w_g_y + p.pz * q.w_g_z + (p.db0 * q.w + q.w_b0);
+        float a, b;
+        rot_ab(b1sq * q.w_rf2 + wz * wz, a, b);
+        const float cx = wy * z1 - wz * y1, cy = wz * x1 - wx * z1, cz = wx * y1 - wy * x1;
+        const float ex = wy * cz - wz * cy, ey = wz * cx - wx * cz, ez = wx * cy - wy * cx;
+        const float drx = a * cx + b * ex, dry = a * cy + b * ey, drz = a * cz + b * ez;
+        const float x2 = x1 + drx, y2 = y1 + dry, z2 = z1 + drz;
+        m.mx += dx1 + drx + x2 * k.h2;
+        m.my += dy1 + dry + y2 * k.h2;
+        two_sum(m.mz, m.lz, dz1 + drz + (z2 - p.pd) * k.h1);
     } else {
-        // Mxy is rotated: fold its low parts. Mz keeps its low part: near equilibrium a free segment adds an
-        // increment far below one ulp of Mz, and rounding Mz to float32 after every segment biases the steady state
-        // (2e-5 of the signal of a 264-TR spoiled GRE), where two-sum keeps it at 5e-7.
-        m.mx += m.lx;
-        m.my += m.ly;
-        m.lx = m.ly = 0.0f;
         if (!(f & HB_SAME_FREE)) {
             sincospif(-2.0f * free_cycles(p, q), &k.s, &k.c);
             k.e1 = expm1f(-q.dt * p.r1);

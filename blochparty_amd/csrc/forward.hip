@@ -90,9 +90,8 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(hb_case cs, int s0, int
     float* stg = lds + wave * kStgFloats;
     float* wred = lds + kWaves * kStgFloats;
     const Spin p = load_spin(cs.spins, cs.pos_lo, np, j);
-    State m;
-    m.mx = cs.state[j], m.my = cs.state[np + j], m.mz = cs.state[2 * np + j];
-    m.lx = cs.state[3 * np + j], m.ly = cs.state[4 * np + j], m.lz = cs.state[5 * np + j];
+    const float b1sq = p.b1 * p.b1;
+    State m = load_state(cs.state, np, j);
     const float* sa = cs.sens_a + (size_t)(blockIdx.x * kWaves + wave) * nt * 1024 + lane;
     float A[NT > 0 ? NT : 1][16];
     if constexpr (NT > 0) {
@@ -106,21 +105,20 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(hb_case cs, int s0, int
     for (int s = s0; s < s1; ++s) {
         const hb_segment q = cs.segs[s];
         const int f = q.flags;
-        step(q, f, p, m, k);
+        step(q, f, p, b1sq, m, k);
         if (f & HB_ADC) {
-            stg[nb * kStgStride + lane] = m.mx + m.lx;
-            stg[(8 + nb) * kStgStride + lane] = m.my + m.ly;
+            stg[nb * kStgStride + lane] = m.mx;
+            stg[(8 + nb) * kStgStride + lane] = m.my;
             if (++nb == 8) {
                 flush<NT>(stg, wred, A, sa, nt, 8, row, rows, partial);
                 row += 8;
                 nb = 0;
             }
         }
-        if (f & HB_SPOIL) m.mx = m.my = m.lx = m.ly = 0.0f;
+        if (f & HB_SPOIL) m.mx = m.my = 0.0f;
     }
     if (nb > 0) flush<NT>(stg, wred, A, sa, nt, nb, row, rows, partial);
-    cs.state[j] = m.mx, cs.state[np + j] = m.my, cs.state[2 * np + j] = m.mz;
-    cs.state[3 * np + j] = m.lx, cs.state[4 * np + j] = m.ly, cs.state[5 * np + j] = m.lz;
+    store_state(cs.state, np, j, m);
 }
 
 // sig[row0 + r][c] = exp(-i psi) sum_b partial[b][r][c], float64, blocks in order.
@@ -180,4 +178,4 @@ extern "C" int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t 
 
 extern "C" const char* hb_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 
-extern "C" int hb_abi_version(void) { return 3; }
+extern "C" int hb_abi_version(void) { return 4; }

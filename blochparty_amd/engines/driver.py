@@ -5,9 +5,9 @@ the HIP kernels' layouts:
 - spins as float32 structure-of-arrays (8, n_pad) ``px py pz pd r1 r2 db0 b1`` (``driver.py:320-335``) plus the
   float64 positions' low parts (3, n_pad), for the float64 k.r phase of free segments;
 - sensitivities in the matrix-core operand order (n_pad / 64, tiles, 16, 64) (see ``csrc/forward.hip``);
-- segments as 96-byte records: float32 dt, rf, g, b0_offset, psi (wrapped mod 2 pi in float64 first,
+- segments as 128-byte records: float32 dt, rf, g, b0_offset, psi (wrapped mod 2 pi in float64 first,
   ``driver.py:357``), float64 dt, moment g dt and b0_offset dt, flags (RF 1, ADC 2, spoil 4, ``driver.py:375-382``,
-  and two cache hints);
+  and cache hints), and the rotation-vector factors -2 pi dt (1, rf, g, b0_offset) formed in float64;
 - the state as float32 hi + lo, split from float64 to ~48 bits (``driver.py:386-433``).
 
 The signal is summed per workgroup into partial rows and reduced across workgroups in float64 (``hb_signal_reduce``),
@@ -30,8 +30,9 @@ from . import hip
 
 BLOCK = 256
 RF, ADC, SPOIL, SAME_FREE, SAME_RF_DT, NEXT_FREE, NEXT_RF_DT = 1, 2, 4, 8, 16, 32, 64
-SEG_DTYPE = np.dtype([("f", "<f4", (8,)), ("d", "<f8", (5,)), ("flags", "<i4"), ("pad", "<i4", (5,))])
-assert SEG_DTYPE.itemsize == 96
+SEG_DTYPE = np.dtype([("f", "<f4", (8,)), ("d", "<f8", (5,)), ("flags", "<i4"), ("pad", "<i4", (5,)),
+                      ("w", "<f4", (8,))])
+assert SEG_DTYPE.itemsize == 128
 PARTIAL_BYTES = 1 << 30  # workgroup partial sums of one dispatch at most (one buffer, reused)
 DEFAULT_CHUNK = 32768  # segments per dispatch at most (the reference's default ``chunk``)
 SAME_TOL = 1e-12  # relative: segments this close share their precession / relaxation factors
@@ -70,6 +71,9 @@ def segment_table(seq: Sequence, starts: list[int]) -> np.ndarray:
             flags[cur[same]] |= bit  # forward order: equal to the previous one of the kind
             flags[prev[same]] |= nxt  # reverse order: equal to the next one of the kind
     rec["flags"] = flags
+    w = -2 * math.pi * dt  # the rotation-vector factors, formed in float64
+    rec["w"] = np.stack([w, w * rf[:, 0], w * rf[:, 1], w * g[:, 0], w * g[:, 1], w * g[:, 2], w * b0,
+                         (w * rf[:, 0]) ** 2 + (w * rf[:, 1]) ** 2], 1).astype(np.float32)
     rec["pad"][:, 0] = np.where(adc, np.cumsum(adc) - 1, -1)  # ADC row
     return rec
 

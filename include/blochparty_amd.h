@@ -28,21 +28,24 @@ extern "C" {
 #define HB_NEXT_FREE 32   /* free segment equal to the next free one in the interval (reverse sweep cache)   */
 #define HB_NEXT_RF_DT 64  /* RF segment with the next RF segment's dt in the interval (reverse sweep cache)   */
 
-/* One segment of the event list, 96 bytes (the reference's segs row + flags + rows, driver.py:357-384). psi is the
+/* One segment of the event list, 128 bytes (the reference's segs row + flags + rows, driver.py:357-384). psi is the
  * receiver phase wrapped to [0, 2 pi) in float64 before the cast (driver.py:357). The float64 block carries the
  * duration, the gradient moment g dt and the b0_offset area, from which free segments take their phase in float64
- * (k . r reduced mod 1 cycle per segment). pad[0] is the ADC row of an ADC segment (the reference's adc_row). */
+ * (k . r reduced mod 1 cycle per segment). The w block holds the segment's rotation-vector factors, formed in float64
+ * on the host: w = -2 pi dt, w_rf = w rf, w_g = w g, w_b0 = w b0_offset, w_rf2 = |w_rf|^2 (a spin's RF rotation
+ * vector is (b1 w_rf, w_g . r + w db0 + w_b0)). pad[0] is the ADC row of an ADC segment (the reference's adc_row). */
 typedef struct hb_segment {
     float dt, rf_x, rf_y, g_x, g_y, g_z, b0, psi;
     double dt64, m_x, m_y, m_z, b0_area;
     int32_t flags, pad[5];
+    float w, w_rf_x, w_rf_y, w_g_x, w_g_y, w_g_z, w_b0, w_rf2;
 } hb_segment;
 
 /* Device buffers of one case (driver.py:320-433). spins: (8, n_pad) px py pz pd r1 r2 db0 b1; pos_lo: (3, n_pad)
  * the float64 positions' remainders (pos = hi + lo) for the float64 k.r phase of free segments; sens_a: coil
  * sensitivities in the matrix-core operand order, (n_pad / 64, n_tiles, 16, 64) floats, n_tiles = ceil(C / 8), rows
- * of a tile Re S_0..7 then Im S_0..7. state: (6, n_pad) mx my mz and their two-sum low parts (the reference's M and
- * L buffers). */
+ * of a tile Re S_0..7 then Im S_0..7. state: (6, n_pad) mx my mz and their low parts (the reference's M and L
+ * buffers); the kernels carry a two-sum low part for mz only and fold the transverse ones in when they load. */
 typedef struct hb_case {
     const float* spins;
     const float* pos_lo;

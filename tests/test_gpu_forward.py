@@ -113,3 +113,17 @@ def test_sinc_slab_3d_subset_matches_oracle(gpu):
     e = rel(out.signal, ref_s)
     print(f"sinc slab 8^3 x4 rel signal {e:.3e} m {rel(out.m, ref_m):.3e}", flush=True)
     assert e < 1e-5 and rel(out.m, ref_m) < 1e-5
+
+
+def test_long_rf_trains_do_not_drift(gpu):
+    """CW saturation as 5,000 RF substeps of 10 us over +-2 kHz offsets: float32 rounding must not build up
+    (the reference's tests/engines/test_gpu.py:307-334 bound: 1e-5 per spin)."""
+    n = 32
+    one = torch.ones(n, dtype=torch.float64)
+    ph = bp.Phantom(torch.zeros(n, 3, dtype=torch.float64), one, one, 0.05 * one,
+                    -torch.linspace(-2000, 2000, n, dtype=torch.float64))
+    seq = bp.SequenceBuilder().rf(5000 * 10e-6, amplitude=42.577 * 1.5, n_steps=5000).readout(4, 10e-6).build()
+    ref_s, ref_m = core.simulate(ph, seq)
+    out = bp.simulate(ph, seq)
+    assert float((out.m.double() - ref_m).abs().max()) < 1e-5
+    assert float((out.signal.to(ref_s.dtype) - ref_s).abs().max()) < 1e-5 * n

@@ -36,8 +36,9 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_segment_record_layout():
-    assert driver.SEG_DTYPE.itemsize == 96
+    assert driver.SEG_DTYPE.itemsize == 128
     assert driver.SEG_DTYPE.fields["d"][1] == 32 and driver.SEG_DTYPE.fields["flags"][1] == 72
+    assert driver.SEG_DTYPE.fields["w"][1] == 96
 
 
 def mfma_16x16x4(a_lanes, b_lanes):
@@ -105,6 +106,9 @@ def test_case_packing_splits_float64_to_48_bits():
     case = driver.prepare_case(ph, busy_sequence(), None, m0, device="cpu")
     st = case.state.double()
     assert ((st[:3, :300] + st[3:, :300]) - m0.T).abs().max() < 2.0**-46 * m0.abs().max()
+    rec = case.segs.numpy().view(driver.SEG_DTYPE)
+    dt = busy_sequence().dt.numpy()
+    np.testing.assert_allclose(rec["w"][:, 0], -2 * np.pi * dt, rtol=1e-7)
     pos = case.spins[:3, :300].double() + case.pos_lo[:, :300].double()
     assert (pos - ph.pos.T).abs().max() < 2.0**-46 * ph.pos.abs().max()
     assert torch.all(case.spins[3, 300:] == 0) and torch.all(case.state[:, 300:] == 0)
