@@ -1,22 +1,32 @@
 """Benchmark of the GPU path on one MI355X: spin-samples/s of simulate(..., engine="gpu") on the 3D GRE 64^3 workload.
 
-    python bench.py [--steps K] [--warmup W] [--n 64] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--n 64] [--no-cpu] [--no-adjoint]
 
-One step = one forward pass over the workload with every input resident in HBM (uploaded before the timed region);
-the throughput is spins x ADC samples x steps / seconds (the reference's metric, benchmarks/bench.py:65). The timed
-region is bracketed by torch.cuda.synchronize(). Inside it, every forward-kernel launch is bracketed by HIP events on
-the stream it runs on, for the roofline of that kernel (algorithmic flops per launch over its mean duration).
+Workload: 3D GRE n^3 x 4 sub-voxel isochromats (1,048,576 spins at n = 64), a three-class smooth random phantom,
++-100 Hz dB0, 8 coils, a 2 ms sinc slab pulse (TBW 4, 200 substeps) every TR of 8 ms, n^2 lines of n samples.
+Every input is resident in HBM before a timed region starts (host packing and upload are outside it); each timed
+region is bracketed by torch.cuda.synchronize(). Throughput = spins x ADC samples x steps / seconds (the reference's
+metric, benchmarks/bench.py:65).
+
+Three legs, K timed steps each after W warmup steps:
+- forward with reused pulses (pulses="reuse": the sinc pulse composed once per spin into its affine map, in float64,
+  inside the timed step) -- the JSON line's ``value``;
+- forward stepping every RF substep (the exact-RF baseline);
+- forward + adjoint of L = sum |signal|^2 w.r.t. the flip angle (through the per-segment RF gradients), the T1 map
+  and the coil maps (the reference's --grad case, benchmarks/bench.py:69-95), exact RF: the forward with checkpoints,
+  dL/dsignal on the device, and the reverse sweep.
+Every forward-kernel launch of the value's leg is bracketed by HIP events on the stream it runs on; the roofline's
+``achieved`` is the algorithmic flops per launch over their mean duration.
 
 The CPU baseline is the FP64 oracle timed on this host's cores (the container quota) on a bounded sample of the same
-workload: a seeded subset of spins over the first TRs; the GPU runs the same sample and reports its relative error
-against the oracle. One JSON line goes to stdout.
+workload (a seeded subset of spins over the first TRs); the GPU runs the same sample, both RF paths, and reports their
+relative errors against the oracle. One JSON line goes to stdout; progress goes to stderr.
 """
 
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -25,13 +35,17 @@ import numpy as np
 import torch
 
 METRIC = "spin·samples/s fwd & fwd+adjoint, 3D GRE 64³; RF surrogate vs exact @1e-4"
-FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (published)
-MFMA_FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix (published)
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, published
 
 # counting convention of the algorithmic work (per spin)
 FLOPS_RF = 48  # exact RF substep
 FLOPS_FREE = 12  # free or ADC event
-FLOPS_COIL = 8  # per coil, per spin-sample (complex MAC)
+FLOPS_COIL = 8  # per coil, per spin-sample (complex multiply-add)
+FLOPS_PULSE = 30  # a reused pulse's map applied at one occurrence
+
+
+def log(msg: str) -> None:
+    print(f"# {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_quota() -> int:
@@ -46,22 +60,42 @@ def cpu_quota() -> int:
     return max(1, n)
 
 
-def dispatch_flops(case, seq_flags: np.ndarray) -> list[float]:
-    """Algorithmic flops of every forward dispatch of ``case`` (counting convention above)."""
-    rf = (seq_flags & 1) != 0
-    adc = (seq_flags & 2) != 0
+def dispatch_flops(case, driver) -> list[float]:
+    """Algorithmic flops of every forward dispatch of ``case``: stepped segments by the counting convention, reused
+    pulses at FLOPS_PULSE per occurrence."""
+    rec = case.segs.cpu().numpy().view(driver.SEG_DTYPE)
+    flags = rec["flags"]
+    rf = (flags & driver.RF) != 0
+    adc = (flags & driver.ADC) != 0
     per_seg = np.where(rf, FLOPS_RF, FLOPS_FREE) + adc * FLOPS_COIL * case.n_coils
+    for _, k, occ in case.pulses:
+        for a, _ in occ:
+            per_seg[a] = FLOPS_PULSE
+            per_seg[a + 1 : a + k] = 0
     cum = np.concatenate([[0], np.cumsum(per_seg, dtype=np.float64)])
     return [float(cum[e] - cum[s]) * case.n for s, e in zip(case.starts, case.ends)]
+
+
+def timed(fn, steps: int, warmup: int) -> float:
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
 
 
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=64, help="grid size of the 3D GRE")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity sample")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint leg")
+    ap.add_argument("--no-exact", action="store_true", help="skip the exact-RF forward leg")
     ap.add_argument("--cpu-trs", type=int, default=32)
     ap.add_argument("--cpu-spins", type=int, default=65536)
     args = ap.parse_args()
@@ -73,25 +107,21 @@ def main() -> None:
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import blochparty_amd as bp
     from blochparty_amd import workloads as W
-    from blochparty_amd.engines import driver, hip
+    from blochparty_amd.engines import adjoint, driver, hip
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     hip.load()
     lib = hip.HIP.lib()
     t_build = time.perf_counter()
     ph, seq, coils = W.gre3d(args.n)
-    case = driver.prepare_case(ph, seq, coils)
+    case_r = driver.prepare_case(ph, seq, coils, reuse=True)
     torch.cuda.synchronize()
-    print(f"# workload built and uploaded in {time.perf_counter() - t_build:.1f} s: N={case.n} S={len(seq)} "
-          f"n_adc={case.n_adc} C={case.n_coils} dispatches={len(case.starts)}", file=sys.stderr, flush=True)
-    flags = case.segs.cpu().numpy().view(driver.SEG_DTYPE)["flags"]
-    flops = dispatch_flops(case, flags)
+    N, n_adc, S = case_r.n, case_r.n_adc, len(seq)
+    work = N * n_adc
+    log(f"workload built and uploaded in {time.perf_counter() - t_build:.1f} s: N={N} S={S} n_adc={n_adc} "
+        f"C={case_r.n_coils}, {case_r.n_pulses} reused pulse(s), {len(case_r.starts)} dispatches")
 
-    for _ in range(args.warmup):
-        driver.run_forward(case, lib)
-    torch.cuda.synchronize()
-    print("# warmup done", file=sys.stderr, flush=True)
-
+    # -- leg 1: forward, reused pulses (the value)
     events = []
 
     def timer(phase):
@@ -99,48 +129,95 @@ def main() -> None:
         ev.record()
         events.append(ev)
 
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        driver.run_forward(case, lib, timer=timer)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        driver.run_forward(case_r, lib)
+    t = timed(lambda: driver.run_forward(case_r, lib, timer=timer), args.steps, 0)
     kernel_ms = [events[2 * i].elapsed_time(events[2 * i + 1]) for i in range(len(events) // 2)]
-    ms_step = 1e3 * elapsed / args.steps
-    value = case.n * case.n_adc * args.steps / elapsed
-    mean_ms = float(np.mean(kernel_ms))
-    mean_flops = float(np.mean(flops))
+    flops = dispatch_flops(case_r, driver)
+    mean_ms, mean_flops = float(np.mean(kernel_ms)), float(np.mean(flops))
     achieved = mean_flops / (mean_ms * 1e-3) / 1e12
-    coil_flops = FLOPS_COIL * case.n_coils * case.n * case.n_adc
-    print(f"# forward: {ms_step:.1f} ms/step, {value:.3e} spin-samples/s; forward kernel mean {mean_ms:.3f} ms over "
-          f"{len(kernel_ms)} launches, {achieved:.1f} TFLOP/s algorithmic", file=sys.stderr, flush=True)
+    value = work * args.steps / t
+    ms_step = 1e3 * t / args.steps
+    log(f"forward (reused pulses): {ms_step:.1f} ms/step, {value:.3e} spin-samples/s; forward kernel mean "
+        f"{mean_ms:.3f} ms over {len(kernel_ms)} launches, {achieved:.1f} TFLOP/s algorithmic")
+    case_r.partial = case_r.maps = None
+    del case_r
+    torch.cuda.empty_cache()
 
-    cpu = None
-    parity = None
+    # -- leg 2: forward, exact RF
+    exact = None
+    if not args.no_exact:
+        case_e = driver.prepare_case(ph, seq, coils)
+        ev_e = []
+        t_e = timed(lambda: driver.run_forward(case_e, lib, timer=lambda _: (ev_e.append(torch.cuda.Event(
+            enable_timing=True)), ev_e[-1].record())), args.steps, args.warmup)
+        k_e = [ev_e[2 * i].elapsed_time(ev_e[2 * i + 1]) for i in range(len(ev_e) // 2)][-len(case_e.starts):]
+        fl_e = float(np.mean(dispatch_flops(case_e, driver)))
+        exact = {"value": work * args.steps / t_e, "ms_per_step": 1e3 * t_e / args.steps,
+                 "kernel_mean_ms": float(np.mean(k_e)),
+                 "kernel_tflops": fl_e / (float(np.mean(k_e)) * 1e-3) / 1e12}
+        log(f"forward (exact RF): {exact['ms_per_step']:.1f} ms/step, {exact['value']:.3e} spin-samples/s; kernel "
+            f"{exact['kernel_mean_ms']:.3f} ms, {exact['kernel_tflops']:.1f} TFLOP/s; reuse speedup "
+            f"{value / exact['value']:.2f}x")
+        del case_e
+        torch.cuda.empty_cache()
+
+    # -- leg 3: forward + adjoint, exact RF
+    fwd_adj = None
+    if not args.no_adjoint:
+        every = adjoint.default_checkpoint_every(S)
+        case_a = driver.prepare_case(ph, seq, coils, chunk=every, aligned=True)
+        nc = next(c for c in adjoint.ADJOINT_COILS if c >= case_a.n_coils)
+        sens = adjoint.adjoint_sens(coils, case_a.n_pad, nc).to(case_a.spins.device)
+        rf64 = seq.rf.double().to(case_a.spins.device)
+        out = {}
+
+        def fwd_adj_step():
+            ckpts = driver.run_forward(case_a, lib, keep=True)
+            sig = case_a.sig[:n_adc]  # L = sum |s|^2: dL/dsignal = 2 s, turned by exp(+i psi)
+            gr, gi = 2 * sig[..., 0], 2 * sig[..., 1]
+            cp, sp = torch.cos(case_a.psi_rows)[:, None], torch.sin(case_a.psi_rows)[:, None]
+            h = torch.zeros(n_adc, nc, 2, dtype=torch.float32, device=sig.device)
+            h[:, : case_a.n_coils, 0] = (gr * cp - gi * sp).float()
+            h[:, : case_a.n_coils, 1] = (gr * sp + gi * cp).float()
+            lam = torch.zeros(3, case_a.n_pad, dtype=torch.float32, device=sig.device)
+            gs, gc, segg, _ = adjoint.run_backward(case_a, ckpts, every, h, lam, sens, nc, True, True, lib)
+            out["flip"] = (segg.view(S, 8)[:, 1:3] * rf64).sum() / 10.0  # rf = flip * shape: dL/dflip
+            out["t1"] = gs[5]
+            out["coils"] = gc
+
+        t_a = timed(fwd_adj_step, args.steps, args.warmup)
+        assert torch.isfinite(out["flip"]) and torch.isfinite(out["t1"]).all() and torch.isfinite(out["coils"]).all()
+        fwd_adj = {"value": work * args.steps / t_a, "ms_per_step": 1e3 * t_a / args.steps, "rf": "exact",
+                   "checkpoint_every": every, "loss": "sum |signal|^2", "wrt": ["flip angle", "T1 map", "coil maps"]}
+        log(f"forward + adjoint (exact RF): {fwd_adj['ms_per_step']:.1f} ms/step, {fwd_adj['value']:.3e} "
+            f"spin-samples/s")
+        del case_a
+        torch.cuda.empty_cache()
+
+    # -- CPU baseline and parity sample
+    cpu = parity = None
     if not args.no_cpu:
         from oracle import core  # the checker / CPU baseline only
 
         cores = cpu_quota()
         torch.set_num_threads(cores)
-        ph_s, c_s = W.subset(ph, coils, min(args.cpu_spins, len(ph)))
+        ph_s, c_s = W.subset(ph, coils, min(args.cpu_spins, N))
         seq_s = W.first_trs(seq, args.cpu_trs)
         t1 = time.perf_counter()
         ref_sig, ref_m = core.simulate(ph_s, seq_s, c_s)
         t_cpu = time.perf_counter() - t1
-        cpu = {
-            "value": len(ph_s) * seq_s.n_adc / t_cpu,
-            "unit": "spin·samples/s",
-            "cores": cores,
-            "kind": "port",
-            "sample": f"FP64 oracle, {len(ph_s)} of the {case.n} spins (seeded subset) over the first {args.cpu_trs} "
-                      f"TRs ({seq_s.n_adc} samples, {len(seq_s)} segments) of the same sequence; {t_cpu:.1f} s",
-        }
-        out = bp.simulate(ph_s, seq_s, c_s, engine="gpu")
-        err_sig = float((out.signal.to(torch.complex128) - ref_sig).abs().max() / ref_sig.abs().max())
-        err_m = float((out.m.double() - ref_m).abs().max() / ref_m.abs().max())
-        parity = {"signal_rel_err": err_sig, "m_rel_err": err_m, "sample": "the CPU-baseline sample"}
-        print(f"# cpu oracle {cpu['value']:.3e} spin-samples/s on {cores} cores; GPU vs oracle on the sample: "
-              f"signal {err_sig:.2e}, m {err_m:.2e}", file=sys.stderr, flush=True)
+        cpu = {"value": len(ph_s) * seq_s.n_adc / t_cpu, "unit": "spin·samples/s", "cores": cores, "kind": "port",
+               "sample": f"FP64 oracle, {len(ph_s)} of the {N} spins (seeded subset) over the first {args.cpu_trs} "
+                         f"TRs ({seq_s.n_adc} samples, {len(seq_s)} segments) of the same sequence; {t_cpu:.1f} s"}
+        parity = {"sample": "the CPU-baseline sample"}
+        for name, opt in (("reuse", "reuse"), ("exact", "step")):
+            o = bp.simulate(ph_s, seq_s, c_s, engine="gpu", pulses=opt)
+            parity[f"{name}_signal_rel_err"] = float((o.signal.to(torch.complex128) - ref_sig).abs().max()
+                                                     / ref_sig.abs().max())
+            parity[f"{name}_m_rel_err"] = float((o.m.double() - ref_m).abs().max() / ref_m.abs().max())
+        log(f"cpu oracle {cpu['value']:.3e} spin-samples/s on {cores} cores; GPU vs oracle on the sample: "
+            f"reuse {parity['reuse_signal_rel_err']:.2e}, exact {parity['exact_signal_rel_err']:.2e}")
 
     line = {
         "metric": METRIC,
@@ -157,13 +234,9 @@ def main() -> None:
         "data": "synthetic",
         "config": {
             "workload": f"3D GRE {args.n}^3 x 4 sub-voxel isochromats, three-class random phantom, +-100 Hz dB0, "
-                        f"2 ms sinc slab (TBW 4, 200 substeps, exact RF), 8 coils, TR 8 ms, forward",
-            "spins": case.n,
-            "segments": len(seq),
-            "n_adc": case.n_adc,
-            "coils": case.n_coils,
-            "rf": "exact",
-            "pass": "forward",
+                        f"2 ms sinc slab (TBW 4, 200 substeps), 8 coils, TR 8 ms; forward, reused pulse maps",
+            "spins": N, "segments": S, "n_adc": n_adc, "coils": int(coils.shape[1]),
+            "rf": "reused per-spin pulse maps (float64, exact)", "pass": "forward",
         },
         "roofline": {
             "bound": "valu",
@@ -176,8 +249,11 @@ def main() -> None:
             "mean_launch_ms": mean_ms,
             "launches": len(kernel_ms),
             "flops_per_launch": mean_flops,
-            "coil_mfma_tflops": coil_flops * args.steps / (sum(kernel_ms) * 1e-3) / 1e12,
         },
+        "exact_rf_forward": exact,
+        "speedup_reuse_vs_exact": None if exact is None else value / exact["value"],
+        "fwd_adjoint": fwd_adj,
+        "surrogate": None,
         "cpu_baseline": cpu,
         "parity": parity,
     }

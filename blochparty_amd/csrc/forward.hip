@@ -101,10 +101,50 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(hb_case cs, int s0, int
             for (int u = 0; u < 16; ++u) A[t][u] = sa[t * 1024 + u * 64];
     }
     Cache k = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float e1_run = 0.0f;  // expm1(-L dt R1) of the last readout run
     int nb = 0, row = 0;
     for (int s = s0; s < s1; ++s) {
         const hb_segment q = cs.segs[s];
         const int f = q.flags;
+        if (f & HB_RUN) {  // L identical free ADC segments: Mxy stepped and sampled per segment, Mz relaxed once
+            const int L = q.pad[3];
+            if (!(f & HB_SAME_FREE)) {
+                sincospif(-2.0f * free_cycles(p, q), &k.s, &k.c);
+                k.e1 = expm1f(-q.dt * p.r1);
+                k.e2 = expm1f(-q.dt * p.r2);
+            }
+            if (!(f & HB_SAME_RUN)) e1_run = expm1f(-(float)L * q.dt * p.r1);
+            for (int i = 0; i < L; ++i) {
+                const float nx = k.c * m.mx - k.s * m.my, ny = k.s * m.mx + k.c * m.my;
+                m.mx = nx + nx * k.e2;
+                m.my = ny + ny * k.e2;
+                stg[nb * kStgStride + lane] = m.mx;
+                stg[(8 + nb) * kStgStride + lane] = m.my;
+                if (++nb == 8) {
+                    flush<NT>(stg, wred, A, sa, nt, 8, row, rows, partial);
+                    row += 8;
+                    nb = 0;
+                }
+            }
+            two_sum(m.mz, m.lz, (m.mz - p.pd) * e1_run);
+            s += L - 1;
+            continue;
+        }
+        if (f & HB_PULSE) {  // a reused pulse: m += Rz(psi) (D Rz(-psi) m + v), then skip its segments
+            const float* U = cs.maps + (size_t)q.pad[1] * 12 * np + j;  // loaded per occurrence: no registers held
+            float sph, cph;
+            sincosf(q.psi, &sph, &cph);
+            const float ux = cph * m.mx + sph * m.my, uy = cph * m.my - sph * m.mx, uz = m.mz;
+            const float dux = U[0] * ux + U[np] * uy + U[2 * np] * uz + U[9 * np];
+            const float duy = U[3 * np] * ux + U[4 * np] * uy + U[5 * np] * uz + U[10 * np];
+            const float duz = U[6 * np] * ux + U[7 * np] * uy + U[8 * np] * uz + U[11 * np];
+            m.mx += cph * dux - sph * duy;
+            m.my += sph * dux + cph * duy;
+            two_sum(m.mz, m.lz, duz);
+            if (f & HB_PULSE_SPOIL) m.mx = m.my = 0.0f;
+            s += q.pad[2] - 1;
+            continue;
+        }
         step(q, f, p, b1sq, m, k);
         if (f & HB_ADC) {
             stg[nb * kStgStride + lane] = m.mx;
@@ -150,6 +190,7 @@ extern "C" int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* part
     if (!cs || cs->n_pad <= 0 || cs->n_pad % kBlock || cs->n < 0 || cs->n > cs->n_pad) return hipErrorInvalidValue;
     if (cs->n_tiles < 1 || cs->n_tiles > kMaxTiles || 8 * cs->n_tiles < cs->n_coils) return hipErrorInvalidValue;
     if (s0 < 0 || s1 < s0 || s1 > cs->n_segments || rows < 0) return hipErrorInvalidValue;
+    if (cs->n_pulses < 0 || (cs->n_pulses > 0 && !cs->maps)) return hipErrorInvalidValue;
     if (s1 == s0) return hipSuccess;
     const dim3 grid(cs->n_pad / kBlock), block(kBlock);
     const size_t lds = sizeof(float) * lds_floats(cs->n_tiles);
@@ -178,4 +219,4 @@ extern "C" int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t 
 
 extern "C" const char* hb_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 
-extern "C" int hb_abi_version(void) { return 4; }
+extern "C" int hb_abi_version(void) { return 6; }

@@ -85,7 +85,9 @@ def simulate(
     (NotImplementedError) unless None. seed: only meaningful for diffusion, which is refused.
     options: ``chunk`` (segments per dispatch, default 32768), ``group`` (256), ``mma`` (the matrix cores always
     contract the coils here; any value is accepted), ``readouts`` ("direct"), ``nufft_width`` / ``nufft_terms``
-    (unused with "direct"), ``pulses`` ("step"). Other keywords raise TypeError.
+    (unused with "direct"), ``pulses`` ("step", or "reuse": a pulse that recurs up to a global RF phase is composed
+    once per spin into its affine map, in float64, and applied at every occurrence; forward only). Other keywords
+    raise TypeError.
     """
     name = resolve(engine)
     if unknown := sorted(set(options) - set(GPU_OPTIONS)):
@@ -104,20 +106,20 @@ def simulate(
         raise NotImplementedError("hip engine: readouts='nufft' is not implemented; use readouts='direct'")
     if pulses not in ("step", "reuse"):
         raise ValueError(f"pulses must be 'step' or 'reuse', not {pulses!r}")
-    if pulses == "reuse":
-        raise NotImplementedError("hip engine: pulses='reuse' is not implemented; use pulses='step'")
     if chunk < 1:
         raise ValueError("chunk must be positive")
     grad = torch.is_grad_enabled() and requires_grad(phantom, seq, coils, m_init)
     check(name, features(phantom, seq, m_init, scanner, maps), grad=grad)
     if grad:
+        if pulses == "reuse":
+            raise ValueError("pulses='reuse' is forward only")
         from .adjoint import simulate_grad
 
         return simulate_grad(phantom, seq, coils, m_init, checkpoint_every)
     from .driver import forward
 
     with torch.no_grad():
-        return forward(phantom, seq, coils, m_init, chunk=chunk)
+        return forward(phantom, seq, coils, m_init, chunk=chunk, reuse=pulses == "reuse")
 
 
 __all__ = ["ENGINES", "GPU_OPTIONS", "available", "resolve", "simulate"]

@@ -57,7 +57,7 @@ class GPUSimulate(torch.autograd.Function):
         res = driver.read_result(case)
         case.partial = None  # the forward's work buffer is not needed any more
         ctx.opts, ctx.case, ctx.ckpts = opts, case, ckpts
-        ctx.sens_re, ctx.sens_im = coils.real.T.float().cpu(), coils.imag.T.float().cpu()  # (C, N)
+        ctx.coils = coils.cpu()
         ctx.like = [None if t is None else (t.dtype, t.device)
                     for t in (pos, pd, t1, t2, db0, coils_ri, dt, rf, grad, adc_phase, m0, b1, b0_offset)]
         sig = torch.view_as_real(res.signal.T.contiguous())  # (n_adc, C, 2)
@@ -77,7 +77,6 @@ class GPUSimulate(torch.autograd.Function):
             return (None,) * 14
         dev = case.spins.device
         n, n_pad, C, S, every = case.n, case.n_pad, case.n_coils, len(opts.is_rf), opts.every
-        n_blocks = n_pad // driver.BLOCK
         adc_idx = torch.tensor(opts.adc).nonzero()[:, 0]
         nc = 0
         if g_sig is not None and case.n_adc:
@@ -90,33 +89,14 @@ class GPUSimulate(torch.autograd.Function):
             gr, gi = g_sig[..., 0].double().cpu(), g_sig[..., 1].double().cpu()
             h[:, :C, 0] = gr * torch.cos(psi) - gi * torch.sin(psi)
             h[:, :C, 1] = gr * torch.sin(psi) + gi * torch.cos(psi)
-        sens = torch.zeros(2 * max(nc, 1), n_pad, dtype=torch.float32)  # rows Re S_0, Im S_0, Re S_1, ...
-        if nc:
-            sens[0 : 2 * C : 2, :n] = ctx.sens_re
-            sens[1 : 2 * C : 2, :n] = ctx.sens_im
+        sens = adjoint_sens(ctx.coils, n_pad, nc)
         lam = torch.zeros(3, n_pad, dtype=torch.float32)
         if g_m is not None:
             lam[:, :n] = g_m.detach().float().cpu().T
         need_seg = any(need[i] for i in (6, 7, 8, 12))
         need_coil = bool(nc) and need[5]
-        gs = torch.zeros(8, n_pad, dtype=torch.float64, device=dev)
-        gc = torch.zeros(2 * max(nc, 1), n_pad, dtype=torch.float64, device=dev)
-        gseg = torch.zeros(n_blocks * every * 8 if need_seg else 1, dtype=torch.float32, device=dev)
-        seg = torch.zeros(S * 8 if need_seg else 1, dtype=torch.float64, device=dev)
-        r8 = torch.empty((every + 7) // 8 * 3 * n_pad, dtype=torch.float32, device=dev)
-        lam_d = lam.to(dev).contiguous()
-        h_d = h.float().to(dev).contiguous()
-        sens_d = sens.to(dev).contiguous()
-        lib = hip.HIP.lib()
-        cs = case.struct()
-        for c in reversed(range(len(ckpts))):
-            s0, s1 = case.starts[c], case.ends[c]
-            ad = hip.HbAdjoint(ckpts[c].data_ptr(), r8.data_ptr(), lam_d.data_ptr(), gs.data_ptr(), gc.data_ptr(),
-                               gseg.data_ptr(), h_d.data_ptr(), sens_d.data_ptr(), nc, int(need_seg), int(need_coil),
-                               every)
-            lib.adjoint_interval(ctypes.byref(cs), ctypes.byref(ad), s0, s1)
-            if need_seg:
-                lib.seg_reduce(gseg.data_ptr(), n_blocks, every, s0, s1 - s0, seg.data_ptr())
+        gs, gc, seg, lam_d = run_backward(case, ckpts, every, h.float().to(dev).contiguous(), lam.to(dev).contiguous(),
+                                          sens.to(dev).contiguous(), nc, need_seg, need_coil)
         hip.HIP.synchronize()
         GS = gs.cpu()[:, :n]
         lam = lam_d.cpu().double()[:, :n]
@@ -148,6 +128,42 @@ class GPUSimulate(torch.autograd.Function):
             out[11] = GS[7]
         return (None, *[None if g is None or not need[i] else g.to(dtype=ctx.like[i][0], device=ctx.like[i][1])
                         for i, g in enumerate(out)])
+
+
+def run_backward(case: driver.DeviceCase, ckpts: list, every: int, h: torch.Tensor, lam: torch.Tensor,
+                 sens: torch.Tensor, nc: int, need_seg: bool, need_coil: bool, lib=None):
+    """The adjoint kernels on the current stream over ``case`` (aligned dispatches, forward run with ``keep``):
+    ``h`` (n_adc, nc, 2) float32 dL/dsignal turned by exp(+i psi), ``lam`` (3, n_pad) float32 dL/dm at the end (it
+    ends as dL/dm0), ``sens`` (2 nc, n_pad). Returns the float64 accumulators (gs (8, n_pad), gc (2 nc, n_pad),
+    seg (S, 8)) and lam."""
+    lib = lib or hip.HIP.lib()
+    dev = case.spins.device
+    n_pad, S = case.n_pad, case.n_segments
+    n_blocks = n_pad // driver.BLOCK
+    gs = torch.zeros(8, n_pad, dtype=torch.float64, device=dev)
+    gc = torch.zeros(2 * max(nc, 1), n_pad, dtype=torch.float64, device=dev)
+    gseg = torch.zeros(n_blocks * every * 8 if need_seg else 1, dtype=torch.float32, device=dev)
+    seg = torch.zeros(S * 8 if need_seg else 1, dtype=torch.float64, device=dev)
+    r8 = torch.empty((every + 7) // 8 * 3 * n_pad, dtype=torch.float32, device=dev)
+    cs = case.struct()
+    for c in reversed(range(len(ckpts))):
+        s0, s1 = case.starts[c], case.ends[c]
+        ad = hip.HbAdjoint(ckpts[c].data_ptr(), r8.data_ptr(), lam.data_ptr(), gs.data_ptr(), gc.data_ptr(),
+                           gseg.data_ptr(), h.data_ptr(), sens.data_ptr(), nc, int(need_seg), int(need_coil), every)
+        lib.adjoint_interval(ctypes.byref(cs), ctypes.byref(ad), s0, s1)
+        if need_seg:
+            lib.seg_reduce(gseg.data_ptr(), n_blocks, every, s0, s1 - s0, seg.data_ptr())
+    return gs, gc, seg, lam
+
+
+def adjoint_sens(coils: torch.Tensor, n_pad: int, nc: int) -> torch.Tensor:
+    """(N, C) complex sensitivities as the adjoint's (2 nc, n_pad) float32 rows Re S_0, Im S_0, Re S_1, ..."""
+    n, C = coils.shape
+    sens = torch.zeros(2 * max(nc, 1), n_pad, dtype=torch.float32)
+    if nc:
+        sens[0 : 2 * C : 2, :n] = coils.real.T.float().cpu()
+        sens[1 : 2 * C : 2, :n] = coils.imag.T.float().cpu()
+    return sens
 
 
 def simulate_grad(phantom: Phantom, seq: Sequence, coils, m_init, checkpoint_every) -> SimResult:

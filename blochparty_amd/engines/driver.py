@@ -26,10 +26,12 @@ import torch
 from ..phantom import Phantom
 from ..result import SimResult
 from ..sequence import Sequence
-from . import hip
+from . import hip, pulses
 
 BLOCK = 256
-RF, ADC, SPOIL, SAME_FREE, SAME_RF_DT, NEXT_FREE, NEXT_RF_DT = 1, 2, 4, 8, 16, 32, 64
+RF, ADC, SPOIL, SAME_FREE, SAME_RF_DT, NEXT_FREE, NEXT_RF_DT, PULSE, PULSE_SPOIL, RUN, SAME_RUN = (
+    1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024)
+MIN_RUN = 4  # readout runs shorter than this are stepped segment by segment
 SEG_DTYPE = np.dtype([("f", "<f4", (8,)), ("d", "<f8", (5,)), ("flags", "<i4"), ("pad", "<i4", (5,)),
                       ("w", "<f4", (8,))])
 assert SEG_DTYPE.itemsize == 128
@@ -42,8 +44,11 @@ def _close(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     return np.abs(a - b) <= SAME_TOL * np.maximum(np.abs(a), np.abs(b))
 
 
-def segment_table(seq: Sequence, starts: list[int]) -> np.ndarray:
-    """The (S,) records of ``seq``; ``starts`` are the dispatch starts (the factor caches restart there)."""
+def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = False) -> np.ndarray:
+    """The (S,) records of ``seq``; ``starts`` are the dispatch starts (the factor caches restart there); ``pulses``:
+    reused pulses (``pulses.reused_pulses``), whose occurrences are applied as maps and not stepped; ``runs``: mark
+    readout runs (identical free ADC segments) for the forward kernel's per-run Mz step (not with checkpoints: the
+    adjoint replays segment by segment)."""
     S = len(seq)
     dt = seq.dt.detach().double().cpu().numpy()
     rf = seq.rf.detach().double().cpu().numpy()
@@ -62,14 +67,26 @@ def segment_table(seq: Sequence, starts: list[int]) -> np.ndarray:
     first = np.zeros(S, dtype=bool)
     first[np.asarray(starts, dtype=np.int64)] = True
     chunk_id = np.cumsum(first) - 1
+    stepped = np.ones(S, dtype=bool)  # segments the kernels step (not inside a reused pulse's occurrence)
+    for pid, (_, k, occ) in enumerate(pulses):
+        for a, ph in occ:
+            stepped[a : a + k] = False
+            flags[a] |= PULSE
+            if spoil[a + k - 1]:
+                flags[a] |= PULSE_SPOIL
+            rec["pad"][a, 1], rec["pad"][a, 2] = pid, k
+            rec["f"][a, 7] = np.float32(ph % (2 * math.pi))
     key = np.stack([dt, g[:, 0], g[:, 1], g[:, 2], b0], 1)
-    for kind, bit, nxt, cols in ((~is_rf, SAME_FREE, NEXT_FREE, slice(0, 5)), (is_rf, SAME_RF_DT, NEXT_RF_DT, slice(0, 1))):
+    for kind, bit, nxt, cols in ((~is_rf & stepped, SAME_FREE, NEXT_FREE, slice(0, 5)),
+                                 (is_rf & stepped, SAME_RF_DT, NEXT_RF_DT, slice(0, 1))):
         idx = np.flatnonzero(kind)
         if len(idx) > 1:
             prev, cur = idx[:-1], idx[1:]
             same = _close(key[cur, cols], key[prev, cols]).all(1) & (chunk_id[cur] == chunk_id[prev])
             flags[cur[same]] |= bit  # forward order: equal to the previous one of the kind
             flags[prev[same]] |= nxt  # reverse order: equal to the next one of the kind
+    if runs:
+        mark_runs(flags, rec, key, is_rf, adc, spoil, chunk_id, stepped)
     rec["flags"] = flags
     w = -2 * math.pi * dt  # the rotation-vector factors, formed in float64
     rec["w"] = np.stack([w, w * rf[:, 0], w * rf[:, 1], w * g[:, 0], w * g[:, 1], w * g[:, 2], w * b0,
@@ -78,10 +95,45 @@ def segment_table(seq: Sequence, starts: list[int]) -> np.ndarray:
     return rec
 
 
-def dispatch_starts(seq: Sequence, chunk: int, rows_cap: int) -> list[int]:
-    """Dispatch boundaries: at most ``chunk`` segments and ``rows_cap`` ADC samples each."""
+def mark_runs(flags, rec, key, is_rf, adc, spoil, chunk_id, stepped) -> None:
+    """HB_RUN on the first of every maximal run (>= MIN_RUN) of consecutive free ADC segments with one key (dt, g,
+    b0_offset), none spoiled, in one dispatch; its length in pad[3]. The run's first segment keeps its SAME_FREE
+    hint (the kernel's factor cache), the rest of the run is skipped; SAME_RUN when the previous run in the dispatch
+    has the same dt and length."""
+    S = len(flags)
+    ok = ~is_rf & adc & ~spoil & stepped
+    same_next = np.zeros(S, dtype=bool)
+    same_next[:-1] = ok[:-1] & ok[1:] & _close(key[1:], key[:-1]).all(1) & (chunk_id[1:] == chunk_id[:-1])
+    prev_key = None
+    s = 0
+    while s < S:
+        if not ok[s]:
+            s += 1
+            continue
+        e = s + 1
+        while e < S and same_next[e - 1]:
+            e += 1
+        if e - s >= MIN_RUN:
+            flags[s] |= RUN
+            rec["pad"][s, 3] = e - s
+            this = (chunk_id[s], key[s, 0], e - s)
+            if prev_key is not None and prev_key[0] == this[0] and prev_key[2] == this[2] and _close(
+                    np.array([prev_key[1]]), np.array([this[1]])).all():
+                flags[s] |= SAME_RUN
+            prev_key = this
+            # the run's inner segments are not stepped: hints that name them would refer to nothing
+            flags[s + 1 : e] &= ~(SAME_FREE | NEXT_FREE)
+        s = e
+
+
+def dispatch_starts(seq: Sequence, chunk: int, rows_cap: int, spans=()) -> list[int]:
+    """Dispatch boundaries: at most ``chunk`` segments and ``rows_cap`` ADC samples each, never inside one of
+    ``spans`` ((first, length) of reused-pulse occurrences, which hold no ADC)."""
     adc = np.asarray(seq.adc, dtype=np.int64)
     S = len(seq)
+    inside = np.zeros(S + 1, dtype=bool)  # a cut before segment e would split a span
+    for a, k in spans:
+        inside[a + 1 : a + k] = True
     starts, s = [], 0
     cum = np.concatenate([[0], np.cumsum(adc)])
     while s < S:
@@ -89,7 +141,13 @@ def dispatch_starts(seq: Sequence, chunk: int, rows_cap: int) -> list[int]:
         e = min(S, s + chunk)
         if cum[e] - cum[s] > rows_cap:  # cut at the segment holding sample number rows_cap of this dispatch
             e = int(np.searchsorted(cum, cum[s] + rows_cap, side="left"))
-        s = max(e, s + 1)
+        while s < e < S and inside[e]:
+            e -= 1
+        if e <= s:  # a span longer than the chunk: cut after it
+            e = s + 1
+            while e < S and inside[e]:
+                e += 1
+        s = e
     return starts
 
 
@@ -129,12 +187,18 @@ class DeviceCase:
     rows: list[int]
     partial: torch.Tensor
     sig: torch.Tensor  # (n_adc, C, 2) float64
+    n_pulses: int = 0
+    maps: torch.Tensor | None = None  # (P, 12, n_pad) reused-pulse maps
+    pulse_table: torch.Tensor | None = None  # (rows, 8) float64
+    pulse_index: torch.Tensor | None = None  # (P, 2) int32
+    pulses: list = field(default_factory=list)
     extra: dict = field(default_factory=dict)
 
     def struct(self) -> hip.HbCase:
         return hip.HbCase(self.spins.data_ptr(), self.pos_lo.data_ptr(), self.sens_a.data_ptr(),
-                          self.segs.data_ptr(), self.state.data_ptr(), self.n, self.n_pad, self.n_coils,
-                          self.n_tiles, self.n_segments)
+                          self.segs.data_ptr(), self.state.data_ptr(),
+                          None if self.maps is None else self.maps.data_ptr(), self.n, self.n_pad, self.n_coils,
+                          self.n_tiles, self.n_segments, self.n_pulses)
 
     @property
     def n_segments(self) -> int:
@@ -151,7 +215,8 @@ def check_inputs(phantom: Phantom, coils) -> None:
 
 
 def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, chunk: int = DEFAULT_CHUNK,
-                 device=None, partial_bytes: int = PARTIAL_BYTES, aligned: bool = False) -> DeviceCase:
+                 device=None, partial_bytes: int = PARTIAL_BYTES, aligned: bool = False,
+                 reuse: bool = False) -> DeviceCase:
     """Upload a case: the buffers of the module docstring, the dispatch plan and the work buffers. ``aligned``:
     dispatches are exactly [c chunk, (c + 1) chunk) (the adjoint's checkpoint intervals), the partial buffer sized
     for the largest one."""
@@ -185,12 +250,17 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
     n_blocks = n_pad // BLOCK
     vals = 16 * n_tiles
     rows_cap = max(8, (partial_bytes // (4 * vals * n_blocks)) // 8 * 8)
-    starts = list(range(0, len(seq), int(chunk))) if aligned else dispatch_starts(seq, int(chunk), rows_cap)
+    reused = pulses.reused_pulses(seq) if reuse else []
+    if reused and aligned:
+        raise ValueError("pulses='reuse' is forward only")
+    spans = [(a, k) for _, k, occ in reused for a, _ in occ]
+    starts = (list(range(0, len(seq), int(chunk))) if aligned
+              else dispatch_starts(seq, int(chunk), rows_cap, spans))
     ends = starts[1:] + [len(seq)]
     adc_cum = np.concatenate([[0], np.cumsum(np.asarray(seq.adc, dtype=np.int64))])
     row_starts = [int(adc_cum[s]) for s in starts]
     rows = [int(adc_cum[e] - adc_cum[s]) for s, e in zip(starts, ends)]
-    rec = segment_table(seq, starts)
+    rec = segment_table(seq, starts, reused, runs=not aligned)
     psi_rows = np.remainder(seq.adc_phase.detach().double().cpu().numpy()[np.asarray(seq.adc, dtype=bool)],
                             2 * math.pi)
     max_rows = max(rows + [0])
@@ -206,7 +276,18 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
         starts=starts, ends=ends, row_starts=row_starts, rows=rows,
         partial=torch.empty(max(1, n_blocks * max_rows * vals), dtype=torch.float32, device=dev),
         sig=torch.zeros(max(n_adc, 1), n_coils, 2, dtype=torch.float64, device=dev),
+        **pulse_buffers(seq, reused, n_pad, dev),
     )
+
+
+def pulse_buffers(seq: Sequence, reused, n_pad: int, dev) -> dict:
+    """The reused pulses' float64 table and index, and room for their per-spin maps."""
+    if not reused:
+        return {}
+    table, index = pulses.pulse_table(seq, reused)
+    return dict(n_pulses=len(reused), pulses=reused,
+                maps=torch.empty(len(reused) * 12 * n_pad, dtype=torch.float32, device=dev),
+                pulse_table=torch.from_numpy(table).to(dev), pulse_index=torch.from_numpy(index).to(dev))
 
 
 def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> list[torch.Tensor] | None:
@@ -218,6 +299,9 @@ def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> l
     st = case.struct()
     n_blocks = case.n_pad // BLOCK
     ckpts = [] if keep else None
+    if case.n_pulses:
+        lib.pulse_maps(hip.ctypes.byref(st), case.pulse_table.data_ptr(), case.pulse_index.data_ptr(),
+                       case.n_pulses, case.maps.data_ptr())
     for s0, s1, r0, nr in zip(case.starts, case.ends, case.row_starts, case.rows):
         if keep:
             ckpts.append(case.state.clone())
@@ -241,8 +325,9 @@ def read_result(case: DeviceCase) -> SimResult:
     return SimResult(signal, m)
 
 
-def forward(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, chunk: int = DEFAULT_CHUNK) -> SimResult:
-    case = prepare_case(phantom, seq, coils, m_init, chunk=chunk)
+def forward(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, chunk: int = DEFAULT_CHUNK,
+            reuse: bool = False) -> SimResult:
+    case = prepare_case(phantom, seq, coils, m_init, chunk=chunk, reuse=reuse)
     run_forward(case)
     hip.HIP.synchronize()
     return read_result(case)

@@ -15,7 +15,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent.parent / "libblochparty_amd.so"
-ABI_VERSION = 4
+ABI_VERSION = 6
 MAX_COILS = 128  # 16 tiles of 8 coils (the forward kernel's LDS budget)
 
 
@@ -28,11 +28,13 @@ class HbCase(ctypes.Structure):
         ("sens_a", ctypes.c_void_p),
         ("segs", ctypes.c_void_p),
         ("state", ctypes.c_void_p),
+        ("maps", ctypes.c_void_p),
         ("n", ctypes.c_int32),
         ("n_pad", ctypes.c_int32),
         ("n_coils", ctypes.c_int32),
         ("n_tiles", ctypes.c_int32),
         ("n_segments", ctypes.c_int32),
+        ("n_pulses", ctypes.c_int32),
     ]
 
 
@@ -64,6 +66,7 @@ SIGNATURES = {
     "hb_signal_reduce": [_P, _I, _I, _I, _I, _P, _I, _P, _V],
     "hb_adjoint_interval": [ctypes.POINTER(HbCase), ctypes.POINTER(HbAdjoint), _I, _I, _V],
     "hb_seg_reduce": [_P, _I, _I, _I, _I, _P, _V],
+    "hb_pulse_maps": [ctypes.POINTER(HbCase), _P, _P, _I, _P, _V],
 }
 
 

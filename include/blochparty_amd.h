@@ -27,13 +27,19 @@ extern "C" {
 #define HB_SAME_RF_DT 16  /* RF segment with the previous RF segment's dt (reuse its relaxation factors)    */
 #define HB_NEXT_FREE 32   /* free segment equal to the next free one in the interval (reverse sweep cache)   */
 #define HB_NEXT_RF_DT 64  /* RF segment with the next RF segment's dt in the interval (reverse sweep cache)   */
+#define HB_PULSE 128      /* first segment of a reused pulse: apply map pad[1] turned by psi, skip pad[2] segs */
+#define HB_PULSE_SPOIL 256 /* the reused pulse's last segment is spoiled                                       */
+#define HB_RUN 512        /* first of pad[3] identical free ADC segments: step Mxy per sample, Mz once        */
+#define HB_SAME_RUN 1024  /* a run with the previous run's dt and length (reuse its Mz factor)              */
 
 /* One segment of the event list, 128 bytes (the reference's segs row + flags + rows, driver.py:357-384). psi is the
  * receiver phase wrapped to [0, 2 pi) in float64 before the cast (driver.py:357). The float64 block carries the
  * duration, the gradient moment g dt and the b0_offset area, from which free segments take their phase in float64
  * (k . r reduced mod 1 cycle per segment). The w block holds the segment's rotation-vector factors, formed in float64
  * on the host: w = -2 pi dt, w_rf = w rf, w_g = w g, w_b0 = w b0_offset, w_rf2 = |w_rf|^2 (a spin's RF rotation
- * vector is (b1 w_rf, w_g . r + w db0 + w_b0)). pad[0] is the ADC row of an ADC segment (the reference's adc_row). */
+ * vector is (b1 w_rf, w_g . r + w db0 + w_b0)). pad[0] is the ADC row of an ADC segment (the reference's adc_row);
+ * a reused pulse's first segment (HB_PULSE) holds the pulse id in pad[1], its length in pad[2] and the occurrence's
+ * RF phase in psi (driver.py:592-596); a readout run's first segment (HB_RUN) its length in pad[3]. */
 typedef struct hb_segment {
     float dt, rf_x, rf_y, g_x, g_y, g_z, b0, psi;
     double dt64, m_x, m_y, m_z, b0_area;
@@ -52,7 +58,8 @@ typedef struct hb_case {
     const float* sens_a;
     const hb_segment* segs;
     float* state;
-    int32_t n, n_pad, n_coils, n_tiles, n_segments;
+    const float* maps; /* (n_pulses, 12, n_pad) maps of reused pulses (hb_pulse_maps), or NULL */
+    int32_t n, n_pad, n_coils, n_tiles, n_segments, n_pulses;
 } hb_case;
 
 /* Forward event loop over segments [s0, s1) (replaces bloch_chunk_mma, cuda/kernels.cu.in:2-185, one launch per
@@ -66,6 +73,13 @@ int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* partial, int32_
  * order. psi: (n_adc,) float64. */
 int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t rows, int32_t n_tiles, int32_t n_coils,
                      const double* psi, int32_t row0, double* sig, void* stream);
+
+/* Per-spin affine maps of reused pulses (replaces pulse_maps, cuda/pulse.cu.in:3-32, launched at driver.py:600-613):
+ * pulse p's K substeps composed in float64 into m -> m + D m + v; maps[(p * 12 + r) * n_pad + spin] holds D's rows
+ * (r = 0..8, D[i][k] at 3 i + k) and v (r = 9..11). table: (rows, 8) float64 dt, rf_x, rf_y, g_x, g_y, g_z,
+ * b0_offset, 0 of every pulse's first occurrence; index: (n_pulses, 2) its (first row, length). */
+int hb_pulse_maps(const hb_case* cs, const double* table, const int32_t* index, int32_t n_pulses, float* maps,
+                  void* stream);
 
 /* One interval of the adjoint (replaces bloch_replay + bloch_adjoint_c{NC}, cuda/replay.cu.in:3-33,
  * cuda/adjoint.cu.in:5-121; one call per interval of adjoint.py:179-220, last interval first). ckpt: (6, n_pad)

@@ -100,6 +100,50 @@ def test_dispatch_plan_and_cache_hints():
     assert (psi >= 0).all() and (psi < 2 * np.pi + 1e-6).all()
 
 
+def walk(rec, s0, s1):
+    """The ADC rows a dispatch stages, in the forward kernel's control flow (runs, pulses, single segments)."""
+    rows, s = [], s0
+    while s < s1:
+        f = rec["flags"][s]
+        if f & driver.RUN:
+            L = rec["pad"][s, 3]
+            rows += list(range(rec["pad"][s, 0], rec["pad"][s, 0] + L))
+            s += L
+        elif f & driver.PULSE:
+            s += rec["pad"][s, 2]
+        else:
+            if f & driver.ADC:
+                rows.append(rec["pad"][s, 0])
+            s += 1
+    assert s == s1
+    return rows
+
+
+@pytest.mark.parametrize("chunk", [10**6, 7, 64])
+def test_readout_runs_cover_every_sample_once(chunk):
+    from blochparty_amd import workloads
+
+    _, seq, _ = workloads.gre2d(16, n_dummy=4, n_coils=1)
+    starts = driver.dispatch_starts(seq, chunk, 1 << 20)
+    rec = driver.segment_table(seq, starts, runs=True)
+    fl, adc = rec["flags"], np.asarray(seq.adc)
+    ends = starts[1:] + [len(seq)]
+    assert sum((walk(rec, a, b) for a, b in zip(starts, ends)), []) == list(range(int(adc.sum())))
+    key = np.stack([seq.dt.numpy(), *seq.grad.numpy().T], 1)
+    first = np.flatnonzero(fl & driver.RUN)
+    assert len(first) > 0 if chunk > 16 else True
+    for s in first:
+        L = rec["pad"][s, 3]
+        assert L >= driver.MIN_RUN
+        assert not np.asarray(seq.is_rf)[s : s + L].any() and adc[s : s + L].all()
+        assert not np.asarray(seq.spoil)[s : s + L].any()
+        assert np.allclose(key[s : s + L], key[s], rtol=1e-12, atol=0)
+        assert not any(a in range(s + 1, s + L) for a in starts)  # never across a dispatch start
+    # without runs (the adjoint's replay) no record carries the run bits
+    plain = driver.segment_table(seq, starts)
+    assert not (plain["flags"] & (driver.RUN | driver.SAME_RUN)).any()
+
+
 def test_case_packing_splits_float64_to_48_bits():
     ph = random_phantom(300, seed=1)
     m0 = torch.randn(300, 3, dtype=torch.float64)
