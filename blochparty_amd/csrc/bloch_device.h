@@ -79,7 +79,7 @@ __device__ __forceinline__ void rot_ab(float p2, float& a, float& b) {
     } else {
         const float phi = sqrtf(p2);
         float sh, ch;
-        sincosf(0.5f * phi, &sh, &ch);
+        sincospif(phi * 0.15915494309189535f, &sh, &ch);  // the half angle phi/2 in units of pi: reduced by rint
         const float sg = sh / phi;
         a = 2.0f * ch * sg;
         b = 2.0f * sg * sg;
@@ -116,8 +116,21 @@ __device__ __forceinline__ void store_state(float* __restrict__ st, int n_pad, i
 // rounding Mz to float32 after every segment biases the steady state (2e-5 of the signal of a 264-TR spoiled GRE,
 // against 5e-7 with two-sum); the transverse components are rotated and spoiled, so their rounding does not build up
 // (a 5,000-substep CW saturation stays within 1e-6 of float64 without it).
-__device__ __forceinline__ void step(const hb_segment& q, int f, const Spin& p, float b1sq, State& m, Cache& k) {
-    if (f & HB_RF) {
+// The fields of a record an RF step reads (dt, flags and the rotation factors): a train loop loads the next one while
+// it steps the current one.
+struct RfRec {
+    float dt, w, w_rf_x, w_rf_y, w_g_x, w_g_y, w_g_z, w_b0, w_rf2;
+    int flags;
+};
+
+__device__ __forceinline__ RfRec load_rf(const hb_segment* __restrict__ r) {
+    const float4 a = *reinterpret_cast<const float4*>(&r->w), b = *reinterpret_cast<const float4*>(&r->w_g_y);
+    return {r->dt, a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, r->flags};
+}
+
+template <class R>
+__device__ __forceinline__ void rf_step(const R& q, int f, const Spin& p, float b1sq, State& m, Cache& k) {
+    {
         if (!(f & HB_SAME_RF_DT)) {
             k.h1 = expm1f(-0.5f * q.dt * p.r1);
             k.h2 = expm1f(-0.5f * q.dt * p.r2);
@@ -135,7 +148,11 @@ __device__ __forceinline__ void step(const hb_segment& q, int f, const Spin& p, 
         m.mx += dx1 + drx + x2 * k.h2;
         m.my += dy1 + dry + y2 * k.h2;
         two_sum(m.mz, m.lz, dz1 + drz + (z2 - p.pd) * k.h1);
-    } else {
+    }
+}
+
+__device__ __forceinline__ void free_step(const hb_segment& q, int f, const Spin& p, State& m, Cache& k) {
+    {
         if (!(f & HB_SAME_FREE)) {
             sincospif(-2.0f * free_cycles(p, q), &k.s, &k.c);
             k.e1 = expm1f(-q.dt * p.r1);
@@ -146,6 +163,13 @@ __device__ __forceinline__ void step(const hb_segment& q, int f, const Spin& p, 
         m.my = ny + ny * k.e2;
         two_sum(m.mz, m.lz, (m.mz - p.pd) * k.e1);
     }
+}
+
+__device__ __forceinline__ void step(const hb_segment& q, int f, const Spin& p, float b1sq, State& m, Cache& k) {
+    if (f & HB_RF)
+        rf_step(q, f, p, b1sq, m, k);
+    else
+        free_step(q, f, p, m, k);
 }
 
 }  // namespace hb

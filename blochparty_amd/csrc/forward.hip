@@ -80,7 +80,7 @@ __device__ __forceinline__ void flush(float* stg, float* wred, const float (&A)[
 }
 
 template <int NT>
-__global__ __launch_bounds__(kBlock) void forward_kernel(hb_case cs, int s0, int s1, float* __restrict__ partial,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 ? 3 : 4))) void forward_kernel(hb_case cs, int s0, int s1, float* __restrict__ partial,
                                                          int rows) {
     extern __shared__ float lds[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -106,6 +106,17 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(hb_case cs, int s0, int
     for (int s = s0; s < s1; ++s) {
         const hb_segment q = cs.segs[s];
         const int f = q.flags;
+        if (f & HB_TRAIN) {  // RF substeps of a shaped pulse, in a loop of their own (only RF state live)
+            const int L = q.pad[3];
+            RfRec cur = load_rf(&cs.segs[s]);
+            for (int i = 1; i <= L; ++i) {
+                const RfRec nxt = load_rf(&cs.segs[s + (i < L ? i : 0)]);  // in flight while this one is stepped
+                rf_step(cur, cur.flags, p, b1sq, m, k);
+                cur = nxt;
+            }
+            s += L - 1;
+            continue;
+        }
         if (f & HB_RUN) {  // L identical free ADC segments: Mxy stepped and sampled per segment, Mz relaxed once
             const int L = q.pad[3];
             if (!(f & HB_SAME_FREE)) {
@@ -133,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(hb_case cs, int s0, int
         if (f & HB_PULSE) {  // a reused pulse: m += Rz(psi) (D Rz(-psi) m + v), then skip its segments
             const float* U = cs.maps + (size_t)q.pad[1] * 12 * np + j;  // loaded per occurrence: no registers held
             float sph, cph;
-            sincosf(q.psi, &sph, &cph);
+            sincospif(q.psi * 0.3183098861837907f, &sph, &cph);
             const float ux = cph * m.mx + sph * m.my, uy = cph * m.my - sph * m.mx, uz = m.mz;
             const float dux = U[0] * ux + U[np] * uy + U[2 * np] * uz + U[9 * np];
             const float duy = U[3 * np] * ux + U[4 * np] * uy + U[5 * np] * uz + U[10 * np];
@@ -219,4 +230,4 @@ extern "C" int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t 
 
 extern "C" const char* hb_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 
-extern "C" int hb_abi_version(void) { return 6; }
+extern "C" int hb_abi_version(void) { return 7; }

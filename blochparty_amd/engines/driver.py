@@ -29,9 +29,10 @@ from ..sequence import Sequence
 from . import hip, pulses
 
 BLOCK = 256
-RF, ADC, SPOIL, SAME_FREE, SAME_RF_DT, NEXT_FREE, NEXT_RF_DT, PULSE, PULSE_SPOIL, RUN, SAME_RUN = (
-    1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024)
+RF, ADC, SPOIL, SAME_FREE, SAME_RF_DT, NEXT_FREE, NEXT_RF_DT, PULSE, PULSE_SPOIL, RUN, SAME_RUN, TRAIN = (
+    1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048)
 MIN_RUN = 4  # readout runs shorter than this are stepped segment by segment
+MIN_TRAIN = 4  # likewise RF trains
 SEG_DTYPE = np.dtype([("f", "<f4", (8,)), ("d", "<f8", (5,)), ("flags", "<i4"), ("pad", "<i4", (5,)),
                       ("w", "<f4", (8,))])
 assert SEG_DTYPE.itemsize == 128
@@ -87,6 +88,7 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
             flags[prev[same]] |= nxt  # reverse order: equal to the next one of the kind
     if runs:
         mark_runs(flags, rec, key, is_rf, adc, spoil, chunk_id, stepped)
+    mark_trains(flags, rec, is_rf, adc, spoil, chunk_id, stepped)
     rec["flags"] = flags
     w = -2 * math.pi * dt  # the rotation-vector factors, formed in float64
     rec["w"] = np.stack([w, w * rf[:, 0], w * rf[:, 1], w * g[:, 0], w * g[:, 1], w * g[:, 2], w * b0,
@@ -123,6 +125,26 @@ def mark_runs(flags, rec, key, is_rf, adc, spoil, chunk_id, stepped) -> None:
             prev_key = this
             # the run's inner segments are not stepped: hints that name them would refer to nothing
             flags[s + 1 : e] &= ~(SAME_FREE | NEXT_FREE)
+        s = e
+
+
+def mark_trains(flags, rec, is_rf, adc, spoil, chunk_id, stepped) -> None:
+    """HB_TRAIN on the first of every maximal run (>= MIN_TRAIN) of consecutive stepped RF segments without ADC or
+    spoil in one dispatch, its length in pad[3]. The forward kernel steps a train in a loop of its own, with the same
+    arithmetic as segment by segment (so checkpointed forwards may use trains too); the segments keep their hints."""
+    ok = is_rf & ~adc & ~spoil & stepped
+    S = len(ok)
+    s = 0
+    while s < S:
+        if not ok[s]:
+            s += 1
+            continue
+        e = s + 1
+        while e < S and ok[e] and chunk_id[e] == chunk_id[s]:
+            e += 1
+        if e - s >= MIN_TRAIN:
+            flags[s] |= TRAIN
+            rec["pad"][s, 3] = e - s
         s = e
 
 

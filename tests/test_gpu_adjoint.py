@@ -143,7 +143,7 @@ def test_ernst_angle_by_gradient_descent(gpu):
     t1, tr, te = 0.1, 5e-3, 2e-3  # 150 TRs = 7.5 T1: the last TR is in the steady state
     ph = bp.Phantom(pos=torch.zeros(1, 3, dtype=f64), pd=torch.ones(1, dtype=f64), t1=torch.full((1,), t1, dtype=f64),
                     t2=torch.full((1,), 0.05, dtype=f64), db0=torch.zeros(1, dtype=f64))
-    flip = torch.tensor(45.0, dtype=f64, requires_grad=True)
+    flip = torch.tensor(60.0, dtype=f64, requires_grad=True)
     opt = torch.optim.LBFGS([flip], lr=1.0, max_iter=50, line_search_fn="strong_wolfe")
 
     def closure():
@@ -158,4 +158,5 @@ def test_ernst_angle_by_gradient_descent(gpu):
 
     opt.step(closure)
     ernst = math.degrees(math.acos(math.exp(-tr / t1)))
-    assert abs(float(flip.detach()) - ernst) < 0.01, (float(flip.detach()), ernst)
+    # |signal| is even in the flip angle: the descent may end at -ernst (as the reference's test allows)
+    assert abs(abs(float(flip.detach())) - ernst) < 0.01, (float(flip.detach()), ernst)

@@ -111,6 +111,11 @@ def walk(rec, s0, s1):
             s += L
         elif f & driver.PULSE:
             s += rec["pad"][s, 2]
+        elif f & driver.TRAIN:
+            L = rec["pad"][s, 3]
+            assert not (rec["flags"][s : s + L] & (driver.ADC | driver.SPOIL)).any()
+            assert (rec["flags"][s : s + L] & driver.RF).all()
+            s += L
         else:
             if f & driver.ADC:
                 rows.append(rec["pad"][s, 0])
@@ -119,11 +124,11 @@ def walk(rec, s0, s1):
     return rows
 
 
-@pytest.mark.parametrize("chunk", [10**6, 7, 64])
-def test_readout_runs_cover_every_sample_once(chunk):
+@pytest.mark.parametrize("chunk", [10**6, 7, 64, 150])
+def test_readout_runs_and_rf_trains_cover_every_sample_once(chunk):
     from blochparty_amd import workloads
 
-    _, seq, _ = workloads.gre2d(16, n_dummy=4, n_coils=1)
+    seq = workloads.first_trs(workloads.gre3d_sequence(8), 6)  # 200-substep sinc pulses, 8-sample readouts
     starts = driver.dispatch_starts(seq, chunk, 1 << 20)
     rec = driver.segment_table(seq, starts, runs=True)
     fl, adc = rec["flags"], np.asarray(seq.adc)
@@ -139,9 +144,14 @@ def test_readout_runs_cover_every_sample_once(chunk):
         assert not np.asarray(seq.spoil)[s : s + L].any()
         assert np.allclose(key[s : s + L], key[s], rtol=1e-12, atol=0)
         assert not any(a in range(s + 1, s + L) for a in starts)  # never across a dispatch start
-    # without runs (the adjoint's replay) no record carries the run bits
+    # without runs (the adjoint's replay) no record carries the run bits; RF trains are marked either way
     plain = driver.segment_table(seq, starts)
     assert not (plain["flags"] & (driver.RUN | driver.SAME_RUN)).any()
+    assert np.array_equal(plain["flags"] & driver.TRAIN, fl & driver.TRAIN)
+    assert sum((walk(plain, a, b) for a, b in zip(starts, ends)), []) == list(range(int(adc.sum())))
+    if chunk > 16:
+        assert (fl & driver.TRAIN).any()
+    assert (rec["pad"][fl & driver.TRAIN != 0, 3] >= driver.MIN_TRAIN).all()
 
 
 def test_case_packing_splits_float64_to_48_bits():
