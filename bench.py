@@ -96,6 +96,9 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity sample")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint leg")
     ap.add_argument("--no-exact", action="store_true", help="skip the exact-RF forward leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the legs, three elementwise copies of MIB MiB (read + write) that calibrate the HBM "
+                         "byte counters of a profiler run")
     ap.add_argument("--cpu-trs", type=int, default=32)
     ap.add_argument("--cpu-spins", type=int, default=65536)
     args = ap.parse_args()
@@ -194,6 +197,15 @@ def main() -> None:
             f"spin-samples/s")
         del case_a
         torch.cuda.empty_cache()
+
+    if args.copy_probe:  # a plain copy of known size: x * 1 into y reads and writes copy_probe MiB each
+        x = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        log(f"copy probe: 3 x {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+        del x, y
 
     # -- CPU baseline and parity sample
     cpu = parity = None

@@ -85,9 +85,11 @@ def simulate(
     (NotImplementedError) unless None. seed: only meaningful for diffusion, which is refused.
     options: ``chunk`` (segments per dispatch, default 32768), ``group`` (256), ``mma`` (the matrix cores always
     contract the coils here; any value is accepted), ``readouts`` ("direct"), ``nufft_width`` / ``nufft_terms``
-    (unused with "direct"), ``pulses`` ("step", or "reuse": a pulse that recurs up to a global RF phase is composed
-    once per spin into its affine map, in float64, and applied at every occurrence; forward only). Other keywords
-    raise TypeError.
+    (unused with "direct"), ``pulses`` ("step"; "reuse": a pulse that recurs up to a global RF phase is composed
+    once per spin into its affine map, in float64, and applied at every occurrence; "surrogate": those maps from
+    the learned RF surrogate, trained on the GPU on first use, with the exact composition as the fallback outside its
+    trained box and as the judge of its error on every run, reported in ``SimResult.surrogate``; both forward only).
+    Other keywords raise TypeError.
     """
     name = resolve(engine)
     if unknown := sorted(set(options) - set(GPU_OPTIONS)):
@@ -104,22 +106,23 @@ def simulate(
         raise ValueError(f"readouts must be 'direct' or 'nufft', not {readouts!r}")
     if readouts == "nufft":
         raise NotImplementedError("hip engine: readouts='nufft' is not implemented; use readouts='direct'")
-    if pulses not in ("step", "reuse"):
-        raise ValueError(f"pulses must be 'step' or 'reuse', not {pulses!r}")
+    if pulses not in ("step", "reuse", "surrogate"):
+        raise ValueError(f"pulses must be 'step', 'reuse' or 'surrogate', not {pulses!r}")
     if chunk < 1:
         raise ValueError("chunk must be positive")
     grad = torch.is_grad_enabled() and requires_grad(phantom, seq, coils, m_init)
     check(name, features(phantom, seq, m_init, scanner, maps), grad=grad)
     if grad:
-        if pulses == "reuse":
-            raise ValueError("pulses='reuse' is forward only")
+        if pulses != "step":
+            raise ValueError(f"pulses={pulses!r} is forward only")
         from .adjoint import simulate_grad
 
         return simulate_grad(phantom, seq, coils, m_init, checkpoint_every)
     from .driver import forward
 
     with torch.no_grad():
-        return forward(phantom, seq, coils, m_init, chunk=chunk, reuse=pulses == "reuse")
+        return forward(phantom, seq, coils, m_init, chunk=chunk, reuse=pulses == "reuse",
+                       surrogate=pulses == "surrogate")
 
 
 __all__ = ["ENGINES", "GPU_OPTIONS", "available", "resolve", "simulate"]

@@ -238,10 +238,12 @@ def check_inputs(phantom: Phantom, coils) -> None:
 
 def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, chunk: int = DEFAULT_CHUNK,
                  device=None, partial_bytes: int = PARTIAL_BYTES, aligned: bool = False,
-                 reuse: bool = False) -> DeviceCase:
+                 reuse: bool = False, surrogate: bool = False) -> DeviceCase:
     """Upload a case: the buffers of the module docstring, the dispatch plan and the work buffers. ``aligned``:
     dispatches are exactly [c chunk, (c + 1) chunk) (the adjoint's checkpoint intervals), the partial buffer sized
-    for the largest one."""
+    for the largest one. ``reuse``: reused pulses applied as per-spin maps; ``surrogate``: those maps from the
+    learned surrogate (trained here on first use)."""
+    reuse = reuse or surrogate
     check_inputs(phantom, coils)
     dev = device if device is not None else hip.HIP.device
     n = len(phantom)
@@ -286,7 +288,7 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
     psi_rows = np.remainder(seq.adc_phase.detach().double().cpu().numpy()[np.asarray(seq.adc, dtype=bool)],
                             2 * math.pi)
     max_rows = max(rows + [0])
-    return DeviceCase(
+    case = DeviceCase(
         n=n, n_pad=n_pad, n_coils=n_coils, n_tiles=n_tiles, n_adc=n_adc,
         spins=spins.to(dev).contiguous(),
         pos_lo=pos_lo.to(dev).contiguous(),
@@ -300,6 +302,11 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
         sig=torch.zeros(max(n_adc, 1), n_coils, 2, dtype=torch.float64, device=dev),
         **pulse_buffers(seq, reused, n_pad, dev),
     )
+    if surrogate and case.n_pulses:
+        from . import surrogate as sur
+
+        sur.prepare(case, seq)
+    return case
 
 
 def pulse_buffers(seq: Sequence, reused, n_pad: int, dev) -> dict:
@@ -321,9 +328,13 @@ def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> l
     st = case.struct()
     n_blocks = case.n_pad // BLOCK
     ckpts = [] if keep else None
-    if case.n_pulses:
+    if case.extra.get("surrogate") is not None:
+        from . import surrogate
+
+        surrogate.compose_maps(case, hip.load())
+    elif case.n_pulses:
         lib.pulse_maps(hip.ctypes.byref(st), case.pulse_table.data_ptr(), case.pulse_index.data_ptr(),
-                       case.n_pulses, case.maps.data_ptr())
+                       case.n_pulses, None, case.maps.data_ptr())
     for s0, s1, r0, nr in zip(case.starts, case.ends, case.row_starts, case.rows):
         if keep:
             ckpts.append(case.state.clone())
@@ -344,12 +355,17 @@ def read_result(case: DeviceCase) -> SimResult:
     signal = torch.complex(sig[..., 0], sig[..., 1]).T.to(torch.complex64).contiguous()
     st = case.state.cpu().double()
     m = (st[:3, : case.n] + st[3:, : case.n]).T.float().contiguous()
-    return SimResult(signal, m)
+    res = SimResult(signal, m)
+    if case.extra.get("surrogate") is not None:
+        from . import surrogate
+
+        res.surrogate = surrogate.report(case)
+    return res
 
 
 def forward(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, chunk: int = DEFAULT_CHUNK,
-            reuse: bool = False) -> SimResult:
-    case = prepare_case(phantom, seq, coils, m_init, chunk=chunk, reuse=reuse)
+            reuse: bool = False, surrogate: bool = False) -> SimResult:
+    case = prepare_case(phantom, seq, coils, m_init, chunk=chunk, reuse=reuse, surrogate=surrogate)
     run_forward(case)
     hip.HIP.synchronize()
     return read_result(case)

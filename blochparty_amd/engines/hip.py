@@ -15,7 +15,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent.parent / "libblochparty_amd.so"
-ABI_VERSION = 7
+ABI_VERSION = 8
 MAX_COILS = 128  # 16 tiles of 8 coils (the forward kernel's LDS budget)
 
 
@@ -66,7 +66,12 @@ SIGNATURES = {
     "hb_signal_reduce": [_P, _I, _I, _I, _I, _P, _I, _P, _V],
     "hb_adjoint_interval": [ctypes.POINTER(HbCase), ctypes.POINTER(HbAdjoint), _I, _I, _V],
     "hb_seg_reduce": [_P, _I, _I, _I, _I, _P, _V],
-    "hb_pulse_maps": [ctypes.POINTER(HbCase), _P, _P, _I, _P, _V],
+    "hb_pulse_maps": [ctypes.POINTER(HbCase), _P, _P, _I, _P, _P, _V],
+    "hb_spinor_blocks": [_P, _P, _P, _I, _P, _V],
+    "hb_mlp_grad": [_P, _P, _P, _I, _P, _P, _V],
+    "hb_mlp_train_step": [_P, _P, _P, _I, _P, _I, ctypes.c_float, _V],
+    "hb_mlp_eval": [_P, _P, _I, _P, _V],
+    "hb_surrogate_maps": [ctypes.POINTER(HbCase), _P, _P, _P, _P, _V],
 }
 
 
@@ -82,6 +87,8 @@ def load() -> ctypes.CDLL:
         lib.hb_error_string.restype = ctypes.c_char_p
         lib.hb_error_string.argtypes = [ctypes.c_int]
         lib.hb_abi_version.restype = ctypes.c_int
+        lib.hb_mlp_workspace.restype = ctypes.c_int64
+        lib.hb_mlp_workspace.argtypes = [ctypes.c_int32]
         if lib.hb_abi_version() != ABI_VERSION:
             raise RuntimeError(f"{LIB_PATH} has ABI {lib.hb_abi_version()}, expected {ABI_VERSION}: rebuild it")
         for name, args in SIGNATURES.items():

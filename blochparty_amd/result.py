@@ -12,9 +12,12 @@ class SimResult:
     """``signal``: (C, n_adc) complex k-space in ADC order; ``m``: (N, 3) magnetization after the last segment.
 
     ``m_bound`` / ``m_pools`` keep the reference's field names; the GPU engine here never fills them (bound and
-    free exchanging pools are refused by its capability check)."""
+    free exchanging pools are refused by its capability check). ``surrogate`` (not in the reference): with
+    ``pulses="surrogate"``, the judge's report of the run (largest map difference against the exact composition on a
+    sample of spins, spins outside the trained box, whether the exact maps replaced the network)."""
 
     signal: Tensor
     m: Tensor
     m_bound: Tensor | None = None
     m_pools: Tensor | None = None
+    surrogate: dict | None = None

@@ -79,9 +79,65 @@ int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t rows, int32
 /* Per-spin affine maps of reused pulses (replaces pulse_maps, cuda/pulse.cu.in:3-32, launched at driver.py:600-613):
  * pulse p's K substeps composed in float64 into m -> m + D m + v; maps[(p * 12 + r) * n_pad + spin] holds D's rows
  * (r = 0..8, D[i][k] at 3 i + k) and v (r = 9..11). table: (rows, 8) float64 dt, rf_x, rf_y, g_x, g_y, g_z,
- * b0_offset, 0 of every pulse's first occurrence; index: (n_pulses, 2) its (first row, length). */
-int hb_pulse_maps(const hb_case* cs, const double* table, const int32_t* index, int32_t n_pulses, float* maps,
-                  void* stream);
+ * b0_offset, 0 of every pulse's first occurrence; index: (n_pulses, 2) its (first row, length). mask: (n_pad,) or
+ * NULL; when given, only spins with mask[spin] != 0 are composed (the others' maps are left as they are). */
+int hb_pulse_maps(const hb_case* cs, const double* table, const int32_t* index, int32_t n_pulses,
+                  const int32_t* mask, float* maps, void* stream);
+
+/* ---- The learned RF surrogate (no counterpart in the reference: it stands where the reference composes a reused
+ * pulse per spin, cuda/pulse.cu.in:3-32). A pulse is cut into HB_SUR_BLOCKS blocks; a network 3 -> 64 -> 64 -> 4 M
+ * (tanh) maps a spin's (df [Hz], u = g . r [Hz], b1) to the blocks' rotations as unit quaternions (w, x, y, z) in the
+ * frame turning with the off-resonance precession; the spin's map is composed from them in float64 with the
+ * relaxation put back half a block before and after each rotation. Float32 parameters, float32 arithmetic. */
+#define HB_MLP_HIDDEN 64
+#define HB_SUR_BLOCKS 10
+#define HB_MLP_PARAMS (HB_MLP_HIDDEN * 3 + HB_MLP_HIDDEN + HB_MLP_HIDDEN * HB_MLP_HIDDEN + HB_MLP_HIDDEN + \
+                       4 * HB_SUR_BLOCKS * HB_MLP_HIDDEN + 4 * HB_SUR_BLOCKS)
+
+/* A network. params: (HB_MLP_PARAMS,) W1 (64, 3), b1 (64), W2 (64, 64), b2 (64), W3 (n_out, 64), b3 (n_out);
+ * adam_m, adam_v: Adam's moments (training only); norm: (6 + 2 n_out,) the input box's centre (3) and half-widths
+ * (3), the outputs' means (n_out) and scales (n_out): the network sees (x - centre) / half-width and is trained on
+ * (y - mean) / scale. hidden must be HB_MLP_HIDDEN and n_out 4 HB_SUR_BLOCKS. */
+typedef struct hb_mlp {
+    float* params;
+    float* adam_m;
+    float* adam_v;
+    const float* norm;
+    int32_t hidden, n_out;
+} hb_mlp;
+
+/* One reused pulse for hb_surrogate_maps: blocks: (HB_SUR_BLOCKS, 2) float64 each block's length tau [s] and
+ * b0_offset area [Hz s]; g: the pulse's (constant) gradient [Hz/m]; pid: the pulse's index in maps. */
+typedef struct hb_sur_pulse {
+    const double* blocks;
+    double g[3];
+    int32_t pid, pad;
+} hb_sur_pulse;
+
+/* Training data: for n inputs x (n, 3) (df, u, b1), the exact interaction-frame block quaternions y (n, 4 M),
+ * products of the substeps' rotations in float64. table as hb_pulse_maps' (one pulse, rows from 0); bounds:
+ * (M + 1,) the blocks' first substeps and the end. */
+int hb_spinor_blocks(const double* table, const int32_t* bounds, const float* x, int32_t n, float* y, void* stream);
+
+/* Floats of workspace hb_mlp_grad and hb_mlp_train_step need for a batch of n. */
+int64_t hb_mlp_workspace(int32_t n);
+
+/* grad (HB_MLP_PARAMS,): the gradient of the mean squared error between the network's outputs on (x - centre) /
+ * half-width and (y - mean) / scale, over the n samples and n_out outputs; per-chunk sums added in a fixed order. */
+int hb_mlp_grad(const hb_mlp* net, const float* x, const float* y, int32_t n, float* work, float* grad, void* stream);
+
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected for step = 1, 2, ...) on that gradient. */
+int hb_mlp_train_step(const hb_mlp* net, const float* x, const float* y, int32_t n, float* work, int32_t step,
+                      float lr, void* stream);
+
+/* y (n, n_out): the network's de-normalised outputs mean + scale * net((x - centre) / half-width). */
+int hb_mlp_eval(const hb_mlp* net, const float* x, int32_t n, float* y, void* stream);
+
+/* Maps of pulse sp->pid for every spin from the network (the maps format of hb_pulse_maps), and outside[spin] = 1
+ * for spins (< n) whose inputs lie outside the trained box (their maps are to be composed exactly with
+ * hb_pulse_maps' mask), else 0. */
+int hb_surrogate_maps(const hb_case* cs, const hb_mlp* net, const hb_sur_pulse* sp, float* maps, int32_t* outside,
+                      void* stream);
 
 /* One interval of the adjoint (replaces bloch_replay + bloch_adjoint_c{NC}, cuda/replay.cu.in:3-33,
  * cuda/adjoint.cu.in:5-121; one call per interval of adjoint.py:179-220, last interval first). ckpt: (6, n_pad)

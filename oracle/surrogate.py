@@ -1,0 +1,186 @@
+"""FP64 CPU oracle of the learned RF surrogate (a path with no counterpart in the reference).
+
+TEST INFRASTRUCTURE. Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU leg may import or run
+anything under ``oracle/``, and only as the checker: the package ``blochparty_amd`` never imports it.
+
+The surrogate replaces the per-spin composition of a reused pulse's substeps (the reference's ``pulse_maps``,
+src/blochparty/engines/gpu/cuda/pulse.cu.in:3-32) by a network. A pulse of K substeps is cut into M blocks; each
+block's pure rotation (no relaxation) is a unit quaternion, taken in the frame that turns with the off-resonance
+precession (the rotation about z by the block's free-precession angle theta = -2 pi f tau, half before and half
+after, is divided out), so that it varies slowly with the offset f. The network maps (df, u, b1) -> the M block
+quaternions; the relaxation is put back exactly, half a block before and after each block's rotation (a Strang
+split of the pulse into M blocks). What each function computes:
+
+- ``substep_quat``  the rotation of one substep, w = -2 pi dt (b1 rf_x, b1 rf_y, f) (as core.rf_segment's rotation)
+- ``block_quats``   the M interaction-frame block quaternions of a pulse at offsets f and B1 scales b1
+- ``block_map``     the affine map m -> m + D m + v of a pulse from its block quaternions (relaxation split around
+                    each block), the maps format of the reused-pulse path
+- ``mlp_forward`` / ``mlp_grad`` / ``adam``: the network 3 -> H -> H -> 4 M (tanh), its mean-squared-error gradient,
+                    and Adam with bias correction
+
+How each function is pinned (tests/test_surrogate_oracle.py):
+- ``substep_quat`` / ``block_quats``: quaternion rotation = core.rotate (a Rodrigues restatement pinned against SciPy)
+  to 1e-12 for complex RF and off-resonance; unit norm; a hard pulse on resonance (closed form cos, sin of flip/2);
+- ``block_map``: with one substep per block (M = K) the split is the exact substep sequence, so it equals
+  core.rf_segment applied K times to 1e-12 (with relaxation); its error at M = 10 against that is under 1e-5
+  (the Strang split error, second order: halving short blocks' length divides it by about four);
+- ``mlp_grad``: central finite differences of ``mlp_forward``'s loss to 1e-7 relative, with a planted mistake (the
+  tanh derivative dropped) that this check catches;
+- ``adam``: torch.optim.Adam (the library routine) to 1e-12.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+TWO_PI = 2.0 * np.pi
+
+
+def qmul(p: np.ndarray, q: np.ndarray) -> np.ndarray:
+    """Hamilton product p q of (..., 4) quaternions (w, x, y, z): the rotation q, then p."""
+    pw, px, py, pz = np.moveaxis(p, -1, 0)
+    qw, qx, qy, qz = np.moveaxis(q, -1, 0)
+    return np.stack([pw * qw - px * qx - py * qy - pz * qz,
+                     pw * qx + px * qw + py * qz - pz * qy,
+                     pw * qy - px * qz + py * qw + pz * qx,
+                     pw * qz + px * qy - py * qx + pz * qw], -1)
+
+
+def quat_rotate(q: np.ndarray, m: np.ndarray) -> np.ndarray:
+    """The right-handed rotation of unit quaternion q applied to vectors m (..., 3)."""
+    return np.einsum("...ij,...j->...i", quat_matrix(q), m)
+
+
+def quat_matrix(q: np.ndarray) -> np.ndarray:
+    """(..., 3, 3) rotation matrix of unit quaternions (..., 4)."""
+    w, x, y, z = np.moveaxis(q, -1, 0)
+    return np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+                     np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+                     np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], -2)
+
+
+def rot_quat(wx, wy, wz) -> np.ndarray:
+    """Unit quaternion of the right-handed rotation by |w| about w."""
+    phi = np.sqrt(wx * wx + wy * wy + wz * wz)
+    s = np.where(phi > 0, np.sin(0.5 * phi) / np.where(phi > 0, phi, 1.0), 0.5)
+    return np.stack([np.cos(0.5 * phi), s * wx, s * wy, s * wz], -1)
+
+
+def substep_quat(dt, rf_x, rf_y, f, b1) -> np.ndarray:
+    """One RF substep's rotation: w = -2 pi dt (b1 rf_x, b1 rf_y, f) (core.py:124-150's rotation vector with the
+    spin's total offset f in Hz)."""
+    w = -TWO_PI * dt
+    f = np.asarray(f, dtype=np.float64)
+    b1 = np.asarray(b1, dtype=np.float64)
+    return rot_quat(w * b1 * rf_x + 0 * f, w * b1 * rf_y + 0 * f, w * f + 0 * b1)
+
+
+def block_bounds(k: int, m: int) -> np.ndarray:
+    """The M + 1 substep indices that cut K substeps into M blocks of near-equal length."""
+    return np.round(np.linspace(0, k, m + 1)).astype(np.int64)
+
+
+def block_quats(table: np.ndarray, bounds: np.ndarray, f, b1) -> np.ndarray:
+    """(N, M, 4) interaction-frame quaternions of the pulse ``table`` ((K, >=3) rows dt, rf_x, rf_y, and b0_offset
+    in column 6 when present) at spin offsets f [Hz] and B1 scales b1 (N,): block j's rotation Q_j (substeps
+    bounds[j]..bounds[j+1]-1 composed in time order, substep k's offset f + b0_offset_k) as
+    Rz(-theta/2) Q_j Rz(-theta/2), theta = -2 pi sum_k dt_k (f + b0_offset_k) its free precession angle."""
+    f = np.asarray(f, dtype=np.float64)
+    b1 = np.asarray(b1, dtype=np.float64)
+    n, m = len(f), len(bounds) - 1
+    b0 = table[:, 6] if table.shape[1] > 6 else np.zeros(len(table))
+    out = np.zeros((n, m, 4))
+    for j in range(m):
+        q = np.zeros((n, 4))
+        q[:, 0] = 1.0
+        for k in range(bounds[j], bounds[j + 1]):
+            q = qmul(substep_quat(table[k, 0], table[k, 1], table[k, 2], f + b0[k], b1), q)
+        tau = table[bounds[j] : bounds[j + 1], 0].sum()
+        area = (table[bounds[j] : bounds[j + 1], 0] * b0[bounds[j] : bounds[j + 1]]).sum()
+        half = rot_quat(0 * f, 0 * f, 0.5 * TWO_PI * (f * tau + area))  # Rz(-theta / 2)
+        out[:, j] = qmul(half, qmul(q, half))
+    return out
+
+
+def block_map(quats: np.ndarray, taus: np.ndarray, f, pd, r1, r2, areas=None) -> tuple[np.ndarray, np.ndarray]:
+    """The affine map of a pulse from its (N, M, 4) interaction-frame block quaternions: D (N, 3, 3) and v (N, 3)
+    with m' = m + D m + v; block j is relax(tau_j / 2), Rz(theta/2) Q~_j Rz(theta/2), relax(tau_j / 2),
+    theta = -2 pi (f tau_j + areas_j) (areas: the blocks' b0_offset areas, default 0)."""
+    f, pd, r1, r2 = (np.asarray(a, dtype=np.float64) for a in (f, pd, r1, r2))
+    areas = np.zeros(len(taus)) if areas is None else areas
+    n, m = quats.shape[:2]
+    A = np.broadcast_to(np.eye(3), (n, 3, 3)).copy()
+    c = np.zeros((n, 3))
+    for j in range(m):
+        e1, e2 = np.exp(-0.5 * taus[j] * r1), np.exp(-0.5 * taus[j] * r2)
+        E = np.zeros((n, 3, 3))
+        E[:, 0, 0] = E[:, 1, 1] = e2
+        E[:, 2, 2] = e1
+        rec = np.zeros((n, 3))
+        rec[:, 2] = pd * (1 - e1)
+        half = rot_quat(0 * f, 0 * f, -0.5 * TWO_PI * (f * taus[j] + areas[j]))  # Rz(theta / 2)
+        q = quats[:, j] / np.linalg.norm(quats[:, j], axis=-1, keepdims=True)
+        R = quat_matrix(qmul(half, qmul(q, half)))
+        for step in (E, R, E):
+            A = step @ A
+            c = np.einsum("nij,nj->ni", step, c)
+            if step is E:
+                c = c + rec
+    return A - np.eye(3), c
+
+
+# -- the network: 3 -> H -> H -> n_out, tanh, as flat FP64 parameters W1 (H, 3), b1 (H), W2 (H, H), b2 (H),
+#    W3 (n_out, H), b3 (n_out) (the order of the device's parameter vector)
+
+def mlp_unpack(p: np.ndarray, h: int, n_out: int):
+    sizes = [(h, 3), (h,), (h, h), (h,), (n_out, h), (n_out,)]
+    out, o = [], 0
+    for s in sizes:
+        k = int(np.prod(s))
+        out.append(p[o : o + k].reshape(s))
+        o += k
+    assert o == len(p)
+    return out
+
+
+def mlp_size(h: int, n_out: int) -> int:
+    return h * 3 + h + h * h + h + n_out * h + n_out
+
+
+def mlp_forward(p: np.ndarray, x: np.ndarray, h: int, n_out: int):
+    """Outputs (N, n_out) and the activations (a1, a2) of normalised inputs x (N, 3)."""
+    W1, b1, W2, b2, W3, b3 = mlp_unpack(p, h, n_out)
+    a1 = np.tanh(x @ W1.T + b1)
+    a2 = np.tanh(a1 @ W2.T + b2)
+    return a2 @ W3.T + b3, (a1, a2)
+
+
+def mlp_loss(p, x, y, h, n_out) -> float:
+    """Mean over samples and outputs of the squared error."""
+    out, _ = mlp_forward(p, x, h, n_out)
+    return float(((out - y) ** 2).mean())
+
+
+def mlp_grad(p: np.ndarray, x: np.ndarray, y: np.ndarray, h: int, n_out: int, tanh_derivative: bool = True):
+    """Gradient of ``mlp_loss`` wrt the flat parameters (backpropagation). ``tanh_derivative=False`` plants a
+    mistake (the activation derivative dropped) for the check that pins this function."""
+    W1, b1, W2, b2, W3, b3 = mlp_unpack(p, h, n_out)
+    out, (a1, a2) = mlp_forward(p, x, h, n_out)
+    d3 = 2.0 * (out - y) / out.size
+    d2 = d3 @ W3
+    if tanh_derivative:
+        d2 = d2 * (1 - a2 * a2)
+    d1 = d2 @ W2
+    if tanh_derivative:
+        d1 = d1 * (1 - a1 * a1)
+    return np.concatenate([(d1.T @ x).ravel(), d1.sum(0), (d2.T @ a1).ravel(), d2.sum(0), (d3.T @ a2).ravel(),
+                           d3.sum(0)])
+
+
+def adam(p, g, m, v, t: int, lr: float, beta1=0.9, beta2=0.999, eps=1e-8):
+    """One Adam step (Kingma & Ba, with bias correction), returning new (p, m, v); t counts from 1."""
+    m = beta1 * m + (1 - beta1) * g
+    v = beta2 * v + (1 - beta2) * g * g
+    mh = m / (1 - beta1**t)
+    vh = v / (1 - beta2**t)
+    return p - lr * mh / (np.sqrt(vh) + eps), m, v

@@ -1,0 +1,202 @@
+"""The learned RF surrogate on the GPU: its kernels against the FP64 oracle (oracle/surrogate.py), the trained
+network against the exact composition (the judge), the end-to-end signal against the FP64 core oracle, and an
+ablation (what training buys; what the network saves over composing the substeps)."""
+
+import numpy as np
+import pytest
+import torch
+
+import blochparty_amd as bp
+from blochparty_amd import workloads as W
+from blochparty_amd.engines import driver, pulses, surrogate as sur
+from cases import rel
+from oracle import core
+from oracle import surrogate as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def slab():
+    """The 3D GRE's reused slab pulse (2 ms sinc, TBW 4, 200 substeps) as the surrogate sees it, on a 16^3 grid."""
+    ph, seq, coils = W.gre3d(16)
+    reused = pulses.reused_pulses(seq)
+    table, index = pulses.pulse_table(seq, reused)
+    off, k = index[0]
+    return ph, seq, coils, sur.shape_of(table[off : off + k])
+
+
+def box_inputs(box, n, seed):
+    g = np.random.default_rng(seed)
+    c, h = np.array(box[0]), np.array(box[1])
+    return ((g.uniform(-1, 1, (n, 3)) * h + c)).astype(np.float32)
+
+
+def test_spinor_blocks_match_the_oracle(gpu, slab):
+    *_, shape = slab
+    x = box_inputs(((0.0, 0.0, 1.0), (150.0, 1200.0, 0.3)), 2000, 0)
+    y = sur.spinor_blocks(shape, torch.from_numpy(x).cuda()).cpu().numpy()
+    xd = x.astype(np.float64)
+    ref = S.block_quats(shape.table, shape.bounds, xd[:, 0] + xd[:, 1] + 0.0, xd[:, 2]).reshape(len(x), -1)
+    assert np.abs(y - ref).max() < 2e-7  # float64 products rounded to float32
+
+
+def random_net(seed=1):
+    box = ((3.0, -20.0, 1.0), (120.0, 900.0, 0.25))
+    rng = np.random.default_rng(seed)
+    mu = torch.from_numpy(rng.normal(size=sur.N_OUT)).float()
+    sd = torch.from_numpy(rng.uniform(0.01, 0.5, sur.N_OUT)).float()
+    net = sur.new_net(box, mu.cuda(), sd.cuda(), seed, "cuda")
+    net.params.copy_(torch.from_numpy(rng.normal(size=sur.N_PARAMS) * 0.3).float())
+    return net, box, mu.double().numpy(), sd.double().numpy()
+
+
+def test_mlp_eval_and_gradient_match_the_oracle(gpu):
+    net, box, mu, sd = random_net()
+    n = 3000  # more than one 512-sample chunk and a ragged last one
+    x = box_inputs(box, n, 2)
+    y = np.random.default_rng(3).normal(size=(n, sur.N_OUT)).astype(np.float32)
+    p = net.params.double().cpu().numpy()
+    xn = (x.astype(np.float64) - np.array(box[0])) / np.array(box[1])
+    out, _ = S.mlp_forward(p, xn, sur.HIDDEN, sur.N_OUT)
+    mine = sur.evaluate(net, torch.from_numpy(x).cuda()).double().cpu().numpy()
+    assert rel(torch.from_numpy(mine), torch.from_numpy(mu + sd * out)) < 1e-5
+    g = sur.gradient(net, torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()).double().cpu().numpy()
+    ref = S.mlp_grad(p, xn, (y - mu) / sd, sur.HIDDEN, sur.N_OUT)
+    assert np.abs(g - ref).max() < 2e-5 * np.abs(ref).max()
+    # deterministic: the same gradient bit for bit
+    g2 = sur.gradient(net, torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()).double().cpu().numpy()
+    assert np.array_equal(g, g2)
+
+
+def test_adam_steps_match_the_oracle(gpu):
+    import ctypes
+
+    from blochparty_amd.engines import hip
+
+    net, box, mu, sd = random_net(4)
+    n = 1024
+    x = torch.from_numpy(box_inputs(box, n, 5)).cuda()
+    y = torch.from_numpy(np.random.default_rng(6).normal(size=(n, sur.N_OUT)).astype(np.float32)).cuda()
+    lib = hip.load()
+    work = torch.empty(lib.hb_mlp_workspace(n), dtype=torch.float32, device="cuda")
+    p = net.params.double().cpu().numpy()
+    m = v = np.zeros_like(p)
+    for t, lr in ((1, 1e-3), (2, 5e-4), (3, 2e-4)):
+        g = sur.gradient(net, x, y).double().cpu().numpy()
+        p, m, v = S.adam(p, g, m, v, t, lr)
+        rc = lib.hb_mlp_train_step(ctypes.byref(net.struct()), x.data_ptr(), y.data_ptr(), n, work.data_ptr(), t, lr,
+                                   sur.stream())
+        assert rc == 0
+        mine = net.params.double().cpu().numpy()
+        assert np.abs(mine - p).max() < 1e-6, t
+        p = mine  # continue from the device's float32 parameters
+
+
+@pytest.fixture(scope="module")
+def trained(gpu, slab):
+    ph, seq, coils, shape = slab
+    case = driver.prepare_case(ph, seq, coils, reuse=True)
+    box = sur.input_box(case.spins, case.pos_lo, case.n, shape.g)
+    return sur.train(shape, box, dev="cuda"), box, shape
+
+
+def test_trained_network_against_the_exact_maps(trained):
+    net, box, shape = trained
+    print(f"trained in {net.train_s:.1f} s, held-out loss {net.final_loss:.2e}")
+    assert net.final_loss < 1e-3
+    x = box_inputs(box, 4000, 7)
+    xd = x.astype(np.float64)
+    f = xd[:, 0] + xd[:, 1]
+    q = sur.evaluate(net, torch.from_numpy(x).cuda()).double().cpu().numpy().reshape(len(x), -1, 4)
+    rng = np.random.default_rng(8)
+    pd, r1, r2 = rng.uniform(0.7, 1.0, len(x)), 1 / rng.uniform(0.8, 4.0, len(x)), 1 / rng.uniform(0.08, 2, len(x))
+    D, v = S.block_map(q, shape.blocks[:, 0], f, pd, r1, r2, shape.blocks[:, 1])
+    exact_q = S.block_quats(shape.table, shape.bounds, f, xd[:, 2])
+    De, ve = S.block_map(exact_q, shape.blocks[:, 0], f, pd, r1, r2, shape.blocks[:, 1])
+    err = max(np.abs(D - De).max(), np.abs(v - ve).max())
+    print(f"map error on {len(x)} box inputs: {err:.2e}")
+    assert err < 1e-3
+
+
+def test_surrogate_simulation_matches_the_oracle(trained, slab):
+    ph, seq, coils, _ = slab
+    seq4 = W.first_trs(seq, 4)
+    res = bp.simulate(ph, seq4, coils, engine="gpu", pulses="surrogate")
+    rep = res.surrogate
+    print(rep)
+    assert rep["covered"] == 1 and not rep["rejected"] and rep["pulses"][0]["outside_box"] == 0
+    assert rep["pulses"][0]["judge_max_map_err"] < 1e-3
+    ref_sig, ref_m = core.simulate(ph, seq4, coils)
+    e_sig = rel(res.signal, ref_sig)
+    exact = bp.simulate(ph, seq4, coils, engine="gpu", pulses="reuse")
+    print(f"k-space error vs oracle: surrogate {e_sig:.2e}, exact maps {rel(exact.signal, ref_sig):.2e}")
+    assert e_sig < 1e-4
+    assert rel(res.m, ref_m) < 1e-3
+
+
+def test_spins_outside_the_box_fall_back_to_the_exact_maps(trained, slab):
+    ph, seq, coils, _ = slab
+    seq2 = W.first_trs(seq, 2)
+    far = bp.Phantom(pos=ph.pos.clone(), pd=ph.pd, t1=ph.t1, t2=ph.t2, db0=ph.db0.clone())
+    far.db0[::3] += 5000.0  # far outside any box trained on the unshifted phantom
+    case = driver.prepare_case(far, seq2, coils, surrogate=True)
+    driver.run_forward(case)
+    out = driver.read_result(case)
+    rep = out.surrogate["pulses"][0]
+    assert rep["outside_box"] == 0  # the box follows the phantom: a new network covers the shifted spins
+    ex = bp.simulate(far, seq2, coils, engine="gpu", pulses="reuse")
+    assert rel(out.signal, ex.signal) < 1e-3
+    # with the network of the unshifted phantom the shifted spins lie outside its box: exact maps for them
+    net, box, shape = trained
+    case.extra["surrogate"] = [(0, shape, net)]
+    driver.run_forward(case)
+    out2 = driver.read_result(case)
+    assert out2.surrogate["pulses"][0]["outside_box"] == (len(far) + 2) // 3
+    assert rel(out2.signal, ex.signal) < 1e-3
+
+
+def test_ablation_training_and_cost(trained, slab):
+    """What the trained network gains: against the same network untrained, the map error; against composing the
+    200 substeps per spin exactly (the method without it), the time to form every spin's maps."""
+    ph, seq, coils, _ = slab
+    net, box, shape = trained
+    case = driver.prepare_case(ph, seq, coils, surrogate=True)
+    driver.run_forward(case)
+    trained_err = driver.read_result(case).surrogate["pulses"][0]["judge_max_map_err"]
+    mu, sd = net.norm[6 : 6 + sur.N_OUT], net.norm[6 + sur.N_OUT :]
+    raw = sur.new_net(box, mu, sd, 0, "cuda")
+    case.extra["surrogate"] = [(0, shape, raw)]
+    driver.run_forward(case)
+    raw_err = driver.read_result(case).surrogate["pulses"][0]["judge_max_map_err"]
+    print(f"judge map error: trained {trained_err:.2e}, untrained {raw_err:.2e}")
+    assert raw_err > 100 * trained_err
+    # time: network maps vs exact substep composition over a 1 M-spin case
+    big_ph, big_seq, big_c = W.gre3d(64)
+    big = driver.prepare_case(big_ph, W.first_trs(big_seq, 2), big_c, reuse=True)
+    from blochparty_amd.engines import hip
+    import ctypes
+
+    lib = hip.load()
+    st = big.struct()
+    outside = torch.zeros(big.n_pad, dtype=torch.int32, device="cuda")
+    blocks = torch.from_numpy(shape.blocks).cuda()
+    sp = sur.HbSurPulse(blocks.data_ptr(), (ctypes.c_double * 3)(*shape.g), 0, 0)
+
+    def timed(fn):
+        fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(3):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / 3
+
+    t_exact = timed(lambda: lib.hb_pulse_maps(ctypes.byref(st), big.pulse_table.data_ptr(), big.pulse_index.data_ptr(),
+                                              big.n_pulses, None, big.maps.data_ptr(), sur.stream()))
+    t_net = timed(lambda: lib.hb_surrogate_maps(ctypes.byref(st), ctypes.byref(net.struct()), ctypes.byref(sp),
+                                                big.maps.data_ptr(), outside.data_ptr(), sur.stream()))
+    print(f"maps of {big.n} spins: exact composition {t_exact:.3f} ms, network {t_net:.3f} ms "
+          f"({t_exact / t_net:.1f}x)")
+    assert t_net < t_exact
