@@ -1,6 +1,6 @@
 """Benchmark of the GPU path on one MI355X: spin-samples/s of simulate(..., engine="gpu") on the 3D GRE 64^3 workload.
 
-    python bench.py [--steps K] [--warmup W] [--n 64] [--no-cpu] [--no-adjoint]
+    python bench.py [--steps K] [--warmup W] [--n 64] [--no-cpu] [--no-adjoint] [--no-exact] [--copy-probe MIB]
 
 Workload: 3D GRE n^3 x 4 sub-voxel isochromats (1,048,576 spins at n = 64), a three-class smooth random phantom,
 +-100 Hz dB0, 8 coils, a 2 ms sinc slab pulse (TBW 4, 200 substeps) every TR of 8 ms, n^2 lines of n samples.
@@ -8,19 +8,22 @@ Every input is resident in HBM before a timed region starts (host packing and up
 region is bracketed by torch.cuda.synchronize(). Throughput = spins x ADC samples x steps / seconds (the reference's
 metric, benchmarks/bench.py:65).
 
-Three legs, K timed steps each after W warmup steps:
-- forward with reused pulses (pulses="reuse": the sinc pulse composed once per spin into its affine map, in float64,
-  inside the timed step) -- the JSON line's ``value``;
+Legs, K timed steps each after W warmup steps:
+- forward with the learned RF surrogate (pulses="surrogate"): every spin's pulse map from the network, the exact
+  maps of every 257th spin composed and compared (the judge), then the event loop -- the JSON line's ``value``. The
+  network is trained once before the timed steps (its training time is reported, not timed);
+- forward with reused exact maps (pulses="reuse": the sinc pulse composed once per spin in float64);
 - forward stepping every RF substep (the exact-RF baseline);
 - forward + adjoint of L = sum |signal|^2 w.r.t. the flip angle (through the per-segment RF gradients), the T1 map
   and the coil maps (the reference's --grad case, benchmarks/bench.py:69-95), exact RF: the forward with checkpoints,
   dL/dsignal on the device, and the reverse sweep.
 Every forward-kernel launch of the value's leg is bracketed by HIP events on the stream it runs on; the roofline's
-``achieved`` is the algorithmic flops per launch over their mean duration.
+``achieved`` is the algorithmic flops per launch over their mean duration. ``traffic`` is the HBM bytes per launch
+of the same kernel from the committed rocprofv3 PMC summary (profiles/), when present.
 
 The CPU baseline is the FP64 oracle timed on this host's cores (the container quota) on a bounded sample of the same
-workload (a seeded subset of spins over the first TRs); the GPU runs the same sample, both RF paths, and reports their
-relative errors against the oracle. One JSON line goes to stdout; progress goes to stderr.
+workload (a seeded subset of spins over the first TRs); the GPU runs the same sample on every RF path and reports
+their relative errors against the oracle. One JSON line goes to stdout; progress goes to stderr.
 """
 
 from __future__ import annotations
@@ -87,6 +90,45 @@ def timed(fn, steps: int, warmup: int) -> float:
     return time.perf_counter() - t0
 
 
+PMC_SUMMARY = "profiles/forward_kernel_gre3d64_reuse_pmc.txt"
+
+
+def profiled_traffic() -> float | None:
+    """HBM bytes per forward-kernel launch from the committed PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy
+    of known size), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), PMC_SUMMARY)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch forward_kernel:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def forward_leg(case, lib, driver, steps: int, warmup: int, work: int, label: str) -> dict:
+    """Time ``steps`` forward passes of ``case``; every forward-kernel launch bracketed by events."""
+    events = []
+
+    def timer(_phase):
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        events.append(ev)
+
+    for _ in range(warmup):
+        driver.run_forward(case, lib)
+    t = timed(lambda: driver.run_forward(case, lib, timer=timer), steps, 0)
+    kernel_ms = [events[2 * i].elapsed_time(events[2 * i + 1]) for i in range(len(events) // 2)]
+    flops = dispatch_flops(case, driver)
+    mean_ms, mean_flops = float(np.mean(kernel_ms)), float(np.mean(flops))
+    out = {"value": work * steps / t, "ms_per_step": 1e3 * t / steps, "kernel_mean_ms": mean_ms,
+           "launches": len(kernel_ms), "flops_per_launch": mean_flops,
+           "kernel_tflops": mean_flops / (mean_ms * 1e-3) / 1e12}
+    log(f"forward ({label}): {out['ms_per_step']:.1f} ms/step, {out['value']:.3e} spin-samples/s; forward kernel "
+        f"mean {mean_ms:.3f} ms over {len(kernel_ms)} launches, {out['kernel_tflops']:.1f} TFLOP/s algorithmic")
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +138,8 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity sample")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint leg")
     ap.add_argument("--no-exact", action="store_true", help="skip the exact-RF forward leg")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the surrogate leg (the value is then the reused exact maps' forward)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the legs, three elementwise copies of MIB MiB (read + write) that calibrate the HBM "
                          "byte counters of a profiler run")
@@ -111,63 +155,48 @@ def main() -> None:
     import blochparty_amd as bp
     from blochparty_amd import workloads as W
     from blochparty_amd.engines import adjoint, driver, hip
+    from blochparty_amd.engines import surrogate as sur
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     hip.load()
     lib = hip.HIP.lib()
     t_build = time.perf_counter()
     ph, seq, coils = W.gre3d(args.n)
+    S = len(seq)
+    log(f"workload built in {time.perf_counter() - t_build:.1f} s: N={len(ph)} S={S} n_adc={seq.n_adc} "
+        f"C={coils.shape[1]}")
+    work = len(ph) * seq.n_adc
+
+    # -- the surrogate leg (the value) and the reused exact maps' leg
+    surrogate = None
+    if not args.no_surrogate:
+        t0 = time.perf_counter()
+        case_s = driver.prepare_case(ph, seq, coils, surrogate=True)
+        torch.cuda.synchronize()
+        nets = case_s.extra["surrogate"]
+        log(f"surrogate: case prepared and {len(nets)} network(s) trained in {time.perf_counter() - t0:.1f} s")
+        leg_s = forward_leg(case_s, lib, driver, args.steps, args.warmup, work, "learned RF surrogate")
+        surrogate = {"forward": leg_s, "judge": sur.report(case_s)}
+        del case_s
+        torch.cuda.empty_cache()
     case_r = driver.prepare_case(ph, seq, coils, reuse=True)
-    torch.cuda.synchronize()
-    N, n_adc, S = case_r.n, case_r.n_adc, len(seq)
-    work = N * n_adc
-    log(f"workload built and uploaded in {time.perf_counter() - t_build:.1f} s: N={N} S={S} n_adc={n_adc} "
-        f"C={case_r.n_coils}, {case_r.n_pulses} reused pulse(s), {len(case_r.starts)} dispatches")
-
-    # -- leg 1: forward, reused pulses (the value)
-    events = []
-
-    def timer(phase):
-        ev = torch.cuda.Event(enable_timing=True)
-        ev.record()
-        events.append(ev)
-
-    for _ in range(args.warmup):
-        driver.run_forward(case_r, lib)
-    t = timed(lambda: driver.run_forward(case_r, lib, timer=timer), args.steps, 0)
-    kernel_ms = [events[2 * i].elapsed_time(events[2 * i + 1]) for i in range(len(events) // 2)]
-    flops = dispatch_flops(case_r, driver)
-    mean_ms, mean_flops = float(np.mean(kernel_ms)), float(np.mean(flops))
-    achieved = mean_flops / (mean_ms * 1e-3) / 1e12
-    value = work * args.steps / t
-    ms_step = 1e3 * t / args.steps
-    log(f"forward (reused pulses): {ms_step:.1f} ms/step, {value:.3e} spin-samples/s; forward kernel mean "
-        f"{mean_ms:.3f} ms over {len(kernel_ms)} launches, {achieved:.1f} TFLOP/s algorithmic")
-    case_r.partial = case_r.maps = None
+    leg_r = forward_leg(case_r, lib, driver, args.steps, args.warmup, work, "reused exact maps")
     del case_r
     torch.cuda.empty_cache()
+    lead = surrogate["forward"] if surrogate else leg_r
 
-    # -- leg 2: forward, exact RF
+    # -- forward, exact RF
     exact = None
     if not args.no_exact:
         case_e = driver.prepare_case(ph, seq, coils)
-        ev_e = []
-        t_e = timed(lambda: driver.run_forward(case_e, lib, timer=lambda _: (ev_e.append(torch.cuda.Event(
-            enable_timing=True)), ev_e[-1].record())), args.steps, args.warmup)
-        k_e = [ev_e[2 * i].elapsed_time(ev_e[2 * i + 1]) for i in range(len(ev_e) // 2)][-len(case_e.starts):]
-        fl_e = float(np.mean(dispatch_flops(case_e, driver)))
-        exact = {"value": work * args.steps / t_e, "ms_per_step": 1e3 * t_e / args.steps,
-                 "kernel_mean_ms": float(np.mean(k_e)),
-                 "kernel_tflops": fl_e / (float(np.mean(k_e)) * 1e-3) / 1e12}
-        log(f"forward (exact RF): {exact['ms_per_step']:.1f} ms/step, {exact['value']:.3e} spin-samples/s; kernel "
-            f"{exact['kernel_mean_ms']:.3f} ms, {exact['kernel_tflops']:.1f} TFLOP/s; reuse speedup "
-            f"{value / exact['value']:.2f}x")
+        exact = forward_leg(case_e, lib, driver, args.steps, args.warmup, work, "exact RF")
         del case_e
         torch.cuda.empty_cache()
 
-    # -- leg 3: forward + adjoint, exact RF
+    # -- forward + adjoint, exact RF
     fwd_adj = None
     if not args.no_adjoint:
+        n_adc = seq.n_adc
         every = adjoint.default_checkpoint_every(S)
         case_a = driver.prepare_case(ph, seq, coils, chunk=every, aligned=True)
         nc = next(c for c in adjoint.ADJOINT_COILS if c >= case_a.n_coils)
@@ -214,31 +243,35 @@ def main() -> None:
 
         cores = cpu_quota()
         torch.set_num_threads(cores)
-        ph_s, c_s = W.subset(ph, coils, min(args.cpu_spins, N))
+        ph_s, c_s = W.subset(ph, coils, min(args.cpu_spins, len(ph)))
         seq_s = W.first_trs(seq, args.cpu_trs)
         t1 = time.perf_counter()
         ref_sig, ref_m = core.simulate(ph_s, seq_s, c_s)
         t_cpu = time.perf_counter() - t1
         cpu = {"value": len(ph_s) * seq_s.n_adc / t_cpu, "unit": "spin·samples/s", "cores": cores, "kind": "port",
-               "sample": f"FP64 oracle, {len(ph_s)} of the {N} spins (seeded subset) over the first {args.cpu_trs} "
-                         f"TRs ({seq_s.n_adc} samples, {len(seq_s)} segments) of the same sequence; {t_cpu:.1f} s"}
+               "sample": f"FP64 oracle, {len(ph_s)} of the {len(ph)} spins (seeded subset) over the first "
+                         f"{args.cpu_trs} TRs ({seq_s.n_adc} samples, {len(seq_s)} segments) of the same sequence; "
+                         f"{t_cpu:.1f} s"}
         parity = {"sample": "the CPU-baseline sample"}
-        for name, opt in (("reuse", "reuse"), ("exact", "step")):
+        paths = (("surrogate", "surrogate"), ("reuse", "reuse"), ("exact", "step"))
+        for name, opt in paths[0 if surrogate else 1:]:
             o = bp.simulate(ph_s, seq_s, c_s, engine="gpu", pulses=opt)
             parity[f"{name}_signal_rel_err"] = float((o.signal.to(torch.complex128) - ref_sig).abs().max()
                                                      / ref_sig.abs().max())
             parity[f"{name}_m_rel_err"] = float((o.m.double() - ref_m).abs().max() / ref_m.abs().max())
         log(f"cpu oracle {cpu['value']:.3e} spin-samples/s on {cores} cores; GPU vs oracle on the sample: "
-            f"reuse {parity['reuse_signal_rel_err']:.2e}, exact {parity['exact_signal_rel_err']:.2e}")
+            + ", ".join(f"{k[:-15]} {v:.2e}" for k, v in parity.items() if k.endswith("signal_rel_err")))
 
+    rf_desc = ("learned surrogate maps (network per spin; exact maps of every 257th spin as the judge)" if surrogate
+               else "reused per-spin pulse maps (float64, exact)")
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": lead["value"],
         "unit": "spin·samples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": lead["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -246,26 +279,27 @@ def main() -> None:
         "data": "synthetic",
         "config": {
             "workload": f"3D GRE {args.n}^3 x 4 sub-voxel isochromats, three-class random phantom, +-100 Hz dB0, "
-                        f"2 ms sinc slab (TBW 4, 200 substeps), 8 coils, TR 8 ms; forward, reused pulse maps",
-            "spins": N, "segments": S, "n_adc": n_adc, "coils": int(coils.shape[1]),
-            "rf": "reused per-spin pulse maps (float64, exact)", "pass": "forward",
+                        f"2 ms sinc slab (TBW 4, 200 substeps), 8 coils, TR 8 ms; forward",
+            "spins": len(ph), "segments": S, "n_adc": seq.n_adc, "coils": int(coils.shape[1]),
+            "rf": rf_desc, "pass": "forward",
         },
         "roofline": {
             "bound": "valu",
             "kernel": "forward_kernel",
-            "achieved": achieved,
+            "achieved": lead["kernel_tflops"],
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
-            "mean_launch_ms": mean_ms,
-            "launches": len(kernel_ms),
-            "flops_per_launch": mean_flops,
+            "frac": lead["kernel_tflops"] / FP32_PEAK_TFLOPS,
+            "traffic": profiled_traffic(),
+            "mean_launch_ms": lead["kernel_mean_ms"],
+            "launches": lead["launches"],
+            "flops_per_launch": lead["flops_per_launch"],
         },
+        "surrogate": surrogate,
+        "reuse_exact_maps_forward": leg_r,
         "exact_rf_forward": exact,
-        "speedup_reuse_vs_exact": None if exact is None else value / exact["value"],
+        "speedup_vs_exact_rf": None if exact is None else lead["value"] / exact["value"],
         "fwd_adjoint": fwd_adj,
-        "surrogate": None,
         "cpu_baseline": cpu,
         "parity": parity,
     }

@@ -10,7 +10,7 @@
 //      the step (adjoint_device.h).
 // Per-spin gradients are summed in registers and added to float64 accumulators at every segment index that is a
 // multiple of 64; per-segment gradients are summed over each wave (butterfly), then over the workgroup's 4 waves in
-// LDS in a fixed order, one partial per workgroup and segment, and over workgroups in order by seg_reduce. Nothing
+// LDS in a fixed order, one partial per workgroup and segment, and over workgroups in a fixed order by seg_reduce. Nothing
 // depends on the interval length (a multiple of 64) or on scheduling: the gradients are the same bit for bit for
 // any checkpoint_every and every run.
 #include "adjoint_device.h"
@@ -26,6 +26,29 @@ struct Stash {
 struct StashXY {
     float x, y;
 };
+
+// Sum over the wave of 8 per-lane values (v[g], g < N_SEG_GRADS; the rest zero), in a fixed order, in 10 cross-lane
+// reads: a reduce-scatter over lane bits 5, 4, 3 (each step keeps half the values and adds the partner's copy of
+// them) leaves lane l with value (l >> 3) & 7 summed over the 8 lanes that differ in those bits; a butterfly over
+// bits 2, 1, 0 completes the sum, which lanes 8 g .. 8 g + 7 then hold for value g.
+__device__ __forceinline__ float wave_sum8(const float (&sg)[N_SEG_GRADS], int lane) {
+    float v[8];
+#pragma unroll
+    for (int g = 0; g < 8; ++g) v[g] = g < N_SEG_GRADS ? sg[g] : 0.0f;
+#pragma unroll
+    for (int h = 4, o = 32; h >= 1; h >>= 1, o >>= 1) {
+        const bool up = lane & o;
+#pragma unroll
+        for (int k = 0; k < h; ++k) {
+            const float send = up ? v[k] : v[k + h], keep = up ? v[k + h] : v[k];
+            v[k] = keep + __shfl_xor(send, o, 64);
+        }
+    }
+    float r = v[0];
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+    return r;
+}
 
 // Effective flags: cache hints honoured only while the cache holds the factors they refer to.
 __device__ __forceinline__ int live_flags(int f, bool free_ok, bool rf_ok, int same_free, int same_rf) {
@@ -54,6 +77,22 @@ __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint
                 r[j] = m.mx, r[np + j] = m.my, r[2 * np + j] = m.mz + m.lz;
             }
             const hb_segment q = cs.segs[s];
+            if (q.flags & HB_TRAIN) {  // an RF train: its own loop, the next record in flight (forward_kernel's)
+                const int L = min(q.pad[3], s1 - s);
+                RfRec cur = load_rf(&cs.segs[s]);
+                for (int i = 1; i <= L; ++i) {
+                    const RfRec nxt = load_rf(&cs.segs[s + (i < L ? i : 0)]);
+                    if (i > 1 && ((s + i - 1 - s0) & 7) == 0) {
+                        float* r = ad.r8 + (size_t)((s + i - 1 - s0) >> 3) * 3 * np;
+                        r[j] = m.mx, r[np + j] = m.my, r[2 * np + j] = m.mz + m.lz;
+                    }
+                    rf_step(cur, live_flags(cur.flags, true, rf_ok, 0, HB_SAME_RF_DT), p, b1sq, m, k);
+                    rf_ok = true;
+                    cur = nxt;
+                }
+                s += L - 1;
+                continue;
+            }
             const int f = live_flags(q.flags, free_ok, rf_ok, HB_SAME_FREE, HB_SAME_RF_DT);
             step(q, f, p, b1sq, m, k);
             if (f & HB_RF) rf_ok = true; else free_ok = true;
@@ -129,13 +168,8 @@ __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint
                 reverse_step<SEG>(q, fr, p, before[i].x, before[i].y, before[i].z, lx, ly, lz, acc, sg, rc);
                 if (f & HB_RF) rrf_ok = true; else rfree_ok = true;
                 if (SEG) {
-#pragma unroll
-                    for (int g = 0; g < N_SEG_GRADS; ++g) {
-                        float v = sg[g];
-#pragma unroll
-                        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-                        if (lane == 0) segbuf[wave][i][g] = v;
-                    }
+                    const float v = wave_sum8(sg, lane);
+                    if ((lane & 7) == 0) segbuf[wave][i][lane >> 3] = v;
                 }
             }
         }
@@ -182,15 +216,30 @@ __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint
     ad.lam[j] = lx, ad.lam[np + j] = ly, ad.lam[2 * np + j] = lz;
 }
 
-// seg[s0 + k][g] = sum over workgroups of gseg[b][k][g], in order, float64.
-__global__ __launch_bounds__(256) void seg_reduce_kernel(const float* __restrict__ gseg, int n_blocks, int k_max,
-                                                         int s0, int len, double* __restrict__ seg) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 8 * len) return;
-    const int k = t >> 3, g = t & 7;
-    double s = 0.0;
-    for (int b = 0; b < n_blocks; ++b) s += (double)gseg[((size_t)b * k_max + k) * 8 + g];
-    seg[(size_t)(s0 + k) * 8 + g] = s;
+// seg[s0 + k][g] = sum over workgroups of gseg[b][k][g], float64, in a fixed order: the workgroups in chunks of
+// kSegChunk summed in order by one lane per (k, g) (consecutive lanes read consecutive floats), then the chunks in
+// order.
+constexpr int kSegChunk = 256;
+
+__global__ __launch_bounds__(256) void seg_chunks_kernel(const float* __restrict__ gseg, int n_blocks, int k_max,
+                                                         int len, double* __restrict__ work) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 8 * len) return;
+    const int b0 = blockIdx.y * kSegChunk, b1 = min(n_blocks, b0 + kSegChunk);
+    const size_t stride = (size_t)k_max * 8;
+    double acc = 0.0;
+#pragma unroll 16
+    for (int b = b0; b < b1; ++b) acc += (double)gseg[b * stride + i];
+    work[blockIdx.y * stride + i] = acc;
+}
+
+__global__ __launch_bounds__(256) void seg_final_kernel(const double* __restrict__ work, int chunks, int k_max,
+                                                        int s0, int len, double* __restrict__ seg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 8 * len) return;
+    double acc = 0.0;
+    for (int c = 0; c < chunks; ++c) acc += work[(size_t)c * k_max * 8 + i];
+    seg[(size_t)s0 * 8 + i] = acc;
 }
 
 template <int NC>
@@ -222,12 +271,20 @@ extern "C" int hb_adjoint_interval(const hb_case* cs, const hb_adjoint* ad, int3
     }
 }
 
+extern "C" int64_t hb_seg_reduce_workspace(int32_t n_blocks, int32_t k_max) {
+    if (n_blocks < 1 || k_max < 0) return 0;
+    return (int64_t)((n_blocks + hb::kSegChunk - 1) / hb::kSegChunk) * k_max * 8;
+}
+
 extern "C" int hb_seg_reduce(const float* gseg, int32_t n_blocks, int32_t k_max, int32_t s0, int32_t len,
-                             double* seg, void* stream) {
+                             double* work, double* seg, void* stream) {
     using namespace hb;
-    if (n_blocks < 1 || len < 0 || len > k_max || s0 < 0) return hipErrorInvalidValue;
+    if (n_blocks < 1 || len < 0 || len > k_max || s0 < 0 || !work) return hipErrorInvalidValue;
     if (len == 0) return hipSuccess;
-    hipLaunchKernelGGL(seg_reduce_kernel, dim3((8 * len + 255) / 256), dim3(256), 0, (hipStream_t)stream, gseg,
-                       n_blocks, k_max, s0, len, seg);
+    const int chunks = (n_blocks + kSegChunk - 1) / kSegChunk;
+    hipLaunchKernelGGL(seg_chunks_kernel, dim3((8 * len + 255) / 256, chunks), dim3(256), 0, (hipStream_t)stream,
+                       gseg, n_blocks, k_max, len, work);
+    hipLaunchKernelGGL(seg_final_kernel, dim3((8 * len + 255) / 256), dim3(256), 0, (hipStream_t)stream, work,
+                       chunks, k_max, s0, len, seg);
     return hipGetLastError();
 }

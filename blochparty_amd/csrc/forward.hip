@@ -172,20 +172,35 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
     store_state(cs.state, np, j, m);
 }
 
-// sig[row0 + r][c] = exp(-i psi) sum_b partial[b][r][c], float64, blocks in order.
-__global__ __launch_bounds__(256) void reduce_kernel(const float* __restrict__ partial, int n_blocks, int rows,
-                                                      int vals, int n_coils, const double* __restrict__ psi, int row0,
-                                                      double* __restrict__ sig) {
+// The cross-block sum in two passes, both in a fixed order (deterministic): the blocks in chunks of kRedChunk,
+// summed in float64 in block order by one lane per partial value (consecutive lanes read consecutive floats); then
+// the chunks in order, demodulated: sig[row0 + r][c] = exp(-i psi) sum_chunks sum_b partial[b][r][c].
+constexpr int kRedChunk = 256;
+
+__global__ __launch_bounds__(256) void reduce_chunks_kernel(const float* __restrict__ partial, int n_blocks,
+                                                            int per_block, double* __restrict__ work) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= per_block) return;
+    const int b0 = blockIdx.y * kRedChunk, b1 = min(n_blocks, b0 + kRedChunk);
+    const float* src = partial + i;
+    double acc = 0.0;
+#pragma unroll 16
+    for (int b = b0; b < b1; ++b) acc += (double)src[(size_t)b * per_block];
+    work[(size_t)blockIdx.y * per_block + i] = acc;
+}
+
+__global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restrict__ work, int chunks, int rows,
+                                                           int vals, int n_coils, const double* __restrict__ psi,
+                                                           int row0, double* __restrict__ sig) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= rows * n_coils) return;
     const int r = idx / n_coils, c = idx - r * n_coils;
     const size_t stride = (size_t)rows * vals;
-    const float* src = partial + (size_t)r * vals + 2 * c;
+    const double* src = work + (size_t)r * vals + 2 * c;
     double re = 0.0, im = 0.0;
-    for (int b = 0; b < n_blocks; ++b) {
-        const float2 v = *reinterpret_cast<const float2*>(src + b * stride);
-        re += (double)v.x;
-        im += (double)v.y;
+    for (int k = 0; k < chunks; ++k) {
+        re += src[k * stride];
+        im += src[k * stride + 1];
     }
     double sp, cp;
     sincos(psi[row0 + r], &sp, &cp);
@@ -215,19 +230,28 @@ extern "C" int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* part
     return hipGetLastError();
 }
 
+extern "C" int64_t hb_signal_reduce_workspace(int32_t n_blocks, int32_t rows, int32_t n_tiles) {
+    if (n_blocks < 1 || rows < 0 || n_tiles < 1) return 0;
+    return (int64_t)((n_blocks + hb::kRedChunk - 1) / hb::kRedChunk) * rows * 16 * n_tiles;
+}
+
 extern "C" int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t rows, int32_t n_tiles,
-                                int32_t n_coils, const double* psi, int32_t row0, double* sig, void* stream) {
+                                int32_t n_coils, const double* psi, int32_t row0, double* work, double* sig,
+                                void* stream) {
     using namespace hb;
     if (rows < 0 || n_blocks < 1 || n_tiles < 1 || n_tiles > kMaxTiles || n_coils < 1 || 8 * n_tiles < n_coils ||
-        row0 < 0)
+        row0 < 0 || !work)
         return hipErrorInvalidValue;
     if (rows == 0) return hipSuccess;
+    const int vals = 16 * n_tiles, per_block = rows * vals, chunks = (n_blocks + kRedChunk - 1) / kRedChunk;
+    hipLaunchKernelGGL(reduce_chunks_kernel, dim3((per_block + 255) / 256, chunks), dim3(256), 0,
+                       (hipStream_t)stream, partial, n_blocks, per_block, work);
     const int total = rows * n_coils;
-    hipLaunchKernelGGL(reduce_kernel, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, partial,
-                       n_blocks, rows, 16 * n_tiles, n_coils, psi, row0, sig);
+    hipLaunchKernelGGL(reduce_final_kernel, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, work, chunks,
+                       rows, vals, n_coils, psi, row0, sig);
     return hipGetLastError();
 }
 
 extern "C" const char* hb_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 
-extern "C" int hb_abi_version(void) { return 8; }
+extern "C" int hb_abi_version(void) { return 10; }

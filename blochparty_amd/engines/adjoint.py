@@ -29,6 +29,7 @@ from ..sequence import Sequence
 from . import driver, hip
 
 ADJOINT_COILS = (1, 2, 4, 8, 16)  # coils held in registers by the adjoint kernel (padded up)
+SEG_CHUNK = 256  # workgroups per first-pass chunk of hb_seg_reduce
 
 
 def default_checkpoint_every(n_segments: int) -> int:
@@ -145,6 +146,8 @@ def run_backward(case: driver.DeviceCase, ckpts: list, every: int, h: torch.Tens
     gseg = torch.zeros(n_blocks * every * 8 if need_seg else 1, dtype=torch.float32, device=dev)
     seg = torch.zeros(S * 8 if need_seg else 1, dtype=torch.float64, device=dev)
     r8 = torch.empty((every + 7) // 8 * 3 * n_pad, dtype=torch.float32, device=dev)
+    red = torch.empty(max(1, -(-n_blocks // SEG_CHUNK) * every * 8 if need_seg else 1), dtype=torch.float64,
+                      device=dev)
     cs = case.struct()
     for c in reversed(range(len(ckpts))):
         s0, s1 = case.starts[c], case.ends[c]
@@ -152,7 +155,7 @@ def run_backward(case: driver.DeviceCase, ckpts: list, every: int, h: torch.Tens
                            gseg.data_ptr(), h.data_ptr(), sens.data_ptr(), nc, int(need_seg), int(need_coil), every)
         lib.adjoint_interval(ctypes.byref(cs), ctypes.byref(ad), s0, s1)
         if need_seg:
-            lib.seg_reduce(gseg.data_ptr(), n_blocks, every, s0, s1 - s0, seg.data_ptr())
+            lib.seg_reduce(gseg.data_ptr(), n_blocks, every, s0, s1 - s0, red.data_ptr(), seg.data_ptr())
     return gs, gc, seg, lam
 
 

@@ -33,6 +33,7 @@ RF, ADC, SPOIL, SAME_FREE, SAME_RF_DT, NEXT_FREE, NEXT_RF_DT, PULSE, PULSE_SPOIL
     1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048)
 MIN_RUN = 4  # readout runs shorter than this are stepped segment by segment
 MIN_TRAIN = 4  # likewise RF trains
+REDUCE_CHUNK = 256  # blocks per first-pass chunk of hb_signal_reduce
 SEG_DTYPE = np.dtype([("f", "<f4", (8,)), ("d", "<f8", (5,)), ("flags", "<i4"), ("pad", "<i4", (5,)),
                       ("w", "<f4", (8,))])
 assert SEG_DTYPE.itemsize == 128
@@ -208,6 +209,7 @@ class DeviceCase:
     row_starts: list[int]
     rows: list[int]
     partial: torch.Tensor
+    red_work: torch.Tensor  # hb_signal_reduce's float64 workspace
     sig: torch.Tensor  # (n_adc, C, 2) float64
     n_pulses: int = 0
     maps: torch.Tensor | None = None  # (P, 12, n_pad) reused-pulse maps
@@ -299,6 +301,8 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
         psi_rows=torch.from_numpy(psi_rows.copy()).to(dev),
         starts=starts, ends=ends, row_starts=row_starts, rows=rows,
         partial=torch.empty(max(1, n_blocks * max_rows * vals), dtype=torch.float32, device=dev),
+        red_work=torch.empty(max(1, -(-n_blocks // REDUCE_CHUNK) * max_rows * vals), dtype=torch.float64,
+                             device=dev),
         sig=torch.zeros(max(n_adc, 1), n_coils, 2, dtype=torch.float64, device=dev),
         **pulse_buffers(seq, reused, n_pad, dev),
     )
@@ -345,7 +349,7 @@ def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> l
             timer("end")
         if nr:
             lib.signal_reduce(case.partial.data_ptr(), n_blocks, nr, case.n_tiles, case.n_coils,
-                              case.psi_rows.data_ptr(), r0, case.sig.data_ptr())
+                              case.psi_rows.data_ptr(), r0, case.red_work.data_ptr(), case.sig.data_ptr())
     return ckpts
 
 

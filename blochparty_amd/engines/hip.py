@@ -15,7 +15,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent.parent / "libblochparty_amd.so"
-ABI_VERSION = 8
+ABI_VERSION = 10
 MAX_COILS = 128  # 16 tiles of 8 coils (the forward kernel's LDS budget)
 
 
@@ -63,9 +63,9 @@ _LIB: ctypes.CDLL | None = None
 _P, _I, _V = ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p
 SIGNATURES = {
     "hb_forward": [ctypes.POINTER(HbCase), _I, _I, _P, _I, _V],
-    "hb_signal_reduce": [_P, _I, _I, _I, _I, _P, _I, _P, _V],
+    "hb_signal_reduce": [_P, _I, _I, _I, _I, _P, _I, _P, _P, _V],
     "hb_adjoint_interval": [ctypes.POINTER(HbCase), ctypes.POINTER(HbAdjoint), _I, _I, _V],
-    "hb_seg_reduce": [_P, _I, _I, _I, _I, _P, _V],
+    "hb_seg_reduce": [_P, _I, _I, _I, _I, _P, _P, _V],
     "hb_pulse_maps": [ctypes.POINTER(HbCase), _P, _P, _I, _P, _P, _V],
     "hb_spinor_blocks": [_P, _P, _P, _I, _P, _V],
     "hb_mlp_grad": [_P, _P, _P, _I, _P, _P, _V],
@@ -89,6 +89,8 @@ def load() -> ctypes.CDLL:
         lib.hb_abi_version.restype = ctypes.c_int
         lib.hb_mlp_workspace.restype = ctypes.c_int64
         lib.hb_mlp_workspace.argtypes = [ctypes.c_int32]
+        lib.hb_signal_reduce_workspace.restype = ctypes.c_int64
+        lib.hb_signal_reduce_workspace.argtypes = [ctypes.c_int32] * 3
         if lib.hb_abi_version() != ABI_VERSION:
             raise RuntimeError(f"{LIB_PATH} has ABI {lib.hb_abi_version()}, expected {ABI_VERSION}: rebuild it")
         for name, args in SIGNATURES.items():

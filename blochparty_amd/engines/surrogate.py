@@ -18,7 +18,8 @@ fallback and the judge:
 Training: ``N_TRAIN`` inputs drawn uniformly from the box (the phantom's range of df, u and b1 plus a margin), their
 exact block quaternions from ``hb_spinor_blocks`` (float64), then ``STEPS`` Adam steps on batches of ``BATCH``
 (``hb_mlp_train_step``: fused forward + backward, weight gradients summed in a fixed order), the step size on a
-cosine schedule. Trained networks are cached per pulse shape and box for the life of the process.
+cosine schedule. Trained networks are kept for the life of the process and reused for any later case of the same
+pulse shape whose spins lie in their box.
 """
 
 from __future__ import annotations
@@ -39,7 +40,7 @@ BLOCKS = 10
 N_OUT = 4 * BLOCKS
 N_PARAMS = HIDDEN * 3 + HIDDEN + HIDDEN * HIDDEN + HIDDEN + N_OUT * HIDDEN + N_OUT
 N_TRAIN = 1 << 20
-STEPS = 6000
+STEPS = 20000
 BATCH = 8192
 LR = (3e-3, 1e-5)  # cosine from the first to the second
 MARGIN = 0.05  # the box extends the phantom's range by this fraction of its width on each side
@@ -224,11 +225,22 @@ def prepare(case, seq) -> None:
         if shape is None:
             continue
         box = input_box(case.spins, case.pos_lo, case.n, shape.g)
-        key = (shape.key(), tuple(np.round(box[0], 6)), tuple(np.round(box[1], 6)))
-        if key not in _CACHE:
-            _CACHE[key] = train(shape, box, dev=case.spins.device)
-        nets.append((pid, shape, _CACHE[key]))
+        net = cached(shape.key(), box)
+        if net is None:
+            net = train(shape, box, dev=case.spins.device)
+            _CACHE[(shape.key(), box)] = net
+        nets.append((pid, shape, net))
     case.extra["surrogate"] = nets
+
+
+def cached(key: str, box: tuple) -> Net | None:
+    """A network trained on this pulse shape whose box holds ``box`` (the smallest such), or None."""
+    best = None
+    for (k, b), net in _CACHE.items():
+        inside = all(abs(c - cb) + h <= hb * (1 + 1e-9) for c, h, cb, hb in zip(box[0], box[1], b[0], b[1]))
+        if k == key and inside and (best is None or np.prod(b[1]) < np.prod(best.box[1])):
+            best = net
+    return best
 
 
 def compose_maps(case, lib) -> None:

@@ -71,10 +71,12 @@ typedef struct hb_case {
 int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* partial, int32_t rows, void* stream);
 
 /* Deterministic cross-block sum (replaces the float atomicAdd of cuda/common.cuh:77-89): for r < rows,
- * sig[((row0 + r) * n_coils + c) * 2 + {re, im}] = exp(-i psi[row0 + r]) * sum over blocks, in float64, blocks in
- * order. psi: (n_adc,) float64. */
+ * sig[((row0 + r) * n_coils + c) * 2 + {re, im}] = exp(-i psi[row0 + r]) * sum over blocks, in float64 in a fixed
+ * order (blocks in chunks of 256, each summed in block order, then the chunks in order). psi: (n_adc,) float64.
+ * work: hb_signal_reduce_workspace(n_blocks, rows, n_tiles) doubles. */
+int64_t hb_signal_reduce_workspace(int32_t n_blocks, int32_t rows, int32_t n_tiles);
 int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t rows, int32_t n_tiles, int32_t n_coils,
-                     const double* psi, int32_t row0, double* sig, void* stream);
+                     const double* psi, int32_t row0, double* work, double* sig, void* stream);
 
 /* Per-spin affine maps of reused pulses (replaces pulse_maps, cuda/pulse.cu.in:3-32, launched at driver.py:600-613):
  * pulse p's K substeps composed in float64 into m -> m + D m + v; maps[(p * 12 + r) * n_pad + spin] holds D's rows
@@ -161,10 +163,13 @@ typedef struct hb_adjoint {
 
 int hb_adjoint_interval(const hb_case* cs, const hb_adjoint* ad, int32_t s0, int32_t s1, void* stream);
 
-/* In-order sum of an interval's per-workgroup segment gradients (replaces seg_reduce, cuda/seg_reduce.cu:3-11):
- * seg[(s0 + k) * 8 + g] = sum over b < n_blocks of gseg[(b * k_max + k) * 8 + g], float64, for k < len. */
-int hb_seg_reduce(const float* gseg, int32_t n_blocks, int32_t k_max, int32_t s0, int32_t len, double* seg,
-                  void* stream);
+/* Fixed-order sum of an interval's per-workgroup segment gradients (replaces seg_reduce, cuda/seg_reduce.cu:3-11):
+ * seg[(s0 + k) * 8 + g] = sum over b < n_blocks of gseg[(b * k_max + k) * 8 + g], float64, for k < len (the
+ * workgroups in chunks of 256, each summed in order, then the chunks in order). work:
+ * hb_seg_reduce_workspace(n_blocks, k_max) doubles. */
+int64_t hb_seg_reduce_workspace(int32_t n_blocks, int32_t k_max);
+int hb_seg_reduce(const float* gseg, int32_t n_blocks, int32_t k_max, int32_t s0, int32_t len, double* work,
+                  double* seg, void* stream);
 
 /* Name of a return code. */
 const char* hb_error_string(int code);

@@ -94,14 +94,18 @@ def test_adam_steps_match_the_oracle(gpu):
 
 
 @pytest.fixture(scope="module")
-def trained(gpu, slab):
+def trained(slab):
+    from blochparty_amd.engines import hip
+
+    if not hip.available():
+        pytest.skip("needs a ROCm GPU and the built library")
     ph, seq, coils, shape = slab
     case = driver.prepare_case(ph, seq, coils, reuse=True)
     box = sur.input_box(case.spins, case.pos_lo, case.n, shape.g)
     return sur.train(shape, box, dev="cuda"), box, shape
 
 
-def test_trained_network_against_the_exact_maps(trained):
+def test_trained_network_against_the_exact_maps(gpu, trained):
     net, box, shape = trained
     print(f"trained in {net.train_s:.1f} s, held-out loss {net.final_loss:.2e}")
     assert net.final_loss < 1e-3
@@ -119,7 +123,7 @@ def test_trained_network_against_the_exact_maps(trained):
     assert err < 1e-3
 
 
-def test_surrogate_simulation_matches_the_oracle(trained, slab):
+def test_surrogate_simulation_matches_the_oracle(gpu, trained, slab):
     ph, seq, coils, _ = slab
     seq4 = W.first_trs(seq, 4)
     res = bp.simulate(ph, seq4, coils, engine="gpu", pulses="surrogate")
@@ -135,7 +139,7 @@ def test_surrogate_simulation_matches_the_oracle(trained, slab):
     assert rel(res.m, ref_m) < 1e-3
 
 
-def test_spins_outside_the_box_fall_back_to_the_exact_maps(trained, slab):
+def test_spins_outside_the_box_fall_back_to_the_exact_maps(gpu, trained, slab):
     ph, seq, coils, _ = slab
     seq2 = W.first_trs(seq, 2)
     far = bp.Phantom(pos=ph.pos.clone(), pd=ph.pd, t1=ph.t1, t2=ph.t2, db0=ph.db0.clone())
@@ -156,7 +160,7 @@ def test_spins_outside_the_box_fall_back_to_the_exact_maps(trained, slab):
     assert rel(out2.signal, ex.signal) < 1e-3
 
 
-def test_ablation_training_and_cost(trained, slab):
+def test_ablation_training_and_cost(gpu, trained, slab):
     """What the trained network gains: against the same network untrained, the map error; against composing the
     200 substeps per spin exactly (the method without it), the time to form every spin's maps."""
     ph, seq, coils, _ = slab
