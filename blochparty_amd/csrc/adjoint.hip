@@ -9,7 +9,7 @@
 //      unspoil, add the ADC sample's source term sum_c h_c conj(S_c) and the coil gradients h_c conj(Mxy), reverse
 //      the step (adjoint_device.h).
 // Per-spin gradients are summed in registers and added to float64 accumulators at every segment index that is a
-// multiple of 64; per-segment gradients are summed over each wave (butterfly), then over the workgroup's 4 waves in
+// multiple of 64; per-segment gradients are summed over each wave (DPP + one MFMA per segment), then over the workgroup's 4 waves in
 // LDS in a fixed order, one partial per workgroup and segment, and over workgroups in a fixed order by seg_reduce. Nothing
 // depends on the interval length (a multiple of 64) or on scheduling: the gradients are the same bit for bit for
 // any checkpoint_every and every run.
@@ -27,27 +27,56 @@ struct StashXY {
     float x, y;
 };
 
-// Sum over the wave of 8 per-lane values (v[g], g < N_SEG_GRADS; the rest zero), in a fixed order, in 10 cross-lane
-// reads: a reduce-scatter over lane bits 5, 4, 3 (each step keeps half the values and adds the partner's copy of
-// them) leaves lane l with value (l >> 3) & 7 summed over the 8 lanes that differ in those bits; a butterfly over
-// bits 2, 1, 0 completes the sum, which lanes 8 g .. 8 g + 7 then hold for value g.
-__device__ __forceinline__ float wave_sum8(const float (&sg)[N_SEG_GRADS], int lane) {
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Per-segment gradients summed over the wave, in a fixed order, without LDS traffic. Within each 16-lane row, a
+// reduce-scatter over lane bits 2, 1, 0 (DPP: half-row mirror, then quad permutations; each step keeps half the
+// values and adds the partner's copy of them) leaves lane l with value l & 7 summed over its 8-lane group, and a
+// row rotation by 8 adds the other group. One MFMA (v_mfma_f32_16x16x4f32, B = those row sums: B[k][n] is lane
+// 16 k + n) then adds the four rows, its A operand a one-hot row selector (A[m][k] = 1 for m = i) putting segment i
+// of the current 8-block into row i of the accumulator: D[i][n] = the wave's sum of value n & 7 of segment i.
+__device__ __forceinline__ void seg_add(f4& d, const float (&sg)[N_SEG_GRADS], int i, int lane) {
     float v[8];
 #pragma unroll
     for (int g = 0; g < 8; ++g) v[g] = g < N_SEG_GRADS ? sg[g] : 0.0f;
+    {
+        const bool up = lane & 4;  // partner: 7 - (lane & 7) in the half row
 #pragma unroll
-    for (int h = 4, o = 32; h >= 1; h >>= 1, o >>= 1) {
-        const bool up = lane & o;
-#pragma unroll
-        for (int k = 0; k < h; ++k) {
-            const float send = up ? v[k] : v[k + h], keep = up ? v[k + h] : v[k];
-            v[k] = keep + __shfl_xor(send, o, 64);
+        for (int k = 0; k < 4; ++k) {
+            const float send = up ? v[k] : v[k + 4], keep = up ? v[k + 4] : v[k];
+            v[k] = keep + dpp<0x141>(send);
         }
     }
-    float r = v[0];
+    {
+        const bool up = lane & 2;  // partner: lane ^ 2
 #pragma unroll
-    for (int o = 4; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
-    return r;
+        for (int k = 0; k < 2; ++k) {
+            const float send = up ? v[k] : v[k + 2], keep = up ? v[k + 2] : v[k];
+            v[k] = keep + dpp<0x4E>(send);
+        }
+    }
+    {
+        const bool up = lane & 1;  // partner: lane ^ 1
+        const float send = up ? v[0] : v[1], keep = up ? v[1] : v[0];
+        v[0] = keep + dpp<0xB1>(send);
+    }
+    const float r = v[0] + dpp<0x128>(v[0]);  // + lane (l + 8) mod 16 of the row
+    const float a = (lane & 15) == i ? 1.0f : 0.0f;
+    d = __builtin_amdgcn_mfma_f32_16x16x4f32(a, r, d, 0, 0, 0);
+}
+
+// The accumulator's rows 0..7 (lanes 0..15 hold rows 0..3, lanes 16..31 rows 4..7, one per register; column
+// n = lane & 15) into segbuf[8][8].
+__device__ __forceinline__ void seg_store(const f4& d, float (*segbuf)[8], int lane) {
+    if (lane < 32 && (lane & 15) < 8) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) segbuf[4 * (lane >> 4) + r][lane & 15] = d[r];
+    }
 }
 
 // Effective flags: cache hints honoured only while the cache holds the factors they refer to.
@@ -141,6 +170,7 @@ __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint
             }
         }
         float sg[N_SEG_GRADS];
+        f4 segd = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int i = 7; i >= 0; --i) {
             if (i < nseg) {
@@ -167,13 +197,11 @@ __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint
                                ((f & HB_NEXT_RF_DT) ? HB_SAME_RF_DT : 0);
                 reverse_step<SEG>(q, fr, p, before[i].x, before[i].y, before[i].z, lx, ly, lz, acc, sg, rc);
                 if (f & HB_RF) rrf_ok = true; else rfree_ok = true;
-                if (SEG) {
-                    const float v = wave_sum8(sg, lane);
-                    if ((lane & 7) == 0) segbuf[wave][i][lane >> 3] = v;
-                }
+                if (SEG) seg_add(segd, sg, i, lane);
             }
         }
         if (SEG) {
+            seg_store(segd, segbuf[wave], lane);
             __syncthreads();
             if (tid < 8 * nseg) {
                 const int i = tid >> 3, g = tid & 7;
