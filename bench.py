@@ -45,6 +45,7 @@ FLOPS_RF = 48  # exact RF substep
 FLOPS_FREE = 12  # free or ADC event
 FLOPS_COIL = 8  # per coil, per spin-sample (complex multiply-add)
 FLOPS_PULSE = 30  # a reused pulse's map applied at one occurrence
+FLOPS_MZ = 3  # a free segment spoiled at its end: Mz relaxation only
 
 
 def log(msg: str) -> None:
@@ -71,6 +72,9 @@ def dispatch_flops(case, driver) -> list[float]:
     rf = (flags & driver.RF) != 0
     adc = (flags & driver.ADC) != 0
     per_seg = np.where(rf, FLOPS_RF, FLOPS_FREE) + adc * FLOPS_COIL * case.n_coils
+    per_seg = np.where((flags & driver.SPOILED_FREE) != 0, FLOPS_MZ, per_seg)  # Mz relaxation only
+    for s in np.flatnonzero(flags & driver.MERGED):  # one free event for the merged run
+        per_seg[s + 1 : s + rec["pad"][s, 3]] = 0
     for _, k, occ in case.pulses:
         for a, _ in occ:
             per_seg[a] = FLOPS_PULSE

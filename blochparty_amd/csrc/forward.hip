@@ -156,7 +156,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
             s += q.pad[2] - 1;
             continue;
         }
+        if (f & HB_SPOILED_FREE) {  // only Mz survives the spoil: relax it, zero Mxy
+            two_sum(m.mz, m.lz, (m.mz - p.pd) * expm1f(-q.dt * p.r1));
+            m.mx = m.my = 0.0f;
+            if (f & HB_MERGED) s += q.pad[3] - 1;
+            continue;
+        }
         step(q, f, p, b1sq, m, k);
+        if (f & HB_MERGED) s += q.pad[3] - 1;
         if (f & HB_ADC) {
             stg[nb * kStgStride + lane] = m.mx;
             stg[(8 + nb) * kStgStride + lane] = m.my;
@@ -254,4 +261,4 @@ extern "C" int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t 
 
 extern "C" const char* hb_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 
-extern "C" int hb_abi_version(void) { return 10; }
+extern "C" int hb_abi_version(void) { return 11; }

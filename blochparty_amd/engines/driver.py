@@ -29,8 +29,8 @@ from ..sequence import Sequence
 from . import hip, pulses
 
 BLOCK = 256
-RF, ADC, SPOIL, SAME_FREE, SAME_RF_DT, NEXT_FREE, NEXT_RF_DT, PULSE, PULSE_SPOIL, RUN, SAME_RUN, TRAIN = (
-    1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048)
+RF, ADC, SPOIL, SAME_FREE, SAME_RF_DT, NEXT_FREE, NEXT_RF_DT, PULSE, PULSE_SPOIL, RUN, SAME_RUN, TRAIN, MERGED, \
+    SPOILED_FREE = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192)
 MIN_RUN = 4  # readout runs shorter than this are stepped segment by segment
 MIN_TRAIN = 4  # likewise RF trains
 REDUCE_CHUNK = 256  # blocks per first-pass chunk of hb_signal_reduce
@@ -79,12 +79,22 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
             rec["pad"][a, 1], rec["pad"][a, 2] = pid, k
             rec["f"][a, 7] = np.float32(ph % (2 * math.pi))
     key = np.stack([dt, g[:, 0], g[:, 1], g[:, 2], b0], 1)
-    for kind, bit, nxt, cols in ((~is_rf & stepped, SAME_FREE, NEXT_FREE, slice(0, 5)),
-                                 (is_rf & stepped, SAME_RF_DT, NEXT_RF_DT, slice(0, 1))):
+    zapped = np.zeros(S, dtype=bool)
+    if runs:
+        merge_free(flags, rec, is_rf, adc, spoil, chunk_id, stepped)
+        # a free segment spoiled at its end without ADC: only Mz matters, the kernel relaxes it and zeroes Mxy (no
+        # precession factors, so it neither fills nor uses the cache)
+        zapped = stepped & ~is_rf & ~adc & ((flags & SPOIL) != 0)
+        flags[zapped] |= SPOILED_FREE
+    # the free segments' cache key in moments (equal to (dt, g, b0_offset) for single segments; merged ones carry
+    # their sums)
+    fkey = rec["d"].copy()
+    for kind, k, bit, nxt, cols in ((~is_rf & stepped & ~zapped, fkey, SAME_FREE, NEXT_FREE, slice(0, 5)),
+                                    (is_rf & stepped, key, SAME_RF_DT, NEXT_RF_DT, slice(0, 1))):
         idx = np.flatnonzero(kind)
         if len(idx) > 1:
             prev, cur = idx[:-1], idx[1:]
-            same = _close(key[cur, cols], key[prev, cols]).all(1) & (chunk_id[cur] == chunk_id[prev])
+            same = _close(k[cur, cols], k[prev, cols]).all(1) & (chunk_id[cur] == chunk_id[prev])
             flags[cur[same]] |= bit  # forward order: equal to the previous one of the kind
             flags[prev[same]] |= nxt  # reverse order: equal to the next one of the kind
     if runs:
@@ -126,6 +136,31 @@ def mark_runs(flags, rec, key, is_rf, adc, spoil, chunk_id, stepped) -> None:
             prev_key = this
             # the run's inner segments are not stepped: hints that name them would refer to nothing
             flags[s + 1 : e] &= ~(SAME_FREE | NEXT_FREE)
+        s = e
+
+
+def merge_free(flags, rec, is_rf, adc, spoil, chunk_id, stepped) -> None:
+    """Merge every maximal run (>= 2) of consecutive stepped free segments without ADC in one dispatch, only the last
+    of them spoiled, into its first record (MERGED, the count in pad[3]): free precession and relaxation compose
+    exactly, so the run is one free segment of the summed duration, gradient moment and b0_offset area. Only for
+    forwards without checkpoints (the adjoint steps and differentiates every segment)."""
+    ok = stepped & ~is_rf & ~adc
+    S = len(ok)
+    s = 0
+    while s < S:
+        if not ok[s]:
+            s += 1
+            continue
+        e = s + 1
+        while e < S and ok[e] and chunk_id[e] == chunk_id[s] and not spoil[e - 1]:
+            e += 1
+        if e - s >= 2:
+            d = rec["d"][s:e].sum(0)
+            rec["d"][s] = d
+            rec["f"][s, 0] = np.float32(d[0])
+            flags[s] |= MERGED | (flags[e - 1] & SPOIL)
+            rec["pad"][s, 3] = e - s
+            stepped[s + 1 : e] = False
         s = e
 
 

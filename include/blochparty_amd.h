@@ -33,6 +33,8 @@ extern "C" {
 #define HB_RUN 512        /* first of pad[3] identical free ADC segments: step Mxy per sample, Mz once        */
 #define HB_SAME_RUN 1024  /* a run with the previous run's dt and length (reuse its Mz factor)              */
 #define HB_TRAIN 2048     /* first of pad[3] consecutive RF segments without ADC or spoil: stepped in one loop */
+#define HB_MERGED 4096    /* first of pad[3] free segments without ADC merged into it (summed dt and moments)   */
+#define HB_SPOILED_FREE 8192 /* free segment spoiled at its end without ADC: Mz relaxes, Mxy is zeroed          */
 
 /* One segment of the event list, 128 bytes (the reference's segs row + flags + rows, driver.py:357-384). psi is the
  * receiver phase wrapped to [0, 2 pi) in float64 before the cast (driver.py:357). The float64 block carries the
@@ -41,7 +43,8 @@ extern "C" {
  * on the host: w = -2 pi dt, w_rf = w rf, w_g = w g, w_b0 = w b0_offset, w_rf2 = |w_rf|^2 (a spin's RF rotation
  * vector is (b1 w_rf, w_g . r + w db0 + w_b0)). pad[0] is the ADC row of an ADC segment (the reference's adc_row);
  * a reused pulse's first segment (HB_PULSE) holds the pulse id in pad[1], its length in pad[2] and the occurrence's
- * RF phase in psi (driver.py:592-596); a readout run's (HB_RUN) or an RF train's (HB_TRAIN) first segment its length in pad[3]. */
+ * RF phase in psi (driver.py:592-596); a readout run's (HB_RUN), an RF train's (HB_TRAIN) or a merged free run's
+ * (HB_MERGED) first segment its length in pad[3]. */
 typedef struct hb_segment {
     float dt, rf_x, rf_y, g_x, g_y, g_z, b0, psi;
     double dt64, m_x, m_y, m_z, b0_area;

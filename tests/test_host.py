@@ -116,6 +116,10 @@ def walk(rec, s0, s1):
             assert not (rec["flags"][s : s + L] & (driver.ADC | driver.SPOIL)).any()
             assert (rec["flags"][s : s + L] & driver.RF).all()
             s += L
+        elif f & driver.MERGED:
+            L = rec["pad"][s, 3]
+            assert not (rec["flags"][s : s + L] & (driver.ADC | driver.RF)).any()
+            s += L
         else:
             if f & driver.ADC:
                 rows.append(rec["pad"][s, 0])
@@ -144,9 +148,20 @@ def test_readout_runs_and_rf_trains_cover_every_sample_once(chunk):
         assert not np.asarray(seq.spoil)[s : s + L].any()
         assert np.allclose(key[s : s + L], key[s], rtol=1e-12, atol=0)
         assert not any(a in range(s + 1, s + L) for a in starts)  # never across a dispatch start
-    # without runs (the adjoint's replay) no record carries the run bits; RF trains are marked either way
+    # merged free runs: the head carries the sums of its segments' duration and moments, only the last spoiled
+    raw = driver.segment_table(seq, [0])
+    for s in np.flatnonzero(fl & driver.MERGED):
+        L = rec["pad"][s, 3]
+        assert L >= 2 and not np.asarray(seq.spoil)[s : s + L - 1].any()
+        np.testing.assert_allclose(rec["d"][s], raw["d"][s : s + L].sum(0), rtol=1e-12, atol=1e-18)
+        assert bool(fl[s] & driver.SPOIL) == bool(np.asarray(seq.spoil)[s + L - 1])
+        assert not any(a in range(s + 1, s + L) for a in starts)
+    zapped = np.flatnonzero(fl & driver.SPOILED_FREE)
+    assert len(zapped) > 0 and not (fl[zapped] & (driver.ADC | driver.RF | driver.SAME_FREE)).any()
+    assert (fl[zapped] & driver.SPOIL).all()
+    # without runs (the adjoint's replay) no record carries the run or merge bits; RF trains are marked either way
     plain = driver.segment_table(seq, starts)
-    assert not (plain["flags"] & (driver.RUN | driver.SAME_RUN)).any()
+    assert not (plain["flags"] & (driver.RUN | driver.SAME_RUN | driver.MERGED | driver.SPOILED_FREE)).any()
     assert np.array_equal(plain["flags"] & driver.TRAIN, fl & driver.TRAIN)
     assert sum((walk(plain, a, b) for a, b in zip(starts, ends)), []) == list(range(int(adc.sum())))
     if chunk > 16:
