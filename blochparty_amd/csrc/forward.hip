@@ -27,18 +27,26 @@ constexpr int kStgStride = 68;  // floats per staged column row: 64 lanes + 4 (s
 constexpr int kStgFloats = 16 * kStgStride;
 constexpr int kMaxTiles = 16;   // 128 coils: LDS = 4 x 16 x 68 + 4 x 8 x 256 floats = 49 KB
 
-__host__ __device__ constexpr int lds_floats(int nt) { return kWaves * kStgFloats + kWaves * 8 * 16 * nt; }
+// LDS: every wave's staging block, then two buffers of the waves' coil sums (flushes alternate between them).
+__host__ __device__ constexpr int lds_floats(int nt) { return kWaves * kStgFloats + 2 * kWaves * 8 * 16 * nt; }
 
 // The coil sums of the nb (<= 8) staged samples of every wave, added over the workgroup, into the partial rows
-// [row, row + nb) of this workgroup. Every lane of the workgroup calls it (it has barriers).
+// [row, row + nb) of this workgroup. Every lane of the workgroup calls it. One barrier: a wave reads only its own
+// staging block (a wave-scope fence orders its lanes' writes before the reads), and the waves' sums alternate
+// between two buffers, so a flush's writes never meet the previous flush's reads (every wave has passed this
+// flush's barrier, after those reads, before it can write the buffer again two flushes later).
 template <int NT>
-__device__ __forceinline__ void flush(float* stg, float* wred, const float (&A)[NT > 0 ? NT : 1][16],
+__device__ __forceinline__ void flush(float* stg, float* wred2, int& par, const float (&A)[NT > 0 ? NT : 1][16],
                                       const float* __restrict__ sa, int nt, int nb, int row, int rows,
                                       float* __restrict__ partial) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, jj = lane & 15;
     const int vals = 16 * nt;
-    __syncthreads();
+    float* wred = wred2 + par * (kWaves * 8 * vals);
+    par ^= 1;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     float b[16];
     {
         const f4* src = reinterpret_cast<const f4*>(stg + jj * kStgStride + 16 * g);
@@ -76,7 +84,6 @@ __device__ __forceinline__ void flush(float* stg, float* wred, const float (&A)[
         for (int w = 1; w < kWaves; ++w) s += wred[w * 8 * vals + v];
         out[v] = s;
     }
-    __syncthreads();
 }
 
 template <int NT>
@@ -102,7 +109,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
     }
     Cache k = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     float e1_run = 0.0f;  // expm1(-L dt R1) of the last readout run
-    int nb = 0, row = 0;
+    int nb = 0, row = 0, par = 0;
     for (int s = s0; s < s1; ++s) {
         const hb_segment q = cs.segs[s];
         const int f = q.flags;
@@ -132,7 +139,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
                 stg[nb * kStgStride + lane] = m.mx;
                 stg[(8 + nb) * kStgStride + lane] = m.my;
                 if (++nb == 8) {
-                    flush<NT>(stg, wred, A, sa, nt, 8, row, rows, partial);
+                    flush<NT>(stg, wred, par, A, sa, nt, 8, row, rows, partial);
                     row += 8;
                     nb = 0;
                 }
@@ -168,14 +175,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
             stg[nb * kStgStride + lane] = m.mx;
             stg[(8 + nb) * kStgStride + lane] = m.my;
             if (++nb == 8) {
-                flush<NT>(stg, wred, A, sa, nt, 8, row, rows, partial);
+                flush<NT>(stg, wred, par, A, sa, nt, 8, row, rows, partial);
                 row += 8;
                 nb = 0;
             }
         }
         if (f & HB_SPOIL) m.mx = m.my = 0.0f;
     }
-    if (nb > 0) flush<NT>(stg, wred, A, sa, nt, nb, row, rows, partial);
+    if (nb > 0) flush<NT>(stg, wred, par, A, sa, nt, nb, row, rows, partial);
     store_state(cs.state, np, j, m);
 }
 
