@@ -36,9 +36,9 @@ __host__ __device__ constexpr int lds_floats(int nt) { return kWaves * kStgFloat
 // between two buffers, so a flush's writes never meet the previous flush's reads (every wave has passed this
 // flush's barrier, after those reads, before it can write the buffer again two flushes later).
 template <int NT>
-__device__ __forceinline__ void flush(float* stg, float* wred2, int& par, const float (&A)[NT > 0 ? NT : 1][16],
-                                      const float* __restrict__ sa, int nt, int nb, int row, int rows,
-                                      float* __restrict__ partial) {
+__device__ __forceinline__ void flush_to(float* stg, float* wred2, int& par, const float (&A)[NT > 0 ? NT : 1][16],
+                                         const float* __restrict__ sa, int nt, int n_out_cols,
+                                         float* __restrict__ out) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, jj = lane & 15;
     const int vals = 16 * nt;
@@ -76,8 +76,7 @@ __device__ __forceinline__ void flush(float* stg, float* wred2, int& par, const 
         }
     }
     __syncthreads();
-    float* out = partial + ((size_t)blockIdx.x * rows + row) * vals;
-    const int n_out = (row + nb <= rows ? nb : (rows > row ? rows - row : 0)) * vals;  // never past the buffer
+    const int n_out = n_out_cols * vals;
     for (int v = tid; v < n_out; v += kBlock) {
         float s = wred[v];
 #pragma unroll
@@ -86,9 +85,82 @@ __device__ __forceinline__ void flush(float* stg, float* wred2, int& par, const 
     }
 }
 
+// The staged samples into the partial rows [row, row + nb) of this workgroup (never past the buffer's rows).
 template <int NT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 ? 3 : 4))) void forward_kernel(hb_case cs, int s0, int s1, float* __restrict__ partial,
-                                                         int rows) {
+__device__ __forceinline__ void flush(float* stg, float* wred2, int& par, const float (&A)[NT > 0 ? NT : 1][16],
+                                      const float* __restrict__ sa, int nt, int nb, int row, int rows,
+                                      float* __restrict__ partial) {
+    const int n_out = row + nb <= rows ? nb : (rows > row ? rows - row : 0);
+    flush_to<NT>(stg, wred2, par, A, sa, nt, n_out, partial + ((size_t)blockIdx.x * rows + row) * (16 * nt));
+}
+
+// One NUFFT readout run of L samples (record q, the run's first segment; pad[1] its slot in the dispatch, pad[2] its
+// kind, pad[3] L). The spin's weight (mx + i my) e^{-2 pi i q (L // 2 + 1)} times the kernel psi at each cell of the
+// workgroup's window and e^{-R2 tau_m} for each decay term is staged column by column (column = local cell * terms +
+// term) and contracted with the coils like samples; the workgroup's window of columns goes to its slot of the grid
+// partials. Then the run is one exact free step of L dt (phase L q, relaxation over L dt), and a spoil at its end.
+template <int NT>
+__device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu, const hb_segment& q, const Spin& p,
+                                          State& m, float* stg, float* wred, int& par,
+                                          const float (&A)[NT > 0 ? NT : 1][16], const float* __restrict__ sa,
+                                          int nt, int lane) {
+    const int L = q.pad[3], slot = q.pad[1], kind = q.pad[2];
+    const int ng = (int)nu.params[0], w = (int)nu.params[1], terms = (int)nu.params[2], shift = (int)nu.params[4];
+    const float beta = nu.params[3];
+    const int vals = 16 * nt;
+    // the phase per sample in cycles, float64 (free_cycles' form without the reduction)
+    const double rx = (double)p.px + (double)p.lx, ry = (double)p.py + (double)p.ly, rz = (double)p.pz + (double)p.lz;
+    const double cyc = rx * q.m_x + ry * q.m_y + rz * q.m_z + (double)p.db0 * q.dt64 + q.b0_area;
+    const double fr = cyc - floor(cyc);
+    const double xg = fr * ng;  // float64: at x ~ ng a float32 position is 1e-5 of a cycle off at the last sample
+    double ph = fr * shift;
+    ph -= rint(ph);
+    float sp, cp;
+    sincospif(-2.0f * (float)ph, &sp, &cp);
+    const float cr = m.mx * cp - m.my * sp, ci = m.mx * sp + m.my * cp;
+    float em[HB_NUFFT_MAX_TERMS];
+#pragma unroll
+    for (int t = 0; t < HB_NUFFT_MAX_TERMS; ++t) em[t] = t < terms ? expf(-nu.params[6 + t] * p.r2) : 0.0f;
+    const int* win = nu.win + ((size_t)kind * gridDim.x + blockIdx.x) * 2;
+    const int wb = win[0], W = win[1];
+    float* out = nu.part + ((size_t)blockIdx.x * nu.runs_max + slot) * nu.cols * vals;
+    int nb = 0, col = 0;
+    const double inv_half = 2.0 / (double)w;
+    for (int lc = 0; lc < W; ++lc) {
+        const float z = (float)(((double)(wb + lc) - xg) * inv_half);
+        const float kv = fabsf(z) < 1.0f ? expf(beta * (sqrtf(1.0f - z * z) - 1.0f)) : 0.0f;
+#pragma unroll
+        for (int t = 0; t < HB_NUFFT_MAX_TERMS; ++t) {
+            if (t < terms) {
+                const float a = kv * em[t];
+                stg[nb * kStgStride + lane] = a * cr;
+                stg[(8 + nb) * kStgStride + lane] = a * ci;
+                if (++nb == 8) {
+                    flush_to<NT>(stg, wred, par, A, sa, nt, 8, out + (size_t)col * vals);
+                    col += 8;
+                    nb = 0;
+                }
+            }
+        }
+    }
+    if (nb > 0) flush_to<NT>(stg, wred, par, A, sa, nt, nb, out + (size_t)col * vals);
+    // the jump over the run: phase L cyc, relaxation over L dt
+    double cl = cyc * (double)L;
+    cl -= rint(cl);
+    float s2, c2;
+    sincospif(-2.0f * (float)cl, &s2, &c2);
+    const float T = (float)L * q.dt;
+    const float e2 = expm1f(-T * p.r2);
+    const float nx = c2 * m.mx - s2 * m.my, ny = s2 * m.mx + c2 * m.my;
+    m.mx = nx + nx * e2;
+    m.my = ny + ny * e2;
+    two_sum(m.mz, m.lz, (m.mz - p.pd) * expm1f(-T * p.r1));
+    if (q.flags & HB_PULSE_SPOIL) m.mx = m.my = 0.0f;
+}
+
+template <int NT, bool NU>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 ? 3 : 4))) void forward_kernel(
+    hb_case cs, int s0, int s1, float* __restrict__ partial, int rows, hb_nufft nu) {
     extern __shared__ float lds[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = blockIdx.x * kBlock + tid;
@@ -113,6 +185,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
     for (int s = s0; s < s1; ++s) {
         const hb_segment q = cs.segs[s];
         const int f = q.flags;
+        if constexpr (NU) {
+            if (f & HB_NUFFT) {  // a readout run as a type-1 NUFFT: spread onto the grid window, jump the run
+                if (nb > 0) {     // samples staged before the run go to their rows first (the staging is reused)
+                    flush<NT>(stg, wred, par, A, sa, nt, nb, row, rows, partial);
+                    row += nb;
+                    nb = 0;
+                }
+                nufft_run<NT>(cs, nu, q, p, m, stg, wred, par, A, sa, nt, lane);
+                s += q.pad[3] - 1;
+                continue;
+            }
+        }
         if (f & HB_TRAIN) {  // RF substeps of a shaped pulse, in a loop of their own (only RF state live)
             const int L = q.pad[3];
             RfRec cur = load_rf(&cs.segs[s]);
@@ -205,7 +289,8 @@ __global__ __launch_bounds__(256) void reduce_chunks_kernel(const float* __restr
 
 __global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restrict__ work, int chunks, int rows,
                                                            int vals, int n_coils, const double* __restrict__ psi,
-                                                           int row0, double* __restrict__ sig) {
+                                                           int row0, const int* __restrict__ row_map,
+                                                           double* __restrict__ sig) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= rows * n_coils) return;
     const int r = idx / n_coils, c = idx - r * n_coils;
@@ -216,31 +301,45 @@ __global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restr
         re += src[k * stride];
         im += src[k * stride + 1];
     }
+    const int dst = row_map ? row_map[row0 + r] : row0 + r;
     double sp, cp;
-    sincos(psi[row0 + r], &sp, &cp);
-    double* out = sig + ((size_t)(row0 + r) * n_coils + c) * 2;
+    sincos(psi[dst], &sp, &cp);
+    double* out = sig + ((size_t)dst * n_coils + c) * 2;
     out[0] = re * cp + im * sp;
     out[1] = im * cp - re * sp;
 }
 
 }  // namespace hb
 
-extern "C" int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* partial, int32_t rows, void* stream) {
+extern "C" int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* partial, int32_t rows,
+                          const hb_nufft* nu, void* stream) {
     using namespace hb;
     if (!cs || cs->n_pad <= 0 || cs->n_pad % kBlock || cs->n < 0 || cs->n > cs->n_pad) return hipErrorInvalidValue;
     if (cs->n_tiles < 1 || cs->n_tiles > kMaxTiles || 8 * cs->n_tiles < cs->n_coils) return hipErrorInvalidValue;
     if (s0 < 0 || s1 < s0 || s1 > cs->n_segments || rows < 0) return hipErrorInvalidValue;
     if (cs->n_pulses < 0 || (cs->n_pulses > 0 && !cs->maps)) return hipErrorInvalidValue;
+    if (nu && (!nu->params || !nu->win || !nu->part || nu->runs_max < 1 || nu->cols < 1)) return hipErrorInvalidValue;
     if (s1 == s0) return hipSuccess;
     const dim3 grid(cs->n_pad / kBlock), block(kBlock);
     const size_t lds = sizeof(float) * lds_floats(cs->n_tiles);
     hipStream_t st = (hipStream_t)stream;
-    if (cs->n_tiles == 1)
-        hipLaunchKernelGGL(forward_kernel<1>, grid, block, lds, st, *cs, s0, s1, partial, rows);
-    else if (cs->n_tiles == 2)
-        hipLaunchKernelGGL(forward_kernel<2>, grid, block, lds, st, *cs, s0, s1, partial, rows);
-    else
-        hipLaunchKernelGGL(forward_kernel<0>, grid, block, lds, st, *cs, s0, s1, partial, rows);
+    const hb_nufft none = {nullptr, nullptr, nullptr, 0, 0};
+    const hb_nufft arg = nu ? *nu : none;
+    if (nu) {
+        if (cs->n_tiles == 1)
+            hipLaunchKernelGGL((forward_kernel<1, true>), grid, block, lds, st, *cs, s0, s1, partial, rows, arg);
+        else if (cs->n_tiles == 2)
+            hipLaunchKernelGGL((forward_kernel<2, true>), grid, block, lds, st, *cs, s0, s1, partial, rows, arg);
+        else
+            hipLaunchKernelGGL((forward_kernel<0, true>), grid, block, lds, st, *cs, s0, s1, partial, rows, arg);
+    } else {
+        if (cs->n_tiles == 1)
+            hipLaunchKernelGGL((forward_kernel<1, false>), grid, block, lds, st, *cs, s0, s1, partial, rows, arg);
+        else if (cs->n_tiles == 2)
+            hipLaunchKernelGGL((forward_kernel<2, false>), grid, block, lds, st, *cs, s0, s1, partial, rows, arg);
+        else
+            hipLaunchKernelGGL((forward_kernel<0, false>), grid, block, lds, st, *cs, s0, s1, partial, rows, arg);
+    }
     return hipGetLastError();
 }
 
@@ -250,8 +349,8 @@ extern "C" int64_t hb_signal_reduce_workspace(int32_t n_blocks, int32_t rows, in
 }
 
 extern "C" int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t rows, int32_t n_tiles,
-                                int32_t n_coils, const double* psi, int32_t row0, double* work, double* sig,
-                                void* stream) {
+                                int32_t n_coils, const double* psi, int32_t row0, const int32_t* row_map,
+                                double* work, double* sig, void* stream) {
     using namespace hb;
     if (rows < 0 || n_blocks < 1 || n_tiles < 1 || n_tiles > kMaxTiles || n_coils < 1 || 8 * n_tiles < n_coils ||
         row0 < 0 || !work)
@@ -262,10 +361,10 @@ extern "C" int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t 
                        (hipStream_t)stream, partial, n_blocks, per_block, work);
     const int total = rows * n_coils;
     hipLaunchKernelGGL(reduce_final_kernel, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, work, chunks,
-                       rows, vals, n_coils, psi, row0, sig);
+                       rows, vals, n_coils, psi, row0, row_map, sig);
     return hipGetLastError();
 }
 
 extern "C" const char* hb_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 
-extern "C" int hb_abi_version(void) { return 11; }
+extern "C" int hb_abi_version(void) { return 12; }

@@ -84,8 +84,9 @@ def simulate(
     default about sqrt(segments)). merge_free_runs must stay True (the core's option). scanner / maps: refused
     (NotImplementedError) unless None. seed: only meaningful for diffusion, which is refused.
     options: ``chunk`` (segments per dispatch, default 32768), ``group`` (256), ``mma`` (the matrix cores always
-    contract the coils here; any value is accepted), ``readouts`` ("direct"), ``nufft_width`` / ``nufft_terms``
-    (unused with "direct"), ``pulses`` ("step"; "reuse": a pulse that recurs up to a global RF phase is composed
+    contract the coils here; any value is accepted), ``readouts`` ("direct", or "nufft": readouts of >= 16 equal
+    samples as a type-1 NUFFT with ``nufft_width`` kernel cells (6) and ``nufft_terms`` decay terms (3); forward
+    only), ``pulses`` ("step"; "reuse": a pulse that recurs up to a global RF phase is composed
     once per spin into its affine map, in float64, and applied at every occurrence; "surrogate": those maps from
     the learned RF surrogate, trained on the GPU on first use, with the exact composition as the fallback outside its
     trained box and as the judge of its error on every run, reported in ``SimResult.surrogate``; both forward only).
@@ -104,8 +105,9 @@ def simulate(
         raise ValueError("the HIP kernels run workgroups of 256")
     if readouts not in ("direct", "nufft"):
         raise ValueError(f"readouts must be 'direct' or 'nufft', not {readouts!r}")
+    nufft = None
     if readouts == "nufft":
-        raise NotImplementedError("hip engine: readouts='nufft' is not implemented; use readouts='direct'")
+        nufft = (int(options.get("nufft_width", 6)), int(options.get("nufft_terms", 3)))
     if pulses not in ("step", "reuse", "surrogate"):
         raise ValueError(f"pulses must be 'step', 'reuse' or 'surrogate', not {pulses!r}")
     if chunk < 1:
@@ -115,6 +117,8 @@ def simulate(
     if grad:
         if pulses != "step":
             raise ValueError(f"pulses={pulses!r} is forward only")
+        if nufft is not None:
+            raise ValueError("readouts='nufft' is forward only")
         from .adjoint import simulate_grad
 
         return simulate_grad(phantom, seq, coils, m_init, checkpoint_every)
@@ -122,7 +126,7 @@ def simulate(
 
     with torch.no_grad():
         return forward(phantom, seq, coils, m_init, chunk=chunk, reuse=pulses == "reuse",
-                       surrogate=pulses == "surrogate")
+                       surrogate=pulses == "surrogate", nufft=nufft)
 
 
 __all__ = ["ENGINES", "GPU_OPTIONS", "available", "resolve", "simulate"]

@@ -30,7 +30,8 @@ from . import hip, pulses
 
 BLOCK = 256
 RF, ADC, SPOIL, SAME_FREE, SAME_RF_DT, NEXT_FREE, NEXT_RF_DT, PULSE, PULSE_SPOIL, RUN, SAME_RUN, TRAIN, MERGED, \
-    SPOILED_FREE = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192)
+    SPOILED_FREE, NUFFT = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384)
+NUFFT_RUNS = 64  # NUFFT runs per dispatch at most (slots of the grid partials)
 MIN_RUN = 4  # readout runs shorter than this are stepped segment by segment
 MIN_TRAIN = 4  # likewise RF trains
 REDUCE_CHUNK = 256  # blocks per first-pass chunk of hb_signal_reduce
@@ -46,7 +47,7 @@ def _close(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     return np.abs(a - b) <= SAME_TOL * np.maximum(np.abs(a), np.abs(b))
 
 
-def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = False) -> np.ndarray:
+def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = False, nufft=None) -> np.ndarray:
     """The (S,) records of ``seq``; ``starts`` are the dispatch starts (the factor caches restart there); ``pulses``:
     reused pulses (``pulses.reused_pulses``), whose occurrences are applied as maps and not stepped; ``runs``: mark
     readout runs (identical free ADC segments) for the forward kernel's per-run Mz step (not with checkpoints: the
@@ -78,6 +79,17 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
                 flags[a] |= PULSE_SPOIL
             rec["pad"][a, 1], rec["pad"][a, 2] = pid, k
             rec["f"][a, 7] = np.float32(ph % (2 * math.pi))
+    if nufft is not None:  # NUFFT readout runs: slot in their dispatch, kind, length; their samples are not stepped
+        slot = {}
+        for (a, k), kind in zip(nufft.runs, nufft.kind):
+            c = int(chunk_id[a])
+            sl = slot.get(c, 0)
+            slot[c] = sl + 1
+            flags[a] |= NUFFT
+            if spoil[a + k - 1]:
+                flags[a] |= PULSE_SPOIL
+            rec["pad"][a, 1], rec["pad"][a, 2], rec["pad"][a, 3] = sl, kind, k
+            stepped[a : a + k] = False
     key = np.stack([dt, g[:, 0], g[:, 1], g[:, 2], b0], 1)
     zapped = np.zeros(S, dtype=bool)
     if runs:
@@ -184,14 +196,19 @@ def mark_trains(flags, rec, is_rf, adc, spoil, chunk_id, stepped) -> None:
         s = e
 
 
-def dispatch_starts(seq: Sequence, chunk: int, rows_cap: int, spans=()) -> list[int]:
-    """Dispatch boundaries: at most ``chunk`` segments and ``rows_cap`` ADC samples each, never inside one of
-    ``spans`` ((first, length) of reused-pulse occurrences, which hold no ADC)."""
-    adc = np.asarray(seq.adc, dtype=np.int64)
+def dispatch_starts(seq: Sequence, chunk: int, rows_cap: int, spans=(), rows_w=None, heads=(),
+                    runs_cap: int = NUFFT_RUNS) -> list[int]:
+    """Dispatch boundaries: at most ``chunk`` segments, ``rows_cap`` partial rows (``rows_w``: the segments whose
+    sample has a partial row; default every ADC segment) and ``runs_cap`` NUFFT runs (``heads``: their first
+    segments) each, never inside one of ``spans`` ((first, length) of reused-pulse occurrences and NUFFT runs)."""
+    adc = np.asarray(seq.adc, dtype=np.int64) if rows_w is None else np.asarray(rows_w, dtype=np.int64)
     S = len(seq)
     inside = np.zeros(S + 1, dtype=bool)  # a cut before segment e would split a span
     for a, k in spans:
         inside[a + 1 : a + k] = True
+    hw = np.zeros(S, dtype=np.int64)
+    hw[np.asarray(list(heads), dtype=np.int64)] = 1
+    cumh = np.concatenate([[0], np.cumsum(hw)])
     starts, s = [], 0
     cum = np.concatenate([[0], np.cumsum(adc)])
     while s < S:
@@ -199,6 +216,8 @@ def dispatch_starts(seq: Sequence, chunk: int, rows_cap: int, spans=()) -> list[
         e = min(S, s + chunk)
         if cum[e] - cum[s] > rows_cap:  # cut at the segment holding sample number rows_cap of this dispatch
             e = int(np.searchsorted(cum, cum[s] + rows_cap, side="left"))
+        if cumh[e] - cumh[s] > runs_cap:  # likewise at the runs_cap-th run head
+            e = min(e, int(np.searchsorted(cumh, cumh[s] + runs_cap, side="left")))
         while s < e < S and inside[e]:
             e -= 1
         if e <= s:  # a span longer than the chunk: cut after it
@@ -252,6 +271,8 @@ class DeviceCase:
     pulse_index: torch.Tensor | None = None  # (P, 2) int32
     pulses: list = field(default_factory=list)
     extra: dict = field(default_factory=dict)
+    nu: dict | None = None  # NUFFT readouts: plan, device tables, work buffers, per-dispatch runs
+    perm: np.ndarray | None = None  # the spins' order on the device (sorted for NUFFT readouts), or None
 
     def struct(self) -> hip.HbCase:
         return hip.HbCase(self.spins.data_ptr(), self.pos_lo.data_ptr(), self.sens_a.data_ptr(),
@@ -275,11 +296,14 @@ def check_inputs(phantom: Phantom, coils) -> None:
 
 def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, chunk: int = DEFAULT_CHUNK,
                  device=None, partial_bytes: int = PARTIAL_BYTES, aligned: bool = False,
-                 reuse: bool = False, surrogate: bool = False) -> DeviceCase:
+                 reuse: bool = False, surrogate: bool = False, nufft=None) -> DeviceCase:
     """Upload a case: the buffers of the module docstring, the dispatch plan and the work buffers. ``aligned``:
     dispatches are exactly [c chunk, (c + 1) chunk) (the adjoint's checkpoint intervals), the partial buffer sized
     for the largest one. ``reuse``: reused pulses applied as per-spin maps; ``surrogate``: those maps from the
-    learned surrogate (trained here on first use)."""
+    learned surrogate (trained here on first use); ``nufft``: (width, terms) of NUFFT readouts (the spins are then
+    sorted along the readout's phase; ``read_result`` restores their order)."""
+    from . import nufft as nu_mod
+
     reuse = reuse or surrogate
     check_inputs(phantom, coils)
     dev = device if device is not None else hip.HIP.device
@@ -287,6 +311,26 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
     n_pad = max(BLOCK, math.ceil(n / BLOCK) * BLOCK)
     if coils is None:
         coils = torch.ones(n, 1, dtype=torch.complex128)
+    plan, order = None, None
+    if nufft is not None:
+        if aligned:
+            raise ValueError("readouts='nufft' is forward only")
+        plan = nu_mod.make_plan(seq, *nufft)
+        if plan is not None:
+            pos_in = phantom.pos.detach().double().cpu()
+            # the kernels' float64 position: float32 hi + float32 lo
+            hi32 = pos_in.float().double()
+            pos_k = (hi32 + (pos_in - hi32).float().double()).numpy()
+            db0_k = phantom.db0.detach().float().double().cpu().numpy()
+            order = nu_mod.sort_order(plan, pos_k, db0_k, seq)
+            if not nu_mod.windows(plan, pos_k[order], db0_k[order], seq, n, n_pad):
+                plan, order = None, None  # windows too wide: the per-sample path
+        if order is not None:
+            idx = torch.from_numpy(order)
+            phantom = Phantom(phantom.pos[idx], phantom.pd[idx], phantom.t1[idx], phantom.t2[idx], phantom.db0[idx],
+                              b1=None if phantom.b1 is None else phantom.b1[idx])
+            coils = coils[idx]
+            m_init = None if m_init is None else m_init[idx]
     n_coils = coils.shape[1]
     n_tiles = math.ceil(n_coils / 8)
     pad = lambda t: torch.cat([t, t.new_zeros(n_pad - n, *t.shape[1:])])
@@ -315,13 +359,20 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
     if reused and aligned:
         raise ValueError("pulses='reuse' is forward only")
     spans = [(a, k) for _, k, occ in reused for a, _ in occ]
+    adc_np = np.asarray(seq.adc, dtype=np.int64)
+    rows_w = adc_np.copy()  # samples with a partial row (not in NUFFT runs)
+    if plan is not None:
+        for a, k in plan.runs:
+            rows_w[a : a + k] = 0
+        spans += list(plan.runs)
     starts = (list(range(0, len(seq), int(chunk))) if aligned
-              else dispatch_starts(seq, int(chunk), rows_cap, spans))
+              else dispatch_starts(seq, int(chunk), rows_cap, spans, rows_w,
+                                   [a for a, _ in plan.runs] if plan is not None else ()))
     ends = starts[1:] + [len(seq)]
-    adc_cum = np.concatenate([[0], np.cumsum(np.asarray(seq.adc, dtype=np.int64))])
+    adc_cum = np.concatenate([[0], np.cumsum(rows_w)])
     row_starts = [int(adc_cum[s]) for s in starts]
     rows = [int(adc_cum[e] - adc_cum[s]) for s, e in zip(starts, ends)]
-    rec = segment_table(seq, starts, reused, runs=not aligned)
+    rec = segment_table(seq, starts, reused, runs=not aligned, nufft=plan)
     psi_rows = np.remainder(seq.adc_phase.detach().double().cpu().numpy()[np.asarray(seq.adc, dtype=bool)],
                             2 * math.pi)
     max_rows = max(rows + [0])
@@ -341,11 +392,39 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
         sig=torch.zeros(max(n_adc, 1), n_coils, 2, dtype=torch.float64, device=dev),
         **pulse_buffers(seq, reused, n_pad, dev),
     )
+    if plan is not None:
+        attach_nufft(case, plan, order, seq, adc_np, starts, ends, dev)
     if surrogate and case.n_pulses:
         from . import surrogate as sur
 
         sur.prepare(case, seq)
     return case
+
+
+def attach_nufft(case: DeviceCase, plan, order, seq: Sequence, adc_np, starts, ends, dev) -> None:
+    """The NUFFT plan's device buffers, the partial rows' map to signal rows, and per dispatch its runs."""
+    from . import nufft as nu_mod
+
+    sig_row = np.cumsum(adc_np) - 1  # signal row of every ADC segment
+    in_run = np.zeros(len(seq), dtype=bool)
+    for a, k in plan.runs:
+        in_run[a : a + k] = True
+    mapped = sig_row[(adc_np == 1) & ~in_run]  # partial row -> signal row
+    heads = np.asarray([a for a, _ in plan.runs], dtype=np.int64)
+    per_disp = [np.flatnonzero((heads >= s0) & (heads < s1)).astype(np.int32) for s0, s1 in zip(starts, ends)]
+    runs_max = max(1, max(len(x) for x in per_disp))
+    wmax = int(plan.win[..., 1].max())
+    cols = wmax * plan.terms
+    vals = 16 * case.n_tiles
+    n_blocks = case.n_pad // BLOCK
+    case.perm = order
+    case.nu = dict(nu_mod.to_device(plan, dev), plan=plan, cols=cols, runs_max=runs_max,
+                   row_map=torch.from_numpy(mapped.astype(np.int32)).to(dev),
+                   row_of=torch.from_numpy(sig_row[heads].astype(np.int32)).to(dev),
+                   kind_of=torch.from_numpy(plan.kind).to(dev),
+                   slots=[torch.from_numpy(x).to(dev) for x in per_disp], n_slots=[len(x) for x in per_disp],
+                   part=torch.empty(n_blocks * runs_max * cols * vals, dtype=torch.float32, device=dev),
+                   grid=torch.empty(runs_max * plan.terms * plan.ng * vals, dtype=torch.float64, device=dev))
 
 
 def pulse_buffers(seq: Sequence, reused, n_pad: int, dev) -> dict:
@@ -374,17 +453,35 @@ def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> l
     elif case.n_pulses:
         lib.pulse_maps(hip.ctypes.byref(st), case.pulse_table.data_ptr(), case.pulse_index.data_ptr(),
                        case.n_pulses, None, case.maps.data_ptr())
-    for s0, s1, r0, nr in zip(case.starts, case.ends, case.row_starts, case.rows):
+    nu = case.nu
+    nu_arg = None
+    if nu is not None:
+        nu_arg = hip.HbNufft(nu["nu"].data_ptr(), nu["nu_win"].data_ptr(), nu["part"].data_ptr(), nu["runs_max"],
+                             nu["cols"])
+    for d, (s0, s1, r0, nr) in enumerate(zip(case.starts, case.ends, case.row_starts, case.rows)):
         if keep:
             ckpts.append(case.state.clone())
         if timer:
             timer("begin")
-        lib.forward(hip.ctypes.byref(st), s0, s1, case.partial.data_ptr(), nr)
+        lib.forward(hip.ctypes.byref(st), s0, s1, case.partial.data_ptr(), nr,
+                    None if nu_arg is None else hip.ctypes.byref(nu_arg))
         if timer:
             timer("end")
         if nr:
             lib.signal_reduce(case.partial.data_ptr(), n_blocks, nr, case.n_tiles, case.n_coils,
-                              case.psi_rows.data_ptr(), r0, case.red_work.data_ptr(), case.sig.data_ptr())
+                              case.psi_rows.data_ptr(), r0,
+                              None if nu is None else nu["row_map"].data_ptr(), case.red_work.data_ptr(),
+                              case.sig.data_ptr())
+        if nu is not None and nu["n_slots"][d]:
+            pl = nu["plan"]
+            lib.nufft_grid(nu["part"].data_ptr(), nu["runs_max"], nu["cols"], case.n_tiles, nu["n_slots"][d],
+                           nu["slots"][d].data_ptr(), nu["kind_of"].data_ptr(), pl.ng, pl.terms,
+                           nu["nu_csr_off"].data_ptr(), nu["nu_csr"].data_ptr(), nu["nu_csr_base"].data_ptr(),
+                           nu["grid"].data_ptr())
+            lib.nufft_signal(nu["grid"].data_ptr(), nu["n_slots"][d], nu["slots"][d].data_ptr(),
+                             nu["row_of"].data_ptr(), pl.n, pl.ng, pl.terms, case.n_tiles, case.n_coils,
+                             nu["nu_tw"].data_ptr(), nu["nu_lw"].data_ptr(), case.psi_rows.data_ptr(),
+                             case.sig.data_ptr())
     return ckpts
 
 
@@ -394,6 +491,10 @@ def read_result(case: DeviceCase) -> SimResult:
     signal = torch.complex(sig[..., 0], sig[..., 1]).T.to(torch.complex64).contiguous()
     st = case.state.cpu().double()
     m = (st[:3, : case.n] + st[3:, : case.n]).T.float().contiguous()
+    if case.perm is not None:  # back to the caller's spin order
+        inv = np.empty_like(case.perm)
+        inv[case.perm] = np.arange(len(case.perm))
+        m = m[torch.from_numpy(inv)].contiguous()
     res = SimResult(signal, m)
     if case.extra.get("surrogate") is not None:
         from . import surrogate
@@ -403,8 +504,8 @@ def read_result(case: DeviceCase) -> SimResult:
 
 
 def forward(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, chunk: int = DEFAULT_CHUNK,
-            reuse: bool = False, surrogate: bool = False) -> SimResult:
-    case = prepare_case(phantom, seq, coils, m_init, chunk=chunk, reuse=reuse, surrogate=surrogate)
+            reuse: bool = False, surrogate: bool = False, nufft=None) -> SimResult:
+    case = prepare_case(phantom, seq, coils, m_init, chunk=chunk, reuse=reuse, surrogate=surrogate, nufft=nufft)
     run_forward(case)
     hip.HIP.synchronize()
     return read_result(case)

@@ -15,7 +15,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent.parent / "libblochparty_amd.so"
-ABI_VERSION = 11
+ABI_VERSION = 12
 MAX_COILS = 128  # 16 tiles of 8 coils (the forward kernel's LDS budget)
 
 
@@ -57,13 +57,27 @@ class HbAdjoint(ctypes.Structure):
     ]
 
 
+class HbNufft(ctypes.Structure):
+    """``hb_nufft`` of the header."""
+
+    _fields_ = [
+        ("params", ctypes.c_void_p),
+        ("win", ctypes.c_void_p),
+        ("part", ctypes.c_void_p),
+        ("runs_max", ctypes.c_int32),
+        ("cols", ctypes.c_int32),
+    ]
+
+
 _LIB: ctypes.CDLL | None = None
 
 # (name, argtypes): every entry point returns int (a hipError_t)
 _P, _I, _V = ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p
 SIGNATURES = {
-    "hb_forward": [ctypes.POINTER(HbCase), _I, _I, _P, _I, _V],
-    "hb_signal_reduce": [_P, _I, _I, _I, _I, _P, _I, _P, _P, _V],
+    "hb_forward": [ctypes.POINTER(HbCase), _I, _I, _P, _I, _P, _V],
+    "hb_signal_reduce": [_P, _I, _I, _I, _I, _P, _I, _P, _P, _P, _V],
+    "hb_nufft_grid": [_P, _I, _I, _I, _I, _P, _P, _I, _I, _P, _P, _P, _P, _V],
+    "hb_nufft_signal": [_P, _I, _P, _P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _V],
     "hb_adjoint_interval": [ctypes.POINTER(HbCase), ctypes.POINTER(HbAdjoint), _I, _I, _V],
     "hb_seg_reduce": [_P, _I, _I, _I, _I, _P, _P, _V],
     "hb_pulse_maps": [ctypes.POINTER(HbCase), _P, _P, _I, _P, _P, _V],

@@ -35,6 +35,9 @@ extern "C" {
 #define HB_TRAIN 2048     /* first of pad[3] consecutive RF segments without ADC or spoil: stepped in one loop */
 #define HB_MERGED 4096    /* first of pad[3] free segments without ADC merged into it (summed dt and moments)   */
 #define HB_SPOILED_FREE 8192 /* free segment spoiled at its end without ADC: Mz relaxes, Mxy is zeroed          */
+#define HB_NUFFT 16384    /* first of pad[3] readout samples taken as a NUFFT: slot pad[1], kind pad[2]; a spoil
+                             at the run's end is HB_PULSE_SPOIL                                                 */
+#define HB_NUFFT_MAX_TERMS 8
 
 /* One segment of the event list, 128 bytes (the reference's segs row + flags + rows, driver.py:357-384). psi is the
  * receiver phase wrapped to [0, 2 pi) in float64 before the cast (driver.py:357). The float64 block carries the
@@ -67,11 +70,25 @@ typedef struct hb_case {
     int32_t n, n_pad, n_coils, n_tiles, n_segments, n_pulses;
 } hb_case;
 
+/* NUFFT readouts of a dispatch (readouts="nufft"; the reference's nu buffer and grid rows, driver.py:549-580).
+ * params: (6 + HB_NUFFT_MAX_TERMS,) float32 ng, width, terms, beta, shift (= n / 2 + 1), cols, then the decay nodes
+ * tau_m [s]; win: (kinds, n_pad / 256, 2) every workgroup's window start (grid cell, may be negative) and width
+ * (cells) per run kind; part: (n_pad / 256, runs_max, cols, 16 n_tiles) the workgroups' window columns (column =
+ * local cell * terms + term) of the dispatch's runs, by slot. */
+typedef struct hb_nufft {
+    const float* params;
+    const int32_t* win;
+    float* part;
+    int32_t runs_max, cols;
+} hb_nufft;
+
 /* Forward event loop over segments [s0, s1) (replaces bloch_chunk_mma, cuda/kernels.cu.in:2-185, one launch per
  * dispatch of driver.py:658-698). Every ADC sample's coil sums per workgroup of 256 spins go to
  * partial[(block * rows + r) * 16 * n_tiles + 2 c + {0: re, 1: im}], r the sample's index among this dispatch's
- * samples (< rows), block = spin / 256; no atomics. */
-int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* partial, int32_t rows, void* stream);
+ * samples that are not in NUFFT runs (< rows), block = spin / 256; no atomics. nu: NULL, or the dispatch's NUFFT
+ * runs (HB_NUFFT records), whose window columns go to nu->part. */
+int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* partial, int32_t rows, const hb_nufft* nu,
+               void* stream);
 
 /* Deterministic cross-block sum (replaces the float atomicAdd of cuda/common.cuh:77-89): for r < rows,
  * sig[((row0 + r) * n_coils + c) * 2 + {re, im}] = exp(-i psi[row0 + r]) * sum over blocks, in float64 in a fixed
@@ -79,7 +96,25 @@ int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* partial, int32_
  * work: hb_signal_reduce_workspace(n_blocks, rows, n_tiles) doubles. */
 int64_t hb_signal_reduce_workspace(int32_t n_blocks, int32_t rows, int32_t n_tiles);
 int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t rows, int32_t n_tiles, int32_t n_coils,
-                     const double* psi, int32_t row0, double* work, double* sig, void* stream);
+                     const double* psi, int32_t row0, const int32_t* row_map, double* work, double* sig,
+                     void* stream);
+
+/* The NUFFT grids of a dispatch's runs, float64 (replaces the float atomics of the reference's grid rows): for slot
+ * r < n_slots (run run_of[r], kind kind_of[run]), term m, cell u < ng and value v < 16 n_tiles,
+ * grid[((r * terms + m) * ng + u) * vals + v] = sum over the entries (block b, local cell lc) of cell u's list, in
+ * order, of part[((b * runs_max + r) * cols + lc * terms + m) * vals + v]. csr_off: (kinds, ng + 1) list offsets
+ * (from csr_base[kind]); csr: (entries, 2). */
+int hb_nufft_grid(const float* part, int32_t runs_max, int32_t cols, int32_t n_tiles, int32_t n_slots,
+                  const int32_t* run_of, const int32_t* kind_of, int32_t ng, int32_t terms, const int32_t* csr_off,
+                  const int32_t* csr, const int32_t* csr_base, double* grid, void* stream);
+
+/* The NUFFT readouts' samples from the grids (replaces driver.py:703-725's FFT and recombination): for slot r,
+ * sample i < n and coil c, sig[(row_of[run] + i) * n_coils + c] = exp(-i psi) sum_m lw[m][i] sum_u
+ * grid[r][m][u][c] tw[i][u], float64. tw: (n, ng, 2) e^{-2 pi i j u / ng} at the modes j = i - n / 2; lw: (terms, n)
+ * Lagrange weights over the kernel's transform; row_of: (runs,) each run's first signal row. */
+int hb_nufft_signal(const double* grid, int32_t n_slots, const int32_t* run_of, const int32_t* row_of, int32_t n,
+                    int32_t ng, int32_t terms, int32_t n_tiles, int32_t n_coils, const double* tw, const double* lw,
+                    const double* psi, double* sig, void* stream);
 
 /* Per-spin affine maps of reused pulses (replaces pulse_maps, cuda/pulse.cu.in:3-32, launched at driver.py:600-613):
  * pulse p's K substeps composed in float64 into m -> m + D m + v; maps[(p * 12 + r) * n_pad + spin] holds D's rows

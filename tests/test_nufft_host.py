@@ -1,0 +1,102 @@
+"""The NUFFT readouts' host plan (engines/nufft.py), CPU only: run selection, the spins' sort and windows, and an
+emulation of the kernels' use of the plan's tables (window columns, gather lists, DFT twiddles, Lagrange weights)
+against the direct sum over spins and samples."""
+
+import numpy as np
+import pytest
+
+from blochparty_amd import workloads as W
+from blochparty_amd.engines import driver, nufft
+
+
+def test_readout_runs_start_at_the_second_sample():
+    seq = W.first_trs(W.gre3d_sequence(32), 4)
+    runs = nufft.readout_runs(seq)
+    assert len(runs) == 4
+    adc, dt = np.asarray(seq.adc), seq.dt.numpy()
+    for a, k in runs:
+        assert k == 31  # 32 samples: the half-dwell first one stays per sample
+        assert adc[a - 1] and dt[a - 1] < dt[a]  # the half-dwell sample before the run
+        assert adc[a : a + k].all() and np.allclose(dt[a : a + k], dt[a])
+    short = W.first_trs(W.gre3d_sequence(16), 2)  # 16-sample readouts leave 15: too short
+    assert nufft.readout_runs(short) == []
+
+
+def emulate(plan, q, r2, c, n_pad):
+    """The kernels' arithmetic on the plan's tables (float64, one coil): window columns per workgroup, the cells'
+    gather, the DFT at the modes, the decay terms."""
+    n_blocks = n_pad // 256
+    w, M, ng = plan.width, plan.terms, plan.ng
+    fr = q - np.floor(q)
+    x = fr * ng
+    cs = c * np.exp(-2j * np.pi * fr * plan.shift)
+    em = np.exp(-np.outer(r2, plan.nodes))  # (N, M)
+    wb, Wd = plan.win[0, :, 0], plan.win[0, :, 1]
+    part = np.zeros((n_blocks, Wd.max(), M), complex)
+    for b in range(n_blocks):
+        sl = slice(256 * b, min(256 * (b + 1), len(q)))
+        for lc in range(Wd[b]):
+            z = (wb[b] + lc - x[sl]) * 2 / w
+            kv = nufft.es_kernel(z, plan.beta)
+            part[b, lc] = (kv[:, None] * em[sl] * cs[sl, None]).sum(0)
+    off, ent = plan.csr_off[0], plan.csr[plan.csr_base[0] :]
+    grid = np.zeros((M, ng), complex)
+    for u in range(ng):
+        for e in range(off[u], off[u + 1]):
+            grid[:, u] += part[ent[e, 0], ent[e, 1]]
+    tw, lw = nufft.dft_tables(plan)
+    T = tw[..., 0] + 1j * tw[..., 1]  # (n, ng)
+    return (lw * (grid @ T.T)).sum(0)  # (n,)
+
+
+@pytest.mark.parametrize("width,terms", [(6, 3), (8, 4)])
+def test_plan_tables_reproduce_the_direct_sum(width, terms):
+    seq = W.first_trs(W.gre3d_sequence(32), 2)
+    plan = nufft.make_plan(seq, width, terms)
+    a, n = plan.runs[0]
+    rng = np.random.default_rng(0)
+    N = 3000
+    pos = rng.uniform(-0.11, 0.11, (N, 3))
+    db0 = rng.uniform(-100, 100, N)
+    order = nufft.sort_order(plan, pos, db0, seq)
+    pos, db0 = pos[order], db0[order]
+    n_pad = -(-N // 256) * 256
+    assert nufft.windows(plan, pos, db0, seq, N, n_pad)
+    q = nufft.sample_cycles(pos, db0, seq, plan.kinds[0])
+    # every spin's kernel support lies in its workgroup's window
+    base = np.floor((q - np.floor(q)) * plan.ng - 0.5 * width).astype(int) + 1
+    blk = np.arange(N) // 256
+    assert (base >= plan.win[0, blk, 0]).all()
+    assert (base + width <= plan.win[0, blk, 0] + plan.win[0, blk, 1]).all()
+    assert plan.win[0, :, 1].max() <= nufft.WMAX
+    r2 = 1 / rng.uniform(0.03, 2.0, N)
+    c = rng.normal(size=N) + 1j * rng.normal(size=N)
+    t = (np.arange(n) + 1) * float(seq.dt[a])
+    direct = (c[None, :] * np.exp(-2j * np.pi * np.outer(np.arange(1, n + 1), q)) * np.exp(-np.outer(t, r2))).sum(1)
+    got = emulate(plan, q, r2, c, n_pad)
+    err = np.abs(got - direct).max() / np.abs(direct).max()
+    assert err < (3e-5 if width == 6 else 2e-6), err
+
+
+def test_records_and_dispatches_keep_runs_whole():
+    seq = W.first_trs(W.gre3d_sequence(32), 6)
+    plan = nufft.make_plan(seq)
+    rows_w = np.asarray(seq.adc, dtype=np.int64).copy()
+    for a, k in plan.runs:
+        rows_w[a : a + k] = 0
+    starts = driver.dispatch_starts(seq, 300, 1 << 20, list(plan.runs), rows_w, [a for a, _ in plan.runs], 2)
+    ends = starts[1:] + [len(seq)]
+    heads = [a for a, _ in plan.runs]
+    for s0, s1 in zip(starts, ends):
+        assert sum(s0 <= h < s1 for h in heads) <= 2
+        for a, k in plan.runs:
+            assert not (a < s0 < a + k)  # never inside a run
+    rec = driver.segment_table(seq, starts, runs=True, nufft=plan)
+    fl = rec["flags"]
+    for (a, k), kind in zip(plan.runs, plan.kind):
+        assert fl[a] & driver.NUFFT and rec["pad"][a, 3] == k and rec["pad"][a, 2] == kind
+        assert not (fl[a + 1 : a + k] & (driver.RUN | driver.MERGED | driver.NUFFT)).any()
+    # slots count up within each dispatch
+    for s0, s1 in zip(starts, ends):
+        sl = [rec["pad"][h, 1] for h in heads if s0 <= h < s1]
+        assert sl == list(range(len(sl)))
