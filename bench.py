@@ -9,10 +9,11 @@ region is bracketed by torch.cuda.synchronize(). Throughput = spins x ADC sample
 metric, benchmarks/bench.py:65).
 
 Legs, K timed steps each after W warmup steps:
-- forward with the learned RF surrogate (pulses="surrogate"): every spin's pulse map from the network, the exact
-  maps of every 257th spin composed and compared (the judge), then the event loop -- the JSON line's ``value``. The
-  network is trained once before the timed steps (its training time is reported, not timed);
-- forward with reused exact maps (pulses="reuse": the sinc pulse composed once per spin in float64);
+- forward with the learned RF surrogate (pulses="surrogate") and NUFFT readouts (readouts="nufft", the default of
+  --readouts): every spin's pulse map from the network, the exact maps of every 257th spin composed and compared
+  (the judge), then the event loop with every readout as a type-1 NUFFT -- the JSON line's ``value``. The network is
+  trained once before the timed steps (its training time is reported, not timed);
+- forward with reused exact maps (pulses="reuse": the sinc pulse composed once per spin in float64), same readouts;
 - forward stepping every RF substep (the exact-RF baseline);
 - forward + adjoint of L = sum |signal|^2 w.r.t. the flip angle (through the per-segment RF gradients), the T1 map
   and the coil maps (the reference's --grad case, benchmarks/bench.py:69-95), exact RF: the forward with checkpoints,
@@ -46,6 +47,7 @@ FLOPS_FREE = 12  # free or ADC event
 FLOPS_COIL = 8  # per coil, per spin-sample (complex multiply-add)
 FLOPS_PULSE = 30  # a reused pulse's map applied at one occurrence
 FLOPS_MZ = 3  # a free segment spoiled at its end: Mz relaxation only
+FLOPS_SPREAD = 4  # a NUFFT grid column per spin: kernel weight x decay term x complex weight
 
 
 def log(msg: str) -> None:
@@ -78,6 +80,11 @@ def dispatch_flops(case, driver) -> list[float]:
     for _, k, occ in case.pulses:
         for a, _ in occ:
             per_seg[a] = FLOPS_PULSE
+            per_seg[a + 1 : a + k] = 0
+    if case.nu is not None:  # a NUFFT run: the jump, and width x terms grid columns (kernel value, decay, coils)
+        pl = case.nu["plan"]
+        for a, k in pl.runs:
+            per_seg[a] = FLOPS_FREE + pl.width * pl.terms * (FLOPS_COIL * case.n_coils + FLOPS_SPREAD)
             per_seg[a + 1 : a + k] = 0
     cum = np.concatenate([[0], np.cumsum(per_seg, dtype=np.float64)])
     return [float(cum[e] - cum[s]) * case.n for s, e in zip(case.starts, case.ends)]
@@ -147,6 +154,8 @@ def main() -> None:
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the legs, three elementwise copies of MIB MiB (read + write) that calibrate the HBM "
                          "byte counters of a profiler run")
+    ap.add_argument("--readouts", choices=("direct", "nufft"), default="nufft",
+                    help="readouts of the surrogate and reused-maps legs (nufft: type-1 NUFFT per readout run)")
     ap.add_argument("--cpu-trs", type=int, default=32)
     ap.add_argument("--cpu-spins", type=int, default=65536)
     args = ap.parse_args()
@@ -175,7 +184,8 @@ def main() -> None:
     surrogate = None
     if not args.no_surrogate:
         t0 = time.perf_counter()
-        case_s = driver.prepare_case(ph, seq, coils, surrogate=True)
+        nufft = (6, 3) if args.readouts == "nufft" else None
+        case_s = driver.prepare_case(ph, seq, coils, surrogate=True, nufft=nufft)
         torch.cuda.synchronize()
         nets = case_s.extra["surrogate"]
         log(f"surrogate: case prepared and {len(nets)} network(s) trained in {time.perf_counter() - t0:.1f} s")
@@ -183,7 +193,7 @@ def main() -> None:
         surrogate = {"forward": leg_s, "judge": sur.report(case_s)}
         del case_s
         torch.cuda.empty_cache()
-    case_r = driver.prepare_case(ph, seq, coils, reuse=True)
+    case_r = driver.prepare_case(ph, seq, coils, reuse=True, nufft=(6, 3) if args.readouts == "nufft" else None)
     leg_r = forward_leg(case_r, lib, driver, args.steps, args.warmup, work, "reused exact maps")
     del case_r
     torch.cuda.empty_cache()
@@ -259,7 +269,8 @@ def main() -> None:
         parity = {"sample": "the CPU-baseline sample"}
         paths = (("surrogate", "surrogate"), ("reuse", "reuse"), ("exact", "step"))
         for name, opt in paths[0 if surrogate else 1:]:
-            o = bp.simulate(ph_s, seq_s, c_s, engine="gpu", pulses=opt)
+            ro = args.readouts if opt != "step" else "direct"
+            o = bp.simulate(ph_s, seq_s, c_s, engine="gpu", pulses=opt, readouts=ro)
             parity[f"{name}_signal_rel_err"] = float((o.signal.to(torch.complex128) - ref_sig).abs().max()
                                                      / ref_sig.abs().max())
             parity[f"{name}_m_rel_err"] = float((o.m.double() - ref_m).abs().max() / ref_m.abs().max())
@@ -268,6 +279,8 @@ def main() -> None:
 
     rf_desc = ("learned surrogate maps (network per spin; exact maps of every 257th spin as the judge)" if surrogate
                else "reused per-spin pulse maps (float64, exact)")
+    rf_desc += f"; readouts {args.readouts}" + (" (type-1 NUFFT, width 6, 3 decay terms)" if args.readouts == "nufft"
+                                                else "")
     line = {
         "metric": METRIC,
         "value": lead["value"],

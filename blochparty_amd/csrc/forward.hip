@@ -38,7 +38,8 @@ __host__ __device__ constexpr int lds_floats(int nt) { return kWaves * kStgFloat
 template <int NT>
 __device__ __forceinline__ void flush_to(float* stg, float* wred2, int& par, const float (&A)[NT > 0 ? NT : 1][16],
                                          const float* __restrict__ sa, int nt, int n_out_cols,
-                                         float* __restrict__ out) {
+                                         float* __restrict__ out, int k_first = 8,
+                                         float* __restrict__ out_rest = nullptr, int k_skip = 0) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, jj = lane & 15;
     const int vals = 16 * nt;
@@ -76,12 +77,16 @@ __device__ __forceinline__ void flush_to(float* stg, float* wred2, int& par, con
         }
     }
     __syncthreads();
-    const int n_out = n_out_cols * vals;
+    // columns [0, k_first) to out, [k_first, k_first + k_skip) dropped, the rest to out_rest
+    const int n_out = n_out_cols * vals, split = k_first * vals, resume = (k_first + k_skip) * vals;
     for (int v = tid; v < n_out; v += kBlock) {
         float s = wred[v];
 #pragma unroll
         for (int w = 1; w < kWaves; ++w) s += wred[w * 8 * vals + v];
-        out[v] = s;
+        if (v < split)
+            out[v] = s;
+        else if (v >= resume)
+            out_rest[v - resume] = s;
     }
 }
 
@@ -94,16 +99,35 @@ __device__ __forceinline__ void flush(float* stg, float* wred2, int& par, const 
     flush_to<NT>(stg, wred2, par, A, sa, nt, n_out, partial + ((size_t)blockIdx.x * rows + row) * (16 * nt));
 }
 
+// A flush of nb staged columns of which the first `pending` are samples (rows [row, row + pending) of the partial
+// rows, row advanced) and the rest grid columns (from grid_out).
+template <int NT>
+__device__ __forceinline__ void flush_cols(float* stg, float* wred, int& par, const float (&A)[NT > 0 ? NT : 1][16],
+                                           const float* __restrict__ sa, int nt, int nb, int pending, int& row,
+                                           int rows, float* __restrict__ partial, float* __restrict__ grid_out) {
+    const int vals = 16 * nt;
+    if (pending == 0) {
+        flush_to<NT>(stg, wred, par, A, sa, nt, nb, grid_out);
+        return;
+    }
+    const int keep = row + pending <= rows ? pending : (rows > row ? rows - row : 0);  // never past the rows
+    flush_to<NT>(stg, wred, par, A, sa, nt, nb, partial + ((size_t)blockIdx.x * rows + row) * vals, keep, grid_out,
+                 pending - keep);
+    row += pending;
+}
+
 // One NUFFT readout run of L samples (record q, the run's first segment; pad[1] its slot in the dispatch, pad[2] its
 // kind, pad[3] L). The spin's weight (mx + i my) e^{-2 pi i q (L // 2 + 1)} times the kernel psi at each cell of the
 // workgroup's window and e^{-R2 tau_m} for each decay term is staged column by column (column = local cell * terms +
 // term) and contracted with the coils like samples; the workgroup's window of columns goes to its slot of the grid
 // partials. Then the run is one exact free step of L dt (phase L q, relaxation over L dt), and a spoil at its end.
+// Samples still staged (nb of them, rows [row, row + nb)) share the first flush with the grid columns.
 template <int NT>
 __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu, const hb_segment& q, const Spin& p,
                                           State& m, float* stg, float* wred, int& par,
                                           const float (&A)[NT > 0 ? NT : 1][16], const float* __restrict__ sa,
-                                          int nt, int lane) {
+                                          int nt, int lane, int& nb_io, int& row, int rows,
+                                          float* __restrict__ partial) {
     const int L = q.pad[3], slot = q.pad[1], kind = q.pad[2];
     const int ng = (int)nu.params[0], w = (int)nu.params[1], terms = (int)nu.params[2], shift = (int)nu.params[4];
     const float beta = nu.params[3];
@@ -124,7 +148,8 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
     const int* win = nu.win + ((size_t)kind * gridDim.x + blockIdx.x) * 2;
     const int wb = win[0], W = win[1];
     float* out = nu.part + ((size_t)blockIdx.x * nu.runs_max + slot) * nu.cols * vals;
-    int nb = 0, col = 0;
+    int nb = nb_io, col = 0, pending = nb_io;  // the staged samples come first in the first flush
+    nb_io = 0;
     const double inv_half = 2.0 / (double)w;
     for (int lc = 0; lc < W; ++lc) {
         const float z = (float)(((double)(wb + lc) - xg) * inv_half);
@@ -136,14 +161,18 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
                 stg[nb * kStgStride + lane] = a * cr;
                 stg[(8 + nb) * kStgStride + lane] = a * ci;
                 if (++nb == 8) {
-                    flush_to<NT>(stg, wred, par, A, sa, nt, 8, out + (size_t)col * vals);
-                    col += 8;
+                    flush_cols<NT>(stg, wred, par, A, sa, nt, 8, pending, row, rows, partial, out + (size_t)col * vals);
+                    col += 8 - pending;
+                    pending = 0;
                     nb = 0;
                 }
             }
         }
     }
-    if (nb > 0) flush_to<NT>(stg, wred, par, A, sa, nt, nb, out + (size_t)col * vals);
+    if (nb > 0) {
+        flush_cols<NT>(stg, wred, par, A, sa, nt, nb, pending, row, rows, partial, out + (size_t)col * vals);
+        pending = 0;
+    }
     // the jump over the run: phase L cyc, relaxation over L dt
     double cl = cyc * (double)L;
     cl -= rint(cl);
@@ -187,12 +216,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
         const int f = q.flags;
         if constexpr (NU) {
             if (f & HB_NUFFT) {  // a readout run as a type-1 NUFFT: spread onto the grid window, jump the run
-                if (nb > 0) {     // samples staged before the run go to their rows first (the staging is reused)
-                    flush<NT>(stg, wred, par, A, sa, nt, nb, row, rows, partial);
-                    row += nb;
-                    nb = 0;
-                }
-                nufft_run<NT>(cs, nu, q, p, m, stg, wred, par, A, sa, nt, lane);
+                nufft_run<NT>(cs, nu, q, p, m, stg, wred, par, A, sa, nt, lane, nb, row, rows, partial);
                 s += q.pad[3] - 1;
                 continue;
             }

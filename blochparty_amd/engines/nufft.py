@@ -21,7 +21,7 @@ at the grid positions x_s = frac(q_s) ng of a 2x-oversampled grid (ng = 2 n):
 On the GPU (``csrc/nufft.hip``, ``csrc/forward.hip``):
 
 - The spins are sorted by frac(q) of the first run's kind, so each workgroup of 256 spins covers a narrow window of
-  grid cells (its start wb and width W per workgroup and run kind, computed here with one cell of margin).
+  grid cells (its start wb and width W per workgroup and run kind, computed here).
 - The forward kernel stages each spin's (cell, term) columns of its window and contracts them with the coil
   sensitivities on the matrix cores, as it does samples. It writes the workgroup's window of grid columns, then
   jumps over the run in one exact free step.
@@ -34,7 +34,6 @@ A run kind whose windows exceed ``WMAX`` cells keeps the per-sample path.
 
 from __future__ import annotations
 
-import math
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -168,8 +167,8 @@ def sort_order(plan: Plan, pos64: np.ndarray, db0: np.ndarray, seq: Sequence) ->
 
 
 def windows(plan: Plan, pos64: np.ndarray, db0: np.ndarray, seq: Sequence, n: int, n_pad: int) -> bool:
-    """Every workgroup's window (start, width incl. the kernel and one cell of margin each side) for every run kind,
-    and the gather lists; False when a window exceeds WMAX (the caller keeps the per-sample path)."""
+    """Every workgroup's window (start, width incl. the kernel's cells) for every run kind, and the gather lists;
+    False when a window exceeds WMAX (the caller keeps the per-sample path)."""
     n_blocks = n_pad // 256
     w = plan.width
     win = np.zeros((len(plan.kinds), n_blocks, 2), dtype=np.int32)
@@ -181,7 +180,9 @@ def windows(plan: Plan, pos64: np.ndarray, db0: np.ndarray, seq: Sequence, n: in
         base = np.floor(x - 0.5 * w).astype(np.int64) + 1
         pad = np.full(n_pad - n, base[-1] if n else 0, dtype=np.int64)  # padding spins: no weight, any cell
         b = np.concatenate([base, pad]).reshape(n_blocks, 256)
-        lo, hi = b.min(1) - 1, b.max(1) + 1
+        # no margin: the kernel takes every window cell with |z| < 1, and a support cell that float64 rounding
+        # could put past the window's edge has |z| ~ 1, where the kernel is e^-beta (1e-6 at width 6)
+        lo, hi = b.min(1), b.max(1)
         width = hi - lo + w
         if width.max() > WMAX:
             return False
@@ -233,4 +234,3 @@ def to_device(plan: Plan, dev) -> dict:
 
 
 __all__ = ["Plan", "make_plan", "readout_runs", "sort_order", "windows", "to_device", "WMAX", "MIN_SAMPLES"]
-_ = math
