@@ -101,7 +101,7 @@ def timed(fn, steps: int, warmup: int) -> float:
     return time.perf_counter() - t0
 
 
-PMC_SUMMARY = "profiles/forward_kernel_gre3d64_pmc.txt"
+PMC_SUMMARY = "profiles/forward_kernel_gre3d64_nufft_pmc.txt"
 
 
 def profiled_traffic() -> float | None:
