@@ -10,7 +10,8 @@
 // exactly, half a block before and after each rotation: a Strang split of the pulse into M blocks.
 //
 // Kernels: the exact spinor products that make the training data (float64); the network's training step (a fused
-// per-sample forward + backward, per-chunk weight gradients summed in a fixed order, Adam); its evaluation; and the
+// per-sample forward + backward, per-chunk weight gradients on the matrix cores summed in a fixed order, Adam); its
+// evaluation; and the
 // per-spin maps, with spins outside the trained input box flagged for the exact composition (hb_pulse_maps' mask).
 #include "bloch_device.h"
 
@@ -180,22 +181,54 @@ __global__ __launch_bounds__(256) void mlp_fwd_bwd_kernel(const float* __restric
     }
 }
 
-// Weight gradients of one layer over one chunk of samples: out[(chunk, r, c)] = sum_s D[r][s] A[c][s], written into
-// the parameter layout (columns c < k_in are W[r][c] at w_off + r k_in + c, column k_in is the bias at b_off + r).
-__global__ __launch_bounds__(256) void mlp_wgrad_kernel(const float* __restrict__ D, const float* __restrict__ A,
-                                                        int rows, int k_in, int n, int w_off, int b_off,
-                                                        float* __restrict__ partial) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int cols = k_in + 1;
-    if (idx >= rows * cols) return;
-    const int r = idx / cols, c = idx - r * cols;
-    const int chunk = blockIdx.y;
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Weight gradients as matrix products on the matrix cores: for each layer, dW|db (rows x (k_in + 1)) =
+// D (rows x n) A^T (n x (k_in + 1)), A's last row the ones of the bias. One wave per 16 x 16 output tile and chunk of
+// kChunk samples, v_mfma_f32_16x16x4f32 over the chunk's samples 4 at a time (lane l supplies D[row m][s] and
+// A[col m][s] with m = l % 16, s = 4 u + l / 16; it receives D_out[4 (l / 16) + r][l % 16]); the tile's entries go to
+// partial[chunk][parameter], summed over the chunks in order by mlp_adam_kernel. Tiles: layer 1 4 x 1, layer 2
+// 4 x 5, layer 3 3 x 5 (rows and columns past the layer read as zeros).
+constexpr int kTiles = 4 * 1 + 4 * 5 + 3 * 5;
+
+__device__ __forceinline__ int param_index(int layer, int r, int c, int k_in) {
+    const int w_off = layer == 0 ? kW1 : layer == 1 ? kW2 : kW3, b_off = layer == 0 ? kB1 : layer == 1 ? kB2 : kB3;
+    return c < k_in ? w_off + r * k_in + c : b_off + r;
+}
+
+__global__ __launch_bounds__(64) void mlp_wgrad_kernel(const float* __restrict__ ws, int n,
+                                                       float* __restrict__ partial) {
+    const int lane = threadIdx.x, t = blockIdx.x, chunk = blockIdx.y;
+    const float* A0 = ws;
+    const float* A1 = A0 + (size_t)4 * n;
+    const float* A2 = A1 + (size_t)(kH + 1) * n;
+    const float* D1 = A2 + (size_t)(kH + 1) * n;
+    const float* D2 = D1 + (size_t)kH * n;
+    const float* D3 = D2 + (size_t)kH * n;
+    int layer, rt, ct;
+    if (t < 4) layer = 0, rt = t, ct = 0;
+    else if (t < 24) layer = 1, rt = (t - 4) / 5, ct = (t - 4) % 5;
+    else layer = 2, rt = (t - 24) / 5, ct = (t - 24) % 5;
+    const float* D = layer == 0 ? D1 : layer == 1 ? D2 : D3;
+    const float* A = layer == 0 ? A0 : layer == 1 ? A1 : A2;
+    const int rows = layer == 2 ? kOut : kH, k_in = layer == 0 ? 3 : kH, cols = k_in + 1;
+    const int m = lane & 15, kq = lane >> 4;
+    const int row = rt * 16 + m, col = ct * 16 + m;
+    const float* dr = D + (size_t)(row < rows ? row : 0) * n;
+    const float* ar = A + (size_t)(col < cols ? col : 0) * n;
     const int s0 = chunk * kChunk, s1 = min(n, s0 + kChunk);
-    const float* d = D + (size_t)r * n;
-    const float* a = A + (size_t)c * n;
-    float acc = 0.0f;
-    for (int s = s0; s < s1; ++s) acc += d[s] * a[s];
-    partial[(size_t)chunk * kParams + (c < k_in ? w_off + r * k_in + c : b_off + r)] = acc;
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int s = s0 + kq; s < s0 + kChunk; s += 4) {
+        const bool ok = s < s1;
+        const float a = (ok && row < rows) ? dr[s] : 0.0f;
+        const float b = (ok && col < cols) ? ar[s] : 0.0f;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int orow = rt * 16 + 4 * kq + r, ocol = ct * 16 + m;
+        if (orow < rows && ocol < cols) partial[(size_t)chunk * kParams + param_index(layer, orow, ocol, k_in)] = acc[r];
+    }
 }
 
 // grad = the chunks' partials summed in chunk order; then Adam (bias corrections c1 = 1/(1 - beta1^t),
@@ -308,22 +341,11 @@ __global__ __launch_bounds__(256) void surrogate_maps_kernel(hb_case cs, const f
 int launch_train_grad(const hb_mlp* net, const float* x, const float* y, int32_t n, float* work, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     const int chunks = (n + kChunk - 1) / kChunk;
-    hipLaunchKernelGGL(mlp_fwd_bwd_kernel, dim3((n + 255) / 256), dim3(256), 0, st, net->params, net->norm, x, y, n,
+    // one wave per workgroup: a batch of 8,192 samples is 128 workgroups (the kernel runs at one wave per SIMD)
+    hipLaunchKernelGGL(mlp_fwd_bwd_kernel, dim3((n + 63) / 64), dim3(64), 0, st, net->params, net->norm, x, y, n,
                        work);
-    float* A0 = work;
-    float* A1 = A0 + (size_t)4 * n;
-    float* A2 = A1 + (size_t)(kH + 1) * n;
-    float* D1 = A2 + (size_t)(kH + 1) * n;
-    float* D2 = D1 + (size_t)kH * n;
-    float* D3 = D2 + (size_t)kH * n;
-    float* partial = D3 + (size_t)kOut * n;
-    const struct {
-        const float *D, *A;
-        int rows, k_in, w_off, b_off;
-    } layers[3] = {{D1, A0, kH, 3, kW1, kB1}, {D2, A1, kH, kH, kW2, kB2}, {D3, A2, kOut, kH, kW3, kB3}};
-    for (const auto& l : layers)
-        hipLaunchKernelGGL(mlp_wgrad_kernel, dim3((l.rows * (l.k_in + 1) + 255) / 256, chunks), dim3(256), 0, st,
-                           l.D, l.A, l.rows, l.k_in, n, l.w_off, l.b_off, partial);
+    float* partial = work + (size_t)(4 + 2 * (kH + 1) + 2 * kH + kOut) * n;
+    hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(kTiles, chunks), dim3(64), 0, st, work, n, partial);
     return hipGetLastError();
 }
 
@@ -373,6 +395,28 @@ extern "C" int hb_mlp_train_step(const hb_mlp* net, const float* x, const float*
     const float c1 = (float)(1.0 / (1.0 - pow(0.9, step))), c2 = (float)(1.0 / (1.0 - pow(0.999, step)));
     hipLaunchKernelGGL(mlp_adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, (hipStream_t)stream, partial,
                        chunks, net->params, net->adam_m, net->adam_v, lr, c1, c2, (float*)nullptr);
+    return hipGetLastError();
+}
+
+extern "C" int hb_mlp_train_steps(const hb_mlp* net, const float* x, const float* y, int32_t batch, int32_t n_steps,
+                                  float* work, int32_t step0, int32_t total_steps, float lr0, float lr1,
+                                  void* stream) {
+    if (int e = check_net(net)) return e;
+    if (!net->adam_m || !net->adam_v || !x || !y || !work || batch < 1 || n_steps < 0 || step0 < 0 ||
+        total_steps < step0 + n_steps)
+        return hipErrorInvalidValue;
+    const int chunks = (batch + kChunk - 1) / kChunk;
+    float* partial = work + hb_mlp_workspace(batch) - (int64_t)chunks * kParams;
+    for (int i = 0; i < n_steps; ++i) {
+        const int t = step0 + i + 1;
+        const double lr = lr1 + 0.5 * (lr0 - lr1) * (1.0 + cos(M_PI * (t - 1) / (double)total_steps));
+        const float c1 = (float)(1.0 / (1.0 - pow(0.9, t))), c2 = (float)(1.0 / (1.0 - pow(0.999, t)));
+        if (int e = launch_train_grad(net, x + (size_t)i * batch * 3, y + (size_t)i * batch * kOut, batch, work,
+                                      stream))
+            return e;
+        hipLaunchKernelGGL(mlp_adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, (hipStream_t)stream, partial,
+                           chunks, net->params, net->adam_m, net->adam_v, (float)lr, c1, c2, (float*)nullptr);
+    }
     return hipGetLastError();
 }
 

@@ -16,9 +16,9 @@ fallback and the judge:
   units), and if it exceeds ``JUDGE_TOL`` every spin falls back to the exact maps.
 
 Training: ``N_TRAIN`` inputs drawn uniformly from the box (the phantom's range of df, u and b1 plus a margin), their
-exact block quaternions from ``hb_spinor_blocks`` (float64), then ``STEPS`` Adam steps on batches of ``BATCH``
-(``hb_mlp_train_step``: fused forward + backward, weight gradients summed in a fixed order), the step size on a
-cosine schedule. Trained networks are kept for the life of the process and reused for any later case of the same
+exact block quaternions from ``hb_spinor_blocks`` (float64), then ``STEPS`` Adam steps on batches of ``BATCH``, the
+training set reshuffled every epoch (``hb_mlp_train_steps``: a fused forward + backward, the weight gradients in
+fixed-order chunk sums, Adam; the step loop runs in C), the step size on a cosine schedule. Trained networks are kept for the life of the process and reused for any later case of the same
 pulse shape whose spins lie in their box.
 """
 
@@ -176,16 +176,16 @@ def train(shape: PulseShape, box: tuple, *, n_train: int = N_TRAIN, steps: int =
     net = new_net(box, mu, sd, seed, dev)
     st = net.struct()
     work = torch.empty(lib.hb_mlp_workspace(batch), dtype=torch.float32, device=dev)
-    xb = torch.empty(batch, 3, dtype=torch.float32, device=dev)
-    yb = torch.empty(batch, N_OUT, dtype=torch.float32, device=dev)
     gdev = torch.Generator(device=dev).manual_seed(seed)
-    for t in range(1, steps + 1):
-        idx = torch.randint(0, n_train, (batch,), generator=gdev, device=dev)
-        torch.index_select(x, 0, idx, out=xb)
-        torch.index_select(y, 0, idx, out=yb)
-        rate = lr[1] + 0.5 * (lr[0] - lr[1]) * (1 + math.cos(math.pi * (t - 1) / steps))
-        _check(lib.hb_mlp_train_step(ctypes.byref(st), xb.data_ptr(), yb.data_ptr(), batch, work.data_ptr(), t,
-                                     ctypes.c_float(rate), stream()), lib, "hb_mlp_train_step")
+    per_epoch = n_train // batch
+    done = 0
+    while done < steps:  # epochs: the training set in a fresh random order, consumed in consecutive batches
+        perm = torch.randperm(n_train, generator=gdev, device=dev)
+        xs, ys = x[perm].contiguous(), y[perm].contiguous()
+        k = min(per_epoch, steps - done)
+        _check(lib.hb_mlp_train_steps(ctypes.byref(st), xs.data_ptr(), ys.data_ptr(), batch, k, work.data_ptr(), done,
+                                      steps, lr[0], lr[1], stream()), lib, "hb_mlp_train_steps")
+        done += k
     # final loss on a held-out draw, in normalised units
     xt = ((torch.rand(65536, 3, generator=gen, dtype=torch.float64) * 2 - 1) * h + c).float().to(dev)
     yt = spinor_blocks(shape, xt)

@@ -170,6 +170,11 @@ int hb_mlp_grad(const hb_mlp* net, const float* x, const float* y, int32_t n, fl
 int hb_mlp_train_step(const hb_mlp* net, const float* x, const float* y, int32_t n, float* work, int32_t step,
                       float lr, void* stream);
 
+/* n_steps Adam steps on consecutive batches of x and y (step i reads rows [i batch, (i + 1) batch)), the step size
+ * on a cosine from lr0 to lr1 over total_steps (these being steps step0 + 1 .. step0 + n_steps). */
+int hb_mlp_train_steps(const hb_mlp* net, const float* x, const float* y, int32_t batch, int32_t n_steps, float* work,
+                       int32_t step0, int32_t total_steps, float lr0, float lr1, void* stream);
+
 /* y (n, n_out): the network's de-normalised outputs mean + scale * net((x - centre) / half-width). */
 int hb_mlp_eval(const hb_mlp* net, const float* x, int32_t n, float* y, void* stream);
 
