@@ -129,8 +129,9 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
                                           int nt, int lane, int& nb_io, int& row, int rows,
                                           float* __restrict__ partial) {
     const int L = q.pad[3], slot = q.pad[1], kind = q.pad[2];
-    const int ng = (int)nu.params[0], w = (int)nu.params[1], terms = (int)nu.params[2], shift = (int)nu.params[4];
-    const float beta = nu.params[3];
+    const int ng = (int)nu.params[0], terms = (int)nu.params[2], shift = (int)nu.params[4];
+    const float h = nu.params[6 + HB_NUFFT_MAX_TERMS];       // 2 / width: z per grid cell
+    const float bl2 = nu.params[7 + HB_NUFFT_MAX_TERMS];     // beta log2(e)
     const int vals = 16 * nt;
     // the phase per sample in cycles, float64 (free_cycles' form without the reduction)
     const double rx = (double)p.px + (double)p.lx, ry = (double)p.py + (double)p.ly, rz = (double)p.pz + (double)p.lz;
@@ -142,24 +143,34 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
     float sp, cp;
     sincospif(-2.0f * (float)ph, &sp, &cp);
     const float cr = m.mx * cp - m.my * sp, ci = m.mx * sp + m.my * cp;
-    float em[HB_NUFFT_MAX_TERMS];
+    // the weight times each decay term e^{-R2 tau_m}
+    float wr[HB_NUFFT_MAX_TERMS], wi[HB_NUFFT_MAX_TERMS];
 #pragma unroll
-    for (int t = 0; t < HB_NUFFT_MAX_TERMS; ++t) em[t] = t < terms ? expf(-nu.params[6 + t] * p.r2) : 0.0f;
+    for (int t = 0; t < HB_NUFFT_MAX_TERMS; ++t) {
+        const float e = t < terms ? expf(-nu.params[6 + t] * p.r2) : 0.0f;
+        wr[t] = e * cr;
+        wi[t] = e * ci;
+    }
     const int* win = nu.win + ((size_t)kind * gridDim.x + blockIdx.x) * 2;
     const int wb = win[0], W = win[1];
     float* out = nu.part + ((size_t)blockIdx.x * nu.runs_max + slot) * nu.cols * vals;
     int nb = nb_io, col = 0, pending = nb_io;  // the staged samples come first in the first flush
     nb_io = 0;
-    const double inv_half = 2.0 / (double)w;
+    // z = (cell - x) 2 / width: at the window's first cell from the float64 position (x ~ ng would leave a float32
+    // z 1e-6 off), then by float32 steps (|z| error under 3e-7 over a 32-cell window)
+    const float z0 = (float)(((double)wb - xg) * (double)h);
     for (int lc = 0; lc < W; ++lc) {
-        const float z = (float)(((double)(wb + lc) - xg) * inv_half);
-        const float kv = fabsf(z) < 1.0f ? expf(beta * (sqrtf(1.0f - z * z) - 1.0f)) : 0.0f;
+        const float z = fmaf((float)lc, h, z0);
+        const float u = fmaf(-z, z, 1.0f);
+        // the ES kernel e^{beta (sqrt(1 - z^2) - 1)} = 2^{beta log2(e) (sqrt(u) - 1)}, zero outside |z| < 1, on the
+        // hardware square root and exp2 (about 1 ulp each; the argument's rounding, |arg| < 20, leaves the
+        // kernel value within 2e-6 relative)
+        const float kv = u > 0.0f ? __builtin_amdgcn_exp2f(fmaf(bl2, __builtin_amdgcn_sqrtf(u), -bl2)) : 0.0f;
 #pragma unroll
         for (int t = 0; t < HB_NUFFT_MAX_TERMS; ++t) {
             if (t < terms) {
-                const float a = kv * em[t];
-                stg[nb * kStgStride + lane] = a * cr;
-                stg[(8 + nb) * kStgStride + lane] = a * ci;
+                stg[nb * kStgStride + lane] = kv * wr[t];
+                stg[(8 + nb) * kStgStride + lane] = kv * wi[t];
                 if (++nb == 8) {
                     flush_cols<NT>(stg, wred, par, A, sa, nt, 8, pending, row, rows, partial, out + (size_t)col * vals);
                     col += 8 - pending;
@@ -187,7 +198,9 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
     if (q.flags & HB_PULSE_SPOIL) m.mx = m.my = 0.0f;
 }
 
-template <int NT, bool NU>
+// NT: coil tiles (0: any, the sensitivities read from memory per flush); NU: NUFFT runs; RF: RF segments are stepped
+// (false when every pulse of the dispatch is applied as a map: the RF step is then left out, and its registers).
+template <int NT, bool NU, bool RF>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 ? 3 : 4))) void forward_kernel(
     hb_case cs, int s0, int s1, float* __restrict__ partial, int rows, hb_nufft nu) {
     extern __shared__ float lds[];
@@ -221,7 +234,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
                 continue;
             }
         }
-        if (f & HB_TRAIN) {  // RF substeps of a shaped pulse, in a loop of their own (only RF state live)
+        if (RF && (f & HB_TRAIN)) {  // RF substeps of a shaped pulse, in a loop of their own (only RF state live)
             const int L = q.pad[3];
             RfRec cur = load_rf(&cs.segs[s]);
             for (int i = 1; i <= L; ++i) {
@@ -277,7 +290,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
             if (f & HB_MERGED) s += q.pad[3] - 1;
             continue;
         }
-        step(q, f, p, b1sq, m, k);
+        if constexpr (RF)
+            step(q, f, p, b1sq, m, k);
+        else
+            free_step(q, f, p, m, k);
         if (f & HB_MERGED) s += q.pad[3] - 1;
         if (f & HB_ADC) {
             stg[nb * kStgStride + lane] = m.mx;
@@ -335,6 +351,28 @@ __global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restr
 
 }  // namespace hb
 
+namespace hb {
+template <bool NU, bool RF>
+static void launch_nt(const hb_case& cs, int s0, int s1, float* partial, int rows, const hb_nufft& nu, dim3 grid,
+                      dim3 block, size_t lds, hipStream_t st) {
+    if (cs.n_tiles == 1)
+        hipLaunchKernelGGL((forward_kernel<1, NU, RF>), grid, block, lds, st, cs, s0, s1, partial, rows, nu);
+    else if (cs.n_tiles == 2)
+        hipLaunchKernelGGL((forward_kernel<2, NU, RF>), grid, block, lds, st, cs, s0, s1, partial, rows, nu);
+    else
+        hipLaunchKernelGGL((forward_kernel<0, NU, RF>), grid, block, lds, st, cs, s0, s1, partial, rows, nu);
+}
+
+template <bool NU>
+static void launch_forward(const hb_case& cs, int s0, int s1, float* partial, int rows, const hb_nufft& nu,
+                           dim3 grid, dim3 block, size_t lds, hipStream_t st) {
+    if (cs.rf_free)
+        launch_nt<NU, false>(cs, s0, s1, partial, rows, nu, grid, block, lds, st);
+    else
+        launch_nt<NU, true>(cs, s0, s1, partial, rows, nu, grid, block, lds, st);
+}
+}  // namespace hb
+
 extern "C" int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* partial, int32_t rows,
                           const hb_nufft* nu, void* stream) {
     using namespace hb;
@@ -349,21 +387,10 @@ extern "C" int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* part
     hipStream_t st = (hipStream_t)stream;
     const hb_nufft none = {nullptr, nullptr, nullptr, 0, 0};
     const hb_nufft arg = nu ? *nu : none;
-    if (nu) {
-        if (cs->n_tiles == 1)
-            hipLaunchKernelGGL((forward_kernel<1, true>), grid, block, lds, st, *cs, s0, s1, partial, rows, arg);
-        else if (cs->n_tiles == 2)
-            hipLaunchKernelGGL((forward_kernel<2, true>), grid, block, lds, st, *cs, s0, s1, partial, rows, arg);
-        else
-            hipLaunchKernelGGL((forward_kernel<0, true>), grid, block, lds, st, *cs, s0, s1, partial, rows, arg);
-    } else {
-        if (cs->n_tiles == 1)
-            hipLaunchKernelGGL((forward_kernel<1, false>), grid, block, lds, st, *cs, s0, s1, partial, rows, arg);
-        else if (cs->n_tiles == 2)
-            hipLaunchKernelGGL((forward_kernel<2, false>), grid, block, lds, st, *cs, s0, s1, partial, rows, arg);
-        else
-            hipLaunchKernelGGL((forward_kernel<0, false>), grid, block, lds, st, *cs, s0, s1, partial, rows, arg);
-    }
+    if (nu)
+        launch_forward<true>(*cs, s0, s1, partial, rows, arg, grid, block, lds, st);
+    else
+        launch_forward<false>(*cs, s0, s1, partial, rows, arg, grid, block, lds, st);
     return hipGetLastError();
 }
 
@@ -391,4 +418,4 @@ extern "C" int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t 
 
 extern "C" const char* hb_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 
-extern "C" int hb_abi_version(void) { return 13; }
+extern "C" int hb_abi_version(void) { return 14; }

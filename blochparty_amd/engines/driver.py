@@ -94,10 +94,7 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
     zapped = np.zeros(S, dtype=bool)
     if runs:
         merge_free(flags, rec, is_rf, adc, spoil, chunk_id, stepped)
-        # a free segment spoiled at its end without ADC: only Mz matters, the kernel relaxes it and zeroes Mxy (no
-        # precession factors, so it neither fills nor uses the cache)
-        zapped = stepped & ~is_rf & ~adc & ((flags & SPOIL) != 0)
-        flags[zapped] |= SPOILED_FREE
+        zapped = zap_spoiled(flags, rec, is_rf, spoil, chunk_id, stepped)
     # the free segments' cache key in moments (equal to (dt, g, b0_offset) for single segments; merged ones carry
     # their sums)
     fkey = rec["d"].copy()
@@ -117,6 +114,8 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
     rec["w"] = np.stack([w, w * rf[:, 0], w * rf[:, 1], w * g[:, 0], w * g[:, 1], w * g[:, 2], w * b0,
                          (w * rf[:, 0]) ** 2 + (w * rf[:, 1]) ** 2], 1).astype(np.float32)
     rec["pad"][:, 0] = np.where(adc, np.cumsum(adc) - 1, -1)  # ADC row
+    ends_adc = np.flatnonzero((flags & MERGED != 0) & (flags & ADC != 0))  # merged runs sampled at their end
+    rec["pad"][ends_adc, 0] = rec["pad"][ends_adc + rec["pad"][ends_adc, 3] - 1, 0]
     return rec
 
 
@@ -152,10 +151,11 @@ def mark_runs(flags, rec, key, is_rf, adc, spoil, chunk_id, stepped) -> None:
 
 
 def merge_free(flags, rec, is_rf, adc, spoil, chunk_id, stepped) -> None:
-    """Merge every maximal run (>= 2) of consecutive stepped free segments without ADC in one dispatch, only the last
-    of them spoiled, into its first record (MERGED, the count in pad[3]): free precession and relaxation compose
-    exactly, so the run is one free segment of the summed duration, gradient moment and b0_offset area. Only for
-    forwards without checkpoints (the adjoint steps and differentiates every segment)."""
+    """Merge every maximal run (>= 2) of consecutive stepped free segments in one dispatch, none sampled but possibly
+    the last, only the last spoiled, into its first record (MERGED, the count in pad[3]; ADC when the last one is
+    sampled, the sample taken after the merged step): free precession and relaxation compose exactly, so the run is
+    one free segment of the summed duration, gradient moment and b0_offset area. Only for forwards without
+    checkpoints (the adjoint steps and differentiates every segment)."""
     ok = stepped & ~is_rf & ~adc
     S = len(ok)
     s = 0
@@ -166,14 +166,47 @@ def merge_free(flags, rec, is_rf, adc, spoil, chunk_id, stepped) -> None:
         e = s + 1
         while e < S and ok[e] and chunk_id[e] == chunk_id[s] and not spoil[e - 1]:
             e += 1
+        if e < S and stepped[e] and not is_rf[e] and adc[e] and chunk_id[e] == chunk_id[s] and not spoil[e - 1]:
+            e += 1  # a sampled free segment ends the run
         if e - s >= 2:
             d = rec["d"][s:e].sum(0)
             rec["d"][s] = d
             rec["f"][s, 0] = np.float32(d[0])
-            flags[s] |= MERGED | (flags[e - 1] & SPOIL)
+            flags[s] |= MERGED | (flags[e - 1] & (SPOIL | ADC))
             rec["pad"][s, 3] = e - s
             stepped[s + 1 : e] = False
         s = e
+
+
+def zap_spoiled(flags, rec, is_rf, spoil, chunk_id, stepped) -> np.ndarray:
+    """SPOILED_FREE on the stepped free records without ADC (single segments or merged runs) whose transverse
+    magnetization never reaches a sample: spoiled at their end, or starting at Mxy = 0 (right after a spoil, or right
+    after another such record). Only Mz matters there: the kernel relaxes it over the record and zeroes Mxy (no
+    precession factors, so such records neither fill nor use the cache). A record that starts at Mxy = 0 right after
+    another one in the same dispatch is folded into it (MERGED: relaxation over the summed duration). Returns the
+    marked records."""
+    S = len(flags)
+    zapped = np.zeros(S, dtype=bool)
+    length = lambda s: int(rec["pad"][s, 3]) if flags[s] & MERGED else 1
+    open_, open_end = -1, -1
+    for s in np.flatnonzero(stepped):
+        s = int(s)
+        free = not is_rf[s] and not flags[s] & ADC
+        if free and s == open_end and chunk_id[s] == chunk_id[open_]:
+            L = length(s)
+            stepped[s] = False
+            rec["d"][open_] += rec["d"][s]  # the sums of the folded record's segments (only the duration is used)
+            rec["f"][open_, 0] = np.float32(rec["d"][open_, 0])
+            open_end = s + L
+            flags[open_] |= MERGED
+            rec["pad"][open_, 3] = open_end - open_
+        elif free and (flags[s] & SPOIL or (s > 0 and spoil[s - 1])):
+            zapped[s] = True
+            open_, open_end = s, s + length(s)
+        else:
+            open_, open_end = -1, -1
+    flags[zapped] |= SPOILED_FREE
+    return zapped
 
 
 def mark_trains(flags, rec, is_rf, adc, spoil, chunk_id, stepped) -> None:
@@ -273,6 +306,7 @@ class DeviceCase:
     extra: dict = field(default_factory=dict)
     nu: dict | None = None  # NUFFT readouts: plan, device tables, work buffers, per-dispatch runs
     perm: np.ndarray | None = None  # the spins' order on the device (sorted for NUFFT readouts), or None
+    rf_free: list[bool] = field(default_factory=list)  # per dispatch: no segment stepped as RF (hb_case.rf_free)
 
     def struct(self) -> hip.HbCase:
         return hip.HbCase(self.spins.data_ptr(), self.pos_lo.data_ptr(), self.sens_a.data_ptr(),
@@ -373,6 +407,11 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
     row_starts = [int(adc_cum[s]) for s in starts]
     rows = [int(adc_cum[e] - adc_cum[s]) for s, e in zip(starts, ends)]
     rec = segment_table(seq, starts, reused, runs=not aligned, nufft=plan)
+    covered = np.zeros(len(seq), dtype=bool)  # segments applied as reused-pulse maps
+    for _, k, occ in reused:
+        for a, _ in occ:
+            covered[a : a + k] = True
+    rf_cum = np.concatenate([[0], np.cumsum(np.asarray(seq.is_rf, dtype=bool) & ~covered)])
     psi_rows = np.remainder(seq.adc_phase.detach().double().cpu().numpy()[np.asarray(seq.adc, dtype=bool)],
                             2 * math.pi)
     max_rows = max(rows + [0])
@@ -386,6 +425,7 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
         state0=state.to(dev).contiguous(),
         psi_rows=torch.from_numpy(psi_rows.copy()).to(dev),
         starts=starts, ends=ends, row_starts=row_starts, rows=rows,
+        rf_free=[bool(rf_cum[e] == rf_cum[s]) for s, e in zip(starts, ends)],
         partial=torch.empty(max(1, n_blocks * max_rows * vals), dtype=torch.float32, device=dev),
         red_work=torch.empty(max(1, -(-n_blocks // REDUCE_CHUNK) * max_rows * vals), dtype=torch.float64,
                              device=dev),
@@ -461,6 +501,7 @@ def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> l
     for d, (s0, s1, r0, nr) in enumerate(zip(case.starts, case.ends, case.row_starts, case.rows)):
         if keep:
             ckpts.append(case.state.clone())
+        st.rf_free = int(bool(case.rf_free) and case.rf_free[d])
         if timer:
             timer("begin")
         lib.forward(hip.ctypes.byref(st), s0, s1, case.partial.data_ptr(), nr,

@@ -15,7 +15,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent.parent / "libblochparty_amd.so"
-ABI_VERSION = 13
+ABI_VERSION = 14
 MAX_COILS = 128  # 16 tiles of 8 coils (the forward kernel's LDS budget)
 
 
@@ -35,6 +35,8 @@ class HbCase(ctypes.Structure):
         ("n_tiles", ctypes.c_int32),
         ("n_segments", ctypes.c_int32),
         ("n_pulses", ctypes.c_int32),
+        ("rf_free", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

@@ -34,6 +34,7 @@ A run kind whose windows exceed ``WMAX`` cells keeps the per-sample path.
 
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -204,10 +205,13 @@ def windows(plan: Plan, pos64: np.ndarray, db0: np.ndarray, seq: Sequence, n: in
 
 
 def device_params(plan: Plan) -> np.ndarray:
-    """The float32 parameter block the kernels read: ng, width, terms, beta, shift, cols, then tau_m (8)."""
-    p = np.zeros(6 + MAX_TERMS, dtype=np.float32)
+    """The float32 parameter block the kernels read: ng, width, terms, beta, shift, cols, then tau_m (8), then
+    2 / width and beta log2(e) (the kernel's z per cell and its exponent in base 2)."""
+    p = np.zeros(8 + MAX_TERMS, dtype=np.float32)
     p[:6] = [plan.ng, plan.width, plan.terms, plan.beta, plan.shift, plan.cols]
     p[6 : 6 + plan.terms] = plan.nodes
+    p[6 + MAX_TERMS] = 2.0 / plan.width
+    p[7 + MAX_TERMS] = plan.beta / math.log(2.0)
     return p
 
 

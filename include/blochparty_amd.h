@@ -68,11 +68,15 @@ typedef struct hb_case {
     float* state;
     const float* maps; /* (n_pulses, 12, n_pad) maps of reused pulses (hb_pulse_maps), or NULL */
     int32_t n, n_pad, n_coils, n_tiles, n_segments, n_pulses;
+    /* hb_forward: nonzero promises that no record of [s0, s1) is stepped as an RF segment (every pulse there is
+     * applied as a map), so the kernel may leave the RF step out; 0 is always correct. Ignored elsewhere. */
+    int32_t rf_free;
+    int32_t reserved; /* 0 */
 } hb_case;
 
 /* NUFFT readouts of a dispatch (readouts="nufft"; the reference's nu buffer and grid rows, driver.py:549-580).
- * params: (6 + HB_NUFFT_MAX_TERMS,) float32 ng, width, terms, beta, shift (= n / 2 + 1), cols, then the decay nodes
- * tau_m [s]; win: (kinds, n_pad / 256, 2) every workgroup's window start (grid cell, may be negative) and width
+ * params: (8 + HB_NUFFT_MAX_TERMS,) float32 ng, width, terms, beta, shift (= n / 2 + 1), cols, then the decay nodes
+ * tau_m [s], then 2 / width and beta log2(e); win: (kinds, n_pad / 256, 2) every workgroup's window start (grid cell, may be negative) and width
  * (cells) per run kind; part: (n_pad / 256, runs_max, cols, 16 n_tiles) the workgroups' window columns (column =
  * local cell * terms + term) of the dispatch's runs, by slot. */
 typedef struct hb_nufft {

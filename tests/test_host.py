@@ -116,9 +116,14 @@ def walk(rec, s0, s1):
             assert not (rec["flags"][s : s + L] & (driver.ADC | driver.SPOIL)).any()
             assert (rec["flags"][s : s + L] & driver.RF).all()
             s += L
-        elif f & driver.MERGED:
+        elif f & driver.MERGED:  # free segments, sampled at most at the end
             L = rec["pad"][s, 3]
-            assert not (rec["flags"][s : s + L] & (driver.ADC | driver.RF)).any()
+            assert not (rec["flags"][s : s + L] & driver.RF).any()
+            assert not (rec["flags"][s + 1 : s + L - 1] & driver.ADC).any()
+            assert not (f & driver.ADC and f & driver.SPOILED_FREE)
+            if f & driver.ADC:
+                assert rec["flags"][s + L - 1] & driver.ADC
+                rows.append(rec["pad"][s, 0])
             s += L
         else:
             if f & driver.ADC:
@@ -148,17 +153,28 @@ def test_readout_runs_and_rf_trains_cover_every_sample_once(chunk):
         assert not np.asarray(seq.spoil)[s : s + L].any()
         assert np.allclose(key[s : s + L], key[s], rtol=1e-12, atol=0)
         assert not any(a in range(s + 1, s + L) for a in starts)  # never across a dispatch start
-    # merged free runs: the head carries the sums of its segments' duration and moments, only the last spoiled
+    # merged free runs: the head carries the sums of its segments' duration and moments; only the last spoiled or
+    # sampled, unless the run never reaches a sample (SPOILED_FREE)
     raw = driver.segment_table(seq, [0])
+    spoil, is_rf = np.asarray(seq.spoil), np.asarray(seq.is_rf)
     for s in np.flatnonzero(fl & driver.MERGED):
         L = rec["pad"][s, 3]
-        assert L >= 2 and not np.asarray(seq.spoil)[s : s + L - 1].any()
+        assert L >= 2 and not is_rf[s : s + L].any() and not adc[s : s + L - 1].any()
         np.testing.assert_allclose(rec["d"][s], raw["d"][s : s + L].sum(0), rtol=1e-12, atol=1e-18)
-        assert bool(fl[s] & driver.SPOIL) == bool(np.asarray(seq.spoil)[s + L - 1])
+        assert np.float32(rec["d"][s, 0]) == rec["f"][s, 0]
+        if not fl[s] & driver.SPOILED_FREE:
+            assert not spoil[s : s + L - 1].any()
+            assert bool(fl[s] & driver.SPOIL) == bool(spoil[s + L - 1])
+            assert bool(fl[s] & driver.ADC) == bool(adc[s + L - 1])
         assert not any(a in range(s + 1, s + L) for a in starts)
+    assert ((fl & driver.MERGED != 0) & (fl & driver.ADC != 0)).any() if chunk > 16 else True
+    # SPOILED_FREE: no sample and no RF; spoiled at its end or starting right after a spoil (Mxy = 0)
     zapped = np.flatnonzero(fl & driver.SPOILED_FREE)
     assert len(zapped) > 0 and not (fl[zapped] & (driver.ADC | driver.RF | driver.SAME_FREE)).any()
-    assert (fl[zapped] & driver.SPOIL).all()
+    for s in zapped:
+        L = rec["pad"][s, 3] if fl[s] & driver.MERGED else 1
+        assert not adc[s : s + L].any() and not is_rf[s : s + L].any()
+        assert spoil[s + L - 1] or (s > 0 and spoil[s - 1]) or spoil[s : s + L].any()
     # without runs (the adjoint's replay) no record carries the run or merge bits; RF trains are marked either way
     plain = driver.segment_table(seq, starts)
     assert not (plain["flags"] & (driver.RUN | driver.SAME_RUN | driver.MERGED | driver.SPOILED_FREE)).any()
