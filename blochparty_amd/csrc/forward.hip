@@ -6,20 +6,29 @@
 // [s0, s1), so segment records are uniform loads (scalar cache) and the control flow never diverges.
 //
 // Coil contraction, per wave: each ADC sample leaves Mxy of the wave's 64 spins; eight samples are staged in LDS as
-// the columns of B (16 x 64: Re Mxy of samples 0..7, then Im), and D = A B on v_mfma_f32_16x16x4f32 (full float32
-// products, float32 accumulation) with A (16 x 64) the wave's sensitivities of one tile of 8 coils, rows Re S_0..7,
-// Im S_0..7, held in registers in the operand order (16 floats per lane per tile, loaded once). With the K index of
-// step u at lane group g being spin 16 g + u, lane l supplies A[l % 16][16 (l / 16) + u] and B[16 (l / 16) + u][l % 16]
-// and receives D[4 (l / 16) + r][l % 16], r = 0..3. The complex sums are then
+// the columns of B (64 x 16: Re Mxy of samples 0..7, then Im), and D = A B on the matrix cores with A (16 x 64) the
+// wave's sensitivities of one tile of 8 coils, rows Re S_0..7, Im S_0..7, held in registers in the operand order.
+// With the K index of step u at lane group g being spin 16 g + u, lane l supplies A[l % 16][16 (l / 16) + u] and
+// B[16 (l / 16) + u][l % 16] and receives D[4 (l / 16) + r][l % 16], r = 0..3. The complex sums are then
 //     P_c(t) = D[c][t] - D[8 + c][8 + t],   Q_c(t) = D[c][8 + t] + D[8 + c][t]      (signal before demodulation),
-// one cross-lane read per value. The four waves' sums are added in LDS in a fixed order and written as this
-// workgroup's partial row; hb_signal_reduce adds the workgroups in float64, in order. No atomics: the signal is the
-// same bit for bit on every run.
+// one cross-lane exchange per value (a swap of the wave's halves and a DPP rotation). The four waves' sums are added
+// in LDS in a fixed order and written as this workgroup's partial row; hb_signal_reduce adds the workgroups in
+// float64, in order. No atomics: the signal is the same bit for bit on every run.
+//
+// Precision of the product: for up to two tiles (16 coils) both operands are split into float16 pairs, x = hi + lo
+// (x scaled into [-1, 1] by powers of two), and D = Ah Bh + Ah Bl + Al Bh on v_mfma_f32_16x16x32_f16: float16
+// products are exact in the float32 accumulator, and the pairs hold each operand to 2^-24 relative (2^-25 absolute
+// below 1/4), so D is as close to the exact sum as a float32 product would be, at six 16-cycle instructions per tile
+// and flush in place of sixteen 32-cycle v_mfma_f32_16x16x4f32. More tiles use the float32 instruction, with the
+// sensitivities read from memory.
 #include "bloch_device.h"
 
 namespace hb {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef float f2 __attribute__((ext_vector_type(2)));
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
@@ -30,15 +39,38 @@ constexpr int kMaxTiles = 16;   // 128 coils: LDS = 4 x 16 x 68 + 4 x 8 x 256 fl
 // LDS: every wave's staging block, then two buffers of the waves' coil sums (flushes alternate between them).
 __host__ __device__ constexpr int lds_floats(int nt) { return kWaves * kStgFloats + 2 * kWaves * 8 * 16 * nt; }
 
+// x = hi + lo in float16 for eight floats: hi rounded to nearest, lo the rounded remainder (exact in float32). For
+// |x| <= 1 the pair holds x to 2^-24 |x| or 2^-25, whichever is larger (lo may be subnormal: the matrix cores take
+// float16 subnormals as they are).
+__device__ __forceinline__ void split8(const float* x, h8& hi, h8& lo) {
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {  // in pairs: one v_cvt_pk_f16_f32 per pair of hi and of lo parts
+        const h2 h = __builtin_convertvector((f2){x[e], x[e + 1]}, h2);
+        const h2 l = __builtin_convertvector((f2){x[e] - (float)h[0], x[e + 1] - (float)h[1]}, h2);
+        hi[e] = h[0], hi[e + 1] = h[1];
+        lo[e] = l[0], lo[e + 1] = l[1];
+    }
+}
+
+// The wave's sensitivities as matrix-core operands. NT > 0: each tile's 16 values per lane (spins 16 g + u,
+// u = 0..15, of row l % 16) scaled by the case's sens_scale and split into float16 hi and lo parts, as the A operands
+// of two v_mfma_f32_16x16x32_f16 (u = 0..7 and 8..15); NT = 0: read from memory in float32 at every flush.
+// out_scale undoes the sensitivities' and the staged values' scales (powers of two: exact).
+template <int NT>
+struct Sens {
+    h8 hi[NT > 0 ? NT : 1][2], lo[NT > 0 ? NT : 1][2];
+    const float* sa;
+    float x_scale, out_scale;
+};
+
 // The coil sums of the nb (<= 8) staged samples of every wave, added over the workgroup, into the partial rows
 // [row, row + nb) of this workgroup. Every lane of the workgroup calls it. One barrier: a wave reads only its own
 // staging block (a wave-scope fence orders its lanes' writes before the reads), and the waves' sums alternate
 // between two buffers, so a flush's writes never meet the previous flush's reads (every wave has passed this
 // flush's barrier, after those reads, before it can write the buffer again two flushes later).
 template <int NT>
-__device__ __forceinline__ void flush_to(float* stg, float* wred2, int& par, const float (&A)[NT > 0 ? NT : 1][16],
-                                         const float* __restrict__ sa, int nt, int n_out_cols,
-                                         float* __restrict__ out, int k_first = 8,
+__device__ __forceinline__ void flush_to(float* stg, float* wred2, int& par, const Sens<NT>& S, int nt,
+                                         int n_out_cols, float* __restrict__ out, int k_first = 8,
                                          float* __restrict__ out_rest = nullptr, int k_skip = 0) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, jj = lane & 15;
@@ -57,32 +89,51 @@ __device__ __forceinline__ void flush_to(float* stg, float* wred2, int& par, con
             b[4 * v] = x[0], b[4 * v + 1] = x[1], b[4 * v + 2] = x[2], b[4 * v + 3] = x[3];
         }
     }
-    const int partner = (lane < 32) ? lane + ((jj < 8) ? 40 : 24) : lane;
     float* mine = wred + wave * 8 * vals;
+    h8 bh[2], bl[2];
+    if constexpr (NT > 0) {
+        split8(b, bh[0], bl[0]);
+        split8(b + 8, bh[1], bl[1]);
+    }
     for (int t = 0; t < nt; ++t) {
         f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
         if constexpr (NT > 0) {
+            // A B = Ah Bh + Ah Bl + Al Bh (+ Al Bl, under 2^-22 of the product: dropped), small terms first; each
+            // product of float16 values is exact in the float32 accumulator
 #pragma unroll
-            for (int u = 0; u < 16; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A[t][u], b[u], acc, 0, 0, 0);
+            for (int i = 0; i < 2; ++i) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(S.lo[t][i], bh[i], acc, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(S.hi[t][i], bl[i], acc, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(S.hi[t][i], bh[i], acc, 0, 0, 0);
         } else {
-            const float* at = sa + t * 1024;
+            const float* at = S.sa + t * 1024;
 #pragma unroll
             for (int u = 0; u < 16; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(at[u * 64], b[u], acc, 0, 0, 0);
         }
+        // lane l < 32 holds D[c][jj] (c = 4 g + r) and needs D[8 + c][jj +- 8], which sits in lane l + 32 +- 8:
+        // swap the wave's halves (v_permlane32_swap), then rotate each row of 16 lanes by 8 (DPP row_ror:8); no LDS
+        f4 v;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const float other = __shfl(acc[r], partner, 64);
-            const float v = (jj < 8) ? acc[r] - other : acc[r] + other;
-            if (lane < 32) mine[(jj & 7) * vals + (8 * t + 4 * g + r) * 2 + (jj >> 3)] = v;
+            const unsigned up = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc[r]), __float_as_uint(acc[r]),
+                                                                 false, false)[1];
+            const float other = __uint_as_float(__builtin_amdgcn_update_dpp(0u, up, 0x128, 0xf, 0xf, false));
+            v[r] = (jj < 8) ? acc[r] - other : acc[r] + other;
         }
+        // this wave's sums as [sample][tile][re, im][coil]: coils 4 g .. 4 g + 3 contiguous, one 16-byte store
+        if (lane < 32) *reinterpret_cast<f4*>(mine + (jj & 7) * vals + 16 * t + 8 * (jj >> 3) + 4 * g) = v;
     }
     __syncthreads();
-    // columns [0, k_first) to out, [k_first, k_first + k_skip) dropped, the rest to out_rest
+    // columns [0, k_first) to out, [k_first, k_first + k_skip) dropped, the rest to out_rest; output value
+    // v = sample * vals + 16 tile + 2 coil + {0: re, 1: im} reads the waves' sums at [sample][tile][re, im][coil]
     const int n_out = n_out_cols * vals, split = k_first * vals, resume = (k_first + k_skip) * vals;
     for (int v = tid; v < n_out; v += kBlock) {
-        float s = wred[v];
+        const int src = (v & ~15) | ((v & 1) << 3) | ((v & 15) >> 1);
+        float s = wred[src];
 #pragma unroll
-        for (int w = 1; w < kWaves; ++w) s += wred[w * 8 * vals + v];
+        for (int w = 1; w < kWaves; ++w) s += wred[w * 8 * vals + src];
+        s *= S.out_scale;
         if (v < split)
             out[v] = s;
         else if (v >= resume)
@@ -92,26 +143,25 @@ __device__ __forceinline__ void flush_to(float* stg, float* wred2, int& par, con
 
 // The staged samples into the partial rows [row, row + nb) of this workgroup (never past the buffer's rows).
 template <int NT>
-__device__ __forceinline__ void flush(float* stg, float* wred2, int& par, const float (&A)[NT > 0 ? NT : 1][16],
-                                      const float* __restrict__ sa, int nt, int nb, int row, int rows,
-                                      float* __restrict__ partial) {
+__device__ __forceinline__ void flush(float* stg, float* wred2, int& par, const Sens<NT>& S, int nt, int nb, int row,
+                                      int rows, float* __restrict__ partial) {
     const int n_out = row + nb <= rows ? nb : (rows > row ? rows - row : 0);
-    flush_to<NT>(stg, wred2, par, A, sa, nt, n_out, partial + ((size_t)blockIdx.x * rows + row) * (16 * nt));
+    flush_to<NT>(stg, wred2, par, S, nt, n_out, partial + ((size_t)blockIdx.x * rows + row) * (16 * nt));
 }
 
 // A flush of nb staged columns of which the first `pending` are samples (rows [row, row + pending) of the partial
 // rows, row advanced) and the rest grid columns (from grid_out).
 template <int NT>
-__device__ __forceinline__ void flush_cols(float* stg, float* wred, int& par, const float (&A)[NT > 0 ? NT : 1][16],
-                                           const float* __restrict__ sa, int nt, int nb, int pending, int& row,
-                                           int rows, float* __restrict__ partial, float* __restrict__ grid_out) {
+__device__ __forceinline__ void flush_cols(float* stg, float* wred, int& par, const Sens<NT>& S, int nt, int nb,
+                                           int pending, int& row, int rows, float* __restrict__ partial,
+                                           float* __restrict__ grid_out) {
     const int vals = 16 * nt;
     if (pending == 0) {
-        flush_to<NT>(stg, wred, par, A, sa, nt, nb, grid_out);
+        flush_to<NT>(stg, wred, par, S, nt, nb, grid_out);
         return;
     }
     const int keep = row + pending <= rows ? pending : (rows > row ? rows - row : 0);  // never past the rows
-    flush_to<NT>(stg, wred, par, A, sa, nt, nb, partial + ((size_t)blockIdx.x * rows + row) * vals, keep, grid_out,
+    flush_to<NT>(stg, wred, par, S, nt, nb, partial + ((size_t)blockIdx.x * rows + row) * vals, keep, grid_out,
                  pending - keep);
     row += pending;
 }
@@ -122,12 +172,10 @@ __device__ __forceinline__ void flush_cols(float* stg, float* wred, int& par, co
 // term) and contracted with the coils like samples; the workgroup's window of columns goes to its slot of the grid
 // partials. Then the run is one exact free step of L dt (phase L q, relaxation over L dt), and a spoil at its end.
 // Samples still staged (nb of them, rows [row, row + nb)) share the first flush with the grid columns.
-template <int NT>
+template <int NT, int NUT>
 __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu, const hb_segment& q, const Spin& p,
-                                          State& m, float* stg, float* wred, int& par,
-                                          const float (&A)[NT > 0 ? NT : 1][16], const float* __restrict__ sa,
-                                          int nt, int lane, int& nb_io, int& row, int rows,
-                                          float* __restrict__ partial) {
+                                          State& m, float* stg, float* wred, int& par, const Sens<NT>& S, int nt,
+                                          int lane, int& nb_io, int& row, int rows, float* __restrict__ partial) {
     const int L = q.pad[3], slot = q.pad[1], kind = q.pad[2];
     const int ng = (int)nu.params[0], terms = (int)nu.params[2], shift = (int)nu.params[4];
     const float h = nu.params[6 + HB_NUFFT_MAX_TERMS];       // 2 / width: z per grid cell
@@ -142,11 +190,11 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
     ph -= rint(ph);
     float sp, cp;
     sincospif(-2.0f * (float)ph, &sp, &cp);
-    const float cr = m.mx * cp - m.my * sp, ci = m.mx * sp + m.my * cp;
-    // the weight times each decay term e^{-R2 tau_m}
-    float wr[HB_NUFFT_MAX_TERMS], wi[HB_NUFFT_MAX_TERMS];
+    const float cr = (m.mx * cp - m.my * sp) * S.x_scale, ci = (m.mx * sp + m.my * cp) * S.x_scale;
+    // the weight (times the staged values' scale) times each decay term e^{-R2 tau_m}
+    float wr[NUT], wi[NUT];
 #pragma unroll
-    for (int t = 0; t < HB_NUFFT_MAX_TERMS; ++t) {
+    for (int t = 0; t < NUT; ++t) {
         const float e = t < terms ? expf(-nu.params[6 + t] * p.r2) : 0.0f;
         wr[t] = e * cr;
         wi[t] = e * ci;
@@ -167,12 +215,12 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
         // kernel value within 2e-6 relative)
         const float kv = u > 0.0f ? __builtin_amdgcn_exp2f(fmaf(bl2, __builtin_amdgcn_sqrtf(u), -bl2)) : 0.0f;
 #pragma unroll
-        for (int t = 0; t < HB_NUFFT_MAX_TERMS; ++t) {
+        for (int t = 0; t < NUT; ++t) {
             if (t < terms) {
                 stg[nb * kStgStride + lane] = kv * wr[t];
                 stg[(8 + nb) * kStgStride + lane] = kv * wi[t];
                 if (++nb == 8) {
-                    flush_cols<NT>(stg, wred, par, A, sa, nt, 8, pending, row, rows, partial, out + (size_t)col * vals);
+                    flush_cols<NT>(stg, wred, par, S, nt, 8, pending, row, rows, partial, out + (size_t)col * vals);
                     col += 8 - pending;
                     pending = 0;
                     nb = 0;
@@ -181,7 +229,7 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
         }
     }
     if (nb > 0) {
-        flush_cols<NT>(stg, wred, par, A, sa, nt, nb, pending, row, rows, partial, out + (size_t)col * vals);
+        flush_cols<NT>(stg, wred, par, S, nt, nb, pending, row, rows, partial, out + (size_t)col * vals);
         pending = 0;
     }
     // the jump over the run: phase L cyc, relaxation over L dt
@@ -198,9 +246,10 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
     if (q.flags & HB_PULSE_SPOIL) m.mx = m.my = 0.0f;
 }
 
-// NT: coil tiles (0: any, the sensitivities read from memory per flush); NU: NUFFT runs; RF: RF segments are stepped
-// (false when every pulse of the dispatch is applied as a map: the RF step is then left out, and its registers).
-template <int NT, bool NU, bool RF>
+// NT: coil tiles (0: any, the sensitivities read from memory per flush); NUT: NUFFT runs with up to NUT decay terms
+// (0: none; the terms' weights are held in registers); RF: RF segments are stepped (false when every pulse of the
+// dispatch is applied as a map: the RF step is then left out, and its registers).
+template <int NT, int NUT, bool RF>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 ? 3 : 4))) void forward_kernel(
     hb_case cs, int s0, int s1, float* __restrict__ partial, int rows, hb_nufft nu) {
     extern __shared__ float lds[];
@@ -213,13 +262,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
     const Spin p = load_spin(cs.spins, cs.pos_lo, np, j);
     const float b1sq = p.b1 * p.b1;
     State m = load_state(cs.state, np, j);
-    const float* sa = cs.sens_a + (size_t)(blockIdx.x * kWaves + wave) * nt * 1024 + lane;
-    float A[NT > 0 ? NT : 1][16];
+    Sens<NT> S;
+    S.sa = cs.sens_a + (size_t)(blockIdx.x * kWaves + wave) * nt * 1024 + lane;
+    S.x_scale = cs.x_scale > 0.0f ? cs.x_scale : 1.0f;
+    const float sens_scale = cs.sens_scale > 0.0f ? cs.sens_scale : 1.0f;
     if constexpr (NT > 0) {
+        S.out_scale = 1.0f / (S.x_scale * sens_scale);
 #pragma unroll
         for (int t = 0; t < NT; ++t)
 #pragma unroll
-            for (int u = 0; u < 16; ++u) A[t][u] = sa[t * 1024 + u * 64];
+            for (int i = 0; i < 2; ++i) {
+                float a[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) a[e] = S.sa[t * 1024 + (8 * i + e) * 64] * sens_scale;
+                split8(a, S.hi[t][i], S.lo[t][i]);
+            }
+    } else {
+        S.out_scale = 1.0f / S.x_scale;
     }
     Cache k = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     float e1_run = 0.0f;  // expm1(-L dt R1) of the last readout run
@@ -227,9 +286,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
     for (int s = s0; s < s1; ++s) {
         const hb_segment q = cs.segs[s];
         const int f = q.flags;
-        if constexpr (NU) {
+        if constexpr (NUT > 0) {
             if (f & HB_NUFFT) {  // a readout run as a type-1 NUFFT: spread onto the grid window, jump the run
-                nufft_run<NT>(cs, nu, q, p, m, stg, wred, par, A, sa, nt, lane, nb, row, rows, partial);
+                nufft_run<NT, NUT>(cs, nu, q, p, m, stg, wred, par, S, nt, lane, nb, row, rows, partial);
                 s += q.pad[3] - 1;
                 continue;
             }
@@ -257,10 +316,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
                 const float nx = k.c * m.mx - k.s * m.my, ny = k.s * m.mx + k.c * m.my;
                 m.mx = nx + nx * k.e2;
                 m.my = ny + ny * k.e2;
-                stg[nb * kStgStride + lane] = m.mx;
-                stg[(8 + nb) * kStgStride + lane] = m.my;
+                stg[nb * kStgStride + lane] = m.mx * S.x_scale;
+                stg[(8 + nb) * kStgStride + lane] = m.my * S.x_scale;
                 if (++nb == 8) {
-                    flush<NT>(stg, wred, par, A, sa, nt, 8, row, rows, partial);
+                    flush<NT>(stg, wred, par, S, nt, 8, row, rows, partial);
                     row += 8;
                     nb = 0;
                 }
@@ -296,17 +355,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
             free_step(q, f, p, m, k);
         if (f & HB_MERGED) s += q.pad[3] - 1;
         if (f & HB_ADC) {
-            stg[nb * kStgStride + lane] = m.mx;
-            stg[(8 + nb) * kStgStride + lane] = m.my;
+            stg[nb * kStgStride + lane] = m.mx * S.x_scale;
+            stg[(8 + nb) * kStgStride + lane] = m.my * S.x_scale;
             if (++nb == 8) {
-                flush<NT>(stg, wred, par, A, sa, nt, 8, row, rows, partial);
+                flush<NT>(stg, wred, par, S, nt, 8, row, rows, partial);
                 row += 8;
                 nb = 0;
             }
         }
         if (f & HB_SPOIL) m.mx = m.my = 0.0f;
     }
-    if (nb > 0) flush<NT>(stg, wred, par, A, sa, nt, nb, row, rows, partial);
+    if (nb > 0) flush<NT>(stg, wred, par, S, nt, nb, row, rows, partial);
     store_state(cs.state, np, j, m);
 }
 
@@ -352,24 +411,24 @@ __global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restr
 }  // namespace hb
 
 namespace hb {
-template <bool NU, bool RF>
+template <int NUT, bool RF>
 static void launch_nt(const hb_case& cs, int s0, int s1, float* partial, int rows, const hb_nufft& nu, dim3 grid,
                       dim3 block, size_t lds, hipStream_t st) {
     if (cs.n_tiles == 1)
-        hipLaunchKernelGGL((forward_kernel<1, NU, RF>), grid, block, lds, st, cs, s0, s1, partial, rows, nu);
+        hipLaunchKernelGGL((forward_kernel<1, NUT, RF>), grid, block, lds, st, cs, s0, s1, partial, rows, nu);
     else if (cs.n_tiles == 2)
-        hipLaunchKernelGGL((forward_kernel<2, NU, RF>), grid, block, lds, st, cs, s0, s1, partial, rows, nu);
+        hipLaunchKernelGGL((forward_kernel<2, NUT, RF>), grid, block, lds, st, cs, s0, s1, partial, rows, nu);
     else
-        hipLaunchKernelGGL((forward_kernel<0, NU, RF>), grid, block, lds, st, cs, s0, s1, partial, rows, nu);
+        hipLaunchKernelGGL((forward_kernel<0, NUT, RF>), grid, block, lds, st, cs, s0, s1, partial, rows, nu);
 }
 
-template <bool NU>
+template <int NUT>
 static void launch_forward(const hb_case& cs, int s0, int s1, float* partial, int rows, const hb_nufft& nu,
                            dim3 grid, dim3 block, size_t lds, hipStream_t st) {
     if (cs.rf_free)
-        launch_nt<NU, false>(cs, s0, s1, partial, rows, nu, grid, block, lds, st);
+        launch_nt<NUT, false>(cs, s0, s1, partial, rows, nu, grid, block, lds, st);
     else
-        launch_nt<NU, true>(cs, s0, s1, partial, rows, nu, grid, block, lds, st);
+        launch_nt<NUT, true>(cs, s0, s1, partial, rows, nu, grid, block, lds, st);
 }
 }  // namespace hb
 
@@ -380,17 +439,21 @@ extern "C" int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* part
     if (cs->n_tiles < 1 || cs->n_tiles > kMaxTiles || 8 * cs->n_tiles < cs->n_coils) return hipErrorInvalidValue;
     if (s0 < 0 || s1 < s0 || s1 > cs->n_segments || rows < 0) return hipErrorInvalidValue;
     if (cs->n_pulses < 0 || (cs->n_pulses > 0 && !cs->maps)) return hipErrorInvalidValue;
-    if (nu && (!nu->params || !nu->win || !nu->part || nu->runs_max < 1 || nu->cols < 1)) return hipErrorInvalidValue;
+    if (nu && (!nu->params || !nu->win || !nu->part || nu->runs_max < 1 || nu->cols < 1 || nu->terms < 1 ||
+               nu->terms > HB_NUFFT_MAX_TERMS))
+        return hipErrorInvalidValue;
     if (s1 == s0) return hipSuccess;
     const dim3 grid(cs->n_pad / kBlock), block(kBlock);
     const size_t lds = sizeof(float) * lds_floats(cs->n_tiles);
     hipStream_t st = (hipStream_t)stream;
-    const hb_nufft none = {nullptr, nullptr, nullptr, 0, 0};
+    const hb_nufft none = {nullptr, nullptr, nullptr, 0, 0, 0, 0};
     const hb_nufft arg = nu ? *nu : none;
-    if (nu)
-        launch_forward<true>(*cs, s0, s1, partial, rows, arg, grid, block, lds, st);
+    if (!nu)
+        launch_forward<0>(*cs, s0, s1, partial, rows, arg, grid, block, lds, st);
+    else if (nu->terms <= 4)
+        launch_forward<4>(*cs, s0, s1, partial, rows, arg, grid, block, lds, st);
     else
-        launch_forward<false>(*cs, s0, s1, partial, rows, arg, grid, block, lds, st);
+        launch_forward<8>(*cs, s0, s1, partial, rows, arg, grid, block, lds, st);
     return hipGetLastError();
 }
 
@@ -418,4 +481,4 @@ extern "C" int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t 
 
 extern "C" const char* hb_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 
-extern "C" int hb_abi_version(void) { return 14; }
+extern "C" int hb_abi_version(void) { return 15; }

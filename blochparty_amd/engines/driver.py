@@ -261,6 +261,14 @@ def dispatch_starts(seq: Sequence, chunk: int, rows_cap: int, spans=(), rows_w=N
     return starts
 
 
+def pow2_scale(bound: float) -> float:
+    """The power of two 2^-e, e = ceil(log2(bound)), that brings values up to ``bound`` into [-1, 1] (1 for a bound
+    of 0 or not finite)."""
+    if not (bound > 0 and math.isfinite(bound)):
+        return 1.0
+    return math.ldexp(1.0, -min(100, max(-100, math.ceil(math.log2(bound)))))
+
+
 def sens_operand(coils: torch.Tensor, n: int, n_pad: int, n_tiles: int) -> torch.Tensor:
     """(N, C) complex sensitivities in the matrix-core A order: (n_pad / 64, tiles, 16 steps, 64 lanes), where lane
     l = 16 g + i of step u holds row i (Re S_0..7, Im S_0..7 of the tile) at spin 16 g + u of the wave."""
@@ -307,12 +315,14 @@ class DeviceCase:
     nu: dict | None = None  # NUFFT readouts: plan, device tables, work buffers, per-dispatch runs
     perm: np.ndarray | None = None  # the spins' order on the device (sorted for NUFFT readouts), or None
     rf_free: list[bool] = field(default_factory=list)  # per dispatch: no segment stepped as RF (hb_case.rf_free)
+    sens_scale: float = 1.0  # powers of two: |sensitivity| sens_scale <= 1, |M| x_scale <= 1 (hb_case)
+    x_scale: float = 1.0
 
     def struct(self) -> hip.HbCase:
         return hip.HbCase(self.spins.data_ptr(), self.pos_lo.data_ptr(), self.sens_a.data_ptr(),
                           self.segs.data_ptr(), self.state.data_ptr(),
                           None if self.maps is None else self.maps.data_ptr(), self.n, self.n_pad, self.n_coils,
-                          self.n_tiles, self.n_segments, self.n_pulses)
+                          self.n_tiles, self.n_segments, self.n_pulses, 0, self.sens_scale, self.x_scale, 0)
 
     @property
     def n_segments(self) -> int:
@@ -412,6 +422,12 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
         for a, _ in occ:
             covered[a : a + k] = True
     rf_cum = np.concatenate([[0], np.cumsum(np.asarray(seq.is_rf, dtype=bool) & ~covered)])
+    # the coil contraction's scales: |M| <= max(PD, |m_init|) along any sequence (rotations keep |M|, relaxation
+    # moves it toward (0, 0, PD))
+    sens_a = sens_operand(coils, n, n_pad, n_tiles)
+    sens_scale = pow2_scale(float(sens_a.abs().max()) if sens_a.numel() else 0.0)
+    x_scale = pow2_scale(max(float(phantom.pd.detach().abs().max()) if n else 0.0,
+                             float(m64.norm(dim=0).max()) if n else 0.0))
     psi_rows = np.remainder(seq.adc_phase.detach().double().cpu().numpy()[np.asarray(seq.adc, dtype=bool)],
                             2 * math.pi)
     max_rows = max(rows + [0])
@@ -419,13 +435,14 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
         n=n, n_pad=n_pad, n_coils=n_coils, n_tiles=n_tiles, n_adc=n_adc,
         spins=spins.to(dev).contiguous(),
         pos_lo=pos_lo.to(dev).contiguous(),
-        sens_a=sens_operand(coils, n, n_pad, n_tiles).to(dev),
+        sens_a=sens_a.to(dev),
         segs=torch.from_numpy(rec.view(np.uint8).copy()).to(dev),
         state=state.to(dev).contiguous(),
         state0=state.to(dev).contiguous(),
         psi_rows=torch.from_numpy(psi_rows.copy()).to(dev),
         starts=starts, ends=ends, row_starts=row_starts, rows=rows,
         rf_free=[bool(rf_cum[e] == rf_cum[s]) for s, e in zip(starts, ends)],
+        sens_scale=sens_scale, x_scale=x_scale,
         partial=torch.empty(max(1, n_blocks * max_rows * vals), dtype=torch.float32, device=dev),
         red_work=torch.empty(max(1, -(-n_blocks // REDUCE_CHUNK) * max_rows * vals), dtype=torch.float64,
                              device=dev),
@@ -497,7 +514,7 @@ def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> l
     nu_arg = None
     if nu is not None:
         nu_arg = hip.HbNufft(nu["nu"].data_ptr(), nu["nu_win"].data_ptr(), nu["part"].data_ptr(), nu["runs_max"],
-                             nu["cols"])
+                             nu["cols"], nu["plan"].terms, 0)
     for d, (s0, s1, r0, nr) in enumerate(zip(case.starts, case.ends, case.row_starts, case.rows)):
         if keep:
             ckpts.append(case.state.clone())

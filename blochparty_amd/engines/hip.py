@@ -15,7 +15,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent.parent / "libblochparty_amd.so"
-ABI_VERSION = 14
+ABI_VERSION = 15
 MAX_COILS = 128  # 16 tiles of 8 coils (the forward kernel's LDS budget)
 
 
@@ -36,6 +36,8 @@ class HbCase(ctypes.Structure):
         ("n_segments", ctypes.c_int32),
         ("n_pulses", ctypes.c_int32),
         ("rf_free", ctypes.c_int32),
+        ("sens_scale", ctypes.c_float),
+        ("x_scale", ctypes.c_float),
         ("reserved", ctypes.c_int32),
     ]
 
@@ -68,6 +70,8 @@ class HbNufft(ctypes.Structure):
         ("part", ctypes.c_void_p),
         ("runs_max", ctypes.c_int32),
         ("cols", ctypes.c_int32),
+        ("terms", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

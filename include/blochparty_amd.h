@@ -71,6 +71,11 @@ typedef struct hb_case {
     /* hb_forward: nonzero promises that no record of [s0, s1) is stepped as an RF segment (every pulse there is
      * applied as a map), so the kernel may leave the RF step out; 0 is always correct. Ignored elsewhere. */
     int32_t rf_free;
+    /* hb_forward: powers of two (0 reads as 1) with |sensitivity| * sens_scale <= 1 and |M| * x_scale <= 1 for every
+     * magnetization the case can reach (|M| <= max(PD, |m_init|)): the coil contraction works on these scaled values
+     * (as float16 hi + lo pairs for up to 16 coils) and undoes the scales exactly. */
+    float sens_scale;
+    float x_scale;
     int32_t reserved; /* 0 */
 } hb_case;
 
@@ -84,6 +89,8 @@ typedef struct hb_nufft {
     const int32_t* win;
     float* part;
     int32_t runs_max, cols;
+    int32_t terms;    /* = params[2], 1..HB_NUFFT_MAX_TERMS (selects the kernel variant) */
+    int32_t reserved; /* 0 */
 } hb_nufft;
 
 /* Forward event loop over segments [s0, s1) (replaces bloch_chunk_mma, cuda/kernels.cu.in:2-185, one launch per
