@@ -54,8 +54,8 @@ __device__ __forceinline__ void reverse_step(const hb_segment& q, int f, const S
     if (f & HB_RF) {
         const float h = 0.5f * dt;
         if (!(f & HB_SAME_RF_DT)) {
-            rc.h1 = expm1f(-h * p.r1);
-            rc.h2 = expm1f(-h * p.r2);
+            rc.h1 = expm1x(-h * p.r1);
+            rc.h2 = expm1x(-h * p.r2);
         }
         const float d1 = rc.h1, d2 = rc.h2, e1 = 1.0f + d1, e2 = 1.0f + d2;
         const float wx0 = w * q.rf_x, wy0 = w * q.rf_y;
@@ -102,9 +102,9 @@ __device__ __forceinline__ void reverse_step(const hb_segment& q, int f, const S
         }
     } else {
         if (!(f & HB_SAME_FREE)) {
-            sincospif(-2.0f * free_cycles(p, q), &rc.sn, &rc.c);
-            rc.d1 = expm1f(-dt * p.r1);
-            rc.d2 = expm1f(-dt * p.r2);
+            sincospi(-2.0f * free_cycles(p, q), &rc.sn, &rc.c);
+            rc.d1 = expm1x(-dt * p.r1);
+            rc.d2 = expm1x(-dt * p.r2);
         }
         const float c = rc.c, sn = rc.sn, e2 = 1.0f + rc.d2, e1 = 1.0f + rc.d1;
         const float ux = c * x0 - sn * y0, uy = sn * x0 + c * y0;

@@ -189,13 +189,13 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
     double ph = fr * shift;
     ph -= rint(ph);
     float sp, cp;
-    sincospif(-2.0f * (float)ph, &sp, &cp);
+    sincospi(-2.0f * (float)ph, &sp, &cp);
     const float cr = (m.mx * cp - m.my * sp) * S.x_scale, ci = (m.mx * sp + m.my * cp) * S.x_scale;
     // the weight (times the staged values' scale) times each decay term e^{-R2 tau_m}
     float wr[NUT], wi[NUT];
 #pragma unroll
     for (int t = 0; t < NUT; ++t) {
-        const float e = t < terms ? expf(-nu.params[6 + t] * p.r2) : 0.0f;
+        const float e = t < terms ? __builtin_amdgcn_exp2f(-(nu.params[6 + t] * 1.4426950408889634f) * p.r2) : 0.0f;
         wr[t] = e * cr;
         wi[t] = e * ci;
     }
@@ -236,13 +236,13 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
     double cl = cyc * (double)L;
     cl -= rint(cl);
     float s2, c2;
-    sincospif(-2.0f * (float)cl, &s2, &c2);
+    sincospi(-2.0f * (float)cl, &s2, &c2);
     const float T = (float)L * q.dt;
-    const float e2 = expm1f(-T * p.r2);
+    const float e2 = expm1x(-T * p.r2);
     const float nx = c2 * m.mx - s2 * m.my, ny = s2 * m.mx + c2 * m.my;
     m.mx = nx + nx * e2;
     m.my = ny + ny * e2;
-    two_sum(m.mz, m.lz, (m.mz - p.pd) * expm1f(-T * p.r1));
+    two_sum(m.mz, m.lz, (m.mz - p.pd) * expm1x(-T * p.r1));
     if (q.flags & HB_PULSE_SPOIL) m.mx = m.my = 0.0f;
 }
 
@@ -307,11 +307,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
         if (f & HB_RUN) {  // L identical free ADC segments: Mxy stepped and sampled per segment, Mz relaxed once
             const int L = q.pad[3];
             if (!(f & HB_SAME_FREE)) {
-                sincospif(-2.0f * free_cycles(p, q), &k.s, &k.c);
-                k.e1 = expm1f(-q.dt * p.r1);
-                k.e2 = expm1f(-q.dt * p.r2);
+                sincospi(-2.0f * free_cycles(p, q), &k.s, &k.c);
+                k.e1 = expm1x(-q.dt * p.r1);
+                k.e2 = expm1x(-q.dt * p.r2);
             }
-            if (!(f & HB_SAME_RUN)) e1_run = expm1f(-(float)L * q.dt * p.r1);
+            if (!(f & HB_SAME_RUN)) e1_run = expm1x(-(float)L * q.dt * p.r1);
             for (int i = 0; i < L; ++i) {
                 const float nx = k.c * m.mx - k.s * m.my, ny = k.s * m.mx + k.c * m.my;
                 m.mx = nx + nx * k.e2;
@@ -330,8 +330,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
         }
         if (f & HB_PULSE) {  // a reused pulse: m += Rz(psi) (D Rz(-psi) m + v), then skip its segments
             const float* U = cs.maps + (size_t)q.pad[1] * 12 * np + j;  // loaded per occurrence: no registers held
-            float sph, cph;
-            sincospif(q.psi * 0.3183098861837907f, &sph, &cph);
+            const float cph = q.w_rf_x, sph = q.w_rf_y;  // cos, sin of the occurrence's RF phase (host float64)
             const float ux = cph * m.mx + sph * m.my, uy = cph * m.my - sph * m.mx, uz = m.mz;
             const float dux = U[0] * ux + U[np] * uy + U[2 * np] * uz + U[9 * np];
             const float duy = U[3 * np] * ux + U[4 * np] * uy + U[5 * np] * uz + U[10 * np];
@@ -344,7 +343,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
             continue;
         }
         if (f & HB_SPOILED_FREE) {  // only Mz survives the spoil: relax it, zero Mxy
-            two_sum(m.mz, m.lz, (m.mz - p.pd) * expm1f(-q.dt * p.r1));
+            two_sum(m.mz, m.lz, (m.mz - p.pd) * expm1x(-q.dt * p.r1));
             m.mx = m.my = 0.0f;
             if (f & HB_MERGED) s += q.pad[3] - 1;
             continue;

@@ -113,6 +113,9 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
     w = -2 * math.pi * dt  # the rotation-vector factors, formed in float64
     rec["w"] = np.stack([w, w * rf[:, 0], w * rf[:, 1], w * g[:, 0], w * g[:, 1], w * g[:, 2], w * b0,
                          (w * rf[:, 0]) ** 2 + (w * rf[:, 1]) ** 2], 1).astype(np.float32)
+    for _, _, occ in pulses:  # a reused pulse's head is never stepped: its w_rf holds cos, sin of the RF phase
+        for a, ph in occ:
+            rec["w"][a, 1], rec["w"][a, 2] = math.cos(ph), math.sin(ph)
     rec["pad"][:, 0] = np.where(adc, np.cumsum(adc) - 1, -1)  # ADC row
     ends_adc = np.flatnonzero((flags & MERGED != 0) & (flags & ADC != 0))  # merged runs sampled at their end
     rec["pad"][ends_adc, 0] = rec["pad"][ends_adc + rec["pad"][ends_adc, 3] - 1, 0]

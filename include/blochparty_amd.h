@@ -45,8 +45,8 @@ extern "C" {
  * (k . r reduced mod 1 cycle per segment). The w block holds the segment's rotation-vector factors, formed in float64
  * on the host: w = -2 pi dt, w_rf = w rf, w_g = w g, w_b0 = w b0_offset, w_rf2 = |w_rf|^2 (a spin's RF rotation
  * vector is (b1 w_rf, w_g . r + w db0 + w_b0)). pad[0] is the ADC row of an ADC segment (the reference's adc_row);
- * a reused pulse's first segment (HB_PULSE) holds the pulse id in pad[1], its length in pad[2] and the occurrence's
- * RF phase in psi (driver.py:592-596); a readout run's (HB_RUN), an RF train's (HB_TRAIN) or a merged free run's
+ * a reused pulse's first segment (HB_PULSE) holds the pulse id in pad[1], its length in pad[2], the occurrence's
+ * RF phase in psi (driver.py:592-596) and its cosine and sine in w_rf_x, w_rf_y (that segment is never stepped); a readout run's (HB_RUN), an RF train's (HB_TRAIN) or a merged free run's
  * (HB_MERGED) first segment its length in pad[3]. */
 typedef struct hb_segment {
     float dt, rf_x, rf_y, g_x, g_y, g_z, b0, psi;
