@@ -7,13 +7,14 @@
 //
 // Coil contraction, per wave: each ADC sample leaves Mxy of the wave's 64 spins; eight samples are staged in LDS as
 // the columns of B (64 x 16: Re Mxy of samples 0..7, then Im), and D = A B on the matrix cores with A (16 x 64) the
-// wave's sensitivities of one tile of 8 coils, rows Re S_0..7, Im S_0..7, held in registers in the operand order.
-// With the K index of step u at lane group g being spin 16 g + u, lane l supplies A[l % 16][16 (l / 16) + u] and
-// B[16 (l / 16) + u][l % 16] and receives D[4 (l / 16) + r][l % 16], r = 0..3. The complex sums are then
-//     P_c(t) = D[c][t] - D[8 + c][8 + t],   Q_c(t) = D[c][8 + t] + D[8 + c][t]      (signal before demodulation),
-// one cross-lane exchange per value (a swap of the wave's halves and a DPP rotation). The four waves' sums are added
-// in LDS in a fixed order and written as this workgroup's partial row; hb_signal_reduce adds the workgroups in
-// float64, in order. No atomics: the signal is the same bit for bit on every run.
+// wave's sensitivities of one tile of 8 coils, rows Re S_0, Im S_0, Re S_1, Im S_1, ..., held in registers in the
+// operand order. With the K index of step u at lane group g being spin 16 g + u, lane l supplies
+// A[l % 16][16 (l / 16) + u] and B[16 (l / 16) + u][l % 16] and receives D[4 (l / 16) + r][l % 16], r = 0..3. The
+// complex sums are then
+//     P_c(t) = D[2c][t] - D[2c + 1][8 + t],   Q_c(t) = D[2c][8 + t] + D[2c + 1][t]   (signal before demodulation),
+// the partner column 8 lanes away in the same row of 16 (a DPP rotation). The four waves' sums are added in LDS in a
+// fixed order and written as this workgroup's partial row; hb_signal_reduce adds the workgroups in float64, in
+// order. No atomics: the signal is the same bit for bit on every run.
 //
 // Precision of the product: for up to two tiles (16 coils) both operands are split into float16 pairs, x = hi + lo
 // (x scaled into [-1, 1] by powers of two), and D = Ah Bh + Ah Bl + Al Bh on v_mfma_f32_16x16x32_f16: float16
@@ -38,6 +39,11 @@ constexpr int kMaxTiles = 16;   // 128 coils: LDS = 4 x 16 x 68 + 4 x 8 x 256 fl
 
 // LDS: every wave's staging block, then two buffers of the waves' coil sums (flushes alternate between them).
 __host__ __device__ constexpr int lds_floats(int nt) { return kWaves * kStgFloats + 2 * kWaves * 8 * 16 * nt; }
+
+// x of lane (jj + 8) mod 16 of this lane's row of 16 (DPP row_ror:8).
+__device__ __forceinline__ float dpp_ror8(float x) {
+    return __uint_as_float(__builtin_amdgcn_update_dpp(0u, __float_as_uint(x), 0x128, 0xf, 0xf, false));
+}
 
 // x = hi + lo in float16 for eight floats: hi rounded to nearest, lo the rounded remainder (exact in float32). For
 // |x| <= 1 the pair holds x to 2^-24 |x| or 2^-25, whichever is larger (lo may be subnormal: the matrix cores take
@@ -111,28 +117,23 @@ __device__ __forceinline__ void flush_to(float* stg, float* wred2, int& par, con
 #pragma unroll
             for (int u = 0; u < 16; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(at[u * 64], b[u], acc, 0, 0, 0);
         }
-        // lane l < 32 holds D[c][jj] (c = 4 g + r) and needs D[8 + c][jj +- 8], which sits in lane l + 32 +- 8:
-        // swap the wave's halves (v_permlane32_swap), then rotate each row of 16 lanes by 8 (DPP row_ror:8); no LDS
-        f4 v;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const unsigned up = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc[r]), __float_as_uint(acc[r]),
-                                                                 false, false)[1];
-            const float other = __uint_as_float(__builtin_amdgcn_update_dpp(0u, up, 0x128, 0xf, 0xf, false));
-            v[r] = (jj < 8) ? acc[r] - other : acc[r] + other;
-        }
-        // this wave's sums as [sample][tile][re, im][coil]: coils 4 g .. 4 g + 3 contiguous, one 16-byte store
-        if (lane < 32) *reinterpret_cast<f4*>(mine + (jj & 7) * vals + 16 * t + 8 * (jj >> 3) + 4 * g) = v;
+        // rows 4 g .. 4 g + 3 of lane (g, jj) are Re S_c0, Im S_c0, Re S_c1, Im S_c1 (c0 = 2 g, c1 = 2 g + 1) at
+        // column jj; the partner column jj +- 8 sits in the same row of 16 lanes: DPP row_ror:8. Lanes jj < 8 form
+        // coil c0's (P, Q) at sample jj, lanes jj >= 8 coil c1's at sample jj - 8.
+        const float r0 = dpp_ror8(acc[0]), r1 = dpp_ror8(acc[1]), r2 = dpp_ror8(acc[2]), r3 = dpp_ror8(acc[3]);
+        const bool first = jj < 8;
+        const f2 pq = first ? (f2){acc[0] - r1, r0 + acc[1]} : (f2){r2 - acc[3], acc[2] + r3};
+        // this wave's sums in the output order [sample][tile][coil][re, im]: one 8-byte store per lane
+        *reinterpret_cast<f2*>(mine + (jj & 7) * vals + 16 * t + 2 * (2 * g + (jj >> 3))) = pq;
     }
     __syncthreads();
-    // columns [0, k_first) to out, [k_first, k_first + k_skip) dropped, the rest to out_rest; output value
-    // v = sample * vals + 16 tile + 2 coil + {0: re, 1: im} reads the waves' sums at [sample][tile][re, im][coil]
+    // columns [0, k_first) to out, [k_first, k_first + k_skip) dropped, the rest to out_rest (output value
+    // v = sample * vals + 16 tile + 2 coil + {0: re, 1: im})
     const int n_out = n_out_cols * vals, split = k_first * vals, resume = (k_first + k_skip) * vals;
     for (int v = tid; v < n_out; v += kBlock) {
-        const int src = (v & ~15) | ((v & 1) << 3) | ((v & 15) >> 1);
-        float s = wred[src];
+        float s = wred[v];
 #pragma unroll
-        for (int w = 1; w < kWaves; ++w) s += wred[w * 8 * vals + src];
+        for (int w = 1; w < kWaves; ++w) s += wred[w * 8 * vals + v];
         s *= S.out_scale;
         if (v < split)
             out[v] = s;
@@ -207,8 +208,9 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
     // z = (cell - x) 2 / width: at the window's first cell from the float64 position (x ~ ng would leave a float32
     // z 1e-6 off), then by float32 steps (|z| error under 3e-7 over a 32-cell window)
     const float z0 = (float)(((double)wb - xg) * (double)h);
-    for (int lc = 0; lc < W; ++lc) {
-        const float z = fmaf((float)lc, h, z0);
+    float lcf = 0.0f;  // lc as a float
+    for (int lc = 0; lc < W; ++lc, lcf += 1.0f) {
+        const float z = fmaf(lcf, h, z0);
         const float u = fmaf(-z, z, 1.0f);
         // the ES kernel e^{beta (sqrt(1 - z^2) - 1)} = 2^{beta log2(e) (sqrt(u) - 1)}, zero outside |z| < 1, on the
         // hardware square root and exp2 (about 1 ulp each; the argument's rounding, |arg| < 20, leaves the
@@ -232,18 +234,21 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
         flush_cols<NT>(stg, wred, par, S, nt, nb, pending, row, rows, partial, out + (size_t)col * vals);
         pending = 0;
     }
-    // the jump over the run: phase L cyc, relaxation over L dt
-    double cl = cyc * (double)L;
-    cl -= rint(cl);
-    float s2, c2;
-    sincospi(-2.0f * (float)cl, &s2, &c2);
+    // the jump over the run: phase L cyc, relaxation over L dt (Mz only when the run's Mxy is never read)
     const float T = (float)L * q.dt;
-    const float e2 = expm1x(-T * p.r2);
-    const float nx = c2 * m.mx - s2 * m.my, ny = s2 * m.mx + c2 * m.my;
-    m.mx = nx + nx * e2;
-    m.my = ny + ny * e2;
+    if (q.flags & (HB_XY_DEAD | HB_PULSE_SPOIL)) {
+        m.mx = m.my = 0.0f;
+    } else {
+        double cl = cyc * (double)L;
+        cl -= rint(cl);
+        float s2, c2;
+        sincospi(-2.0f * (float)cl, &s2, &c2);
+        const float e2 = expm1x(-T * p.r2);
+        const float nx = c2 * m.mx - s2 * m.my, ny = s2 * m.mx + c2 * m.my;
+        m.mx = nx + nx * e2;
+        m.my = ny + ny * e2;
+    }
     two_sum(m.mz, m.lz, (m.mz - p.pd) * expm1x(-T * p.r1));
-    if (q.flags & HB_PULSE_SPOIL) m.mx = m.my = 0.0f;
 }
 
 // NT: coil tiles (0: any, the sensitivities read from memory per flush); NUT: NUFFT runs with up to NUT decay terms

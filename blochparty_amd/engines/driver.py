@@ -30,7 +30,7 @@ from . import hip, pulses
 
 BLOCK = 256
 RF, ADC, SPOIL, SAME_FREE, SAME_RF_DT, NEXT_FREE, NEXT_RF_DT, PULSE, PULSE_SPOIL, RUN, SAME_RUN, TRAIN, MERGED, \
-    SPOILED_FREE, NUFFT = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384)
+    SPOILED_FREE, NUFFT, XY_DEAD = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384, 32768)
 NUFFT_RUNS = 64  # NUFFT runs per dispatch at most (slots of the grid partials)
 MIN_RUN = 4  # readout runs shorter than this are stepped segment by segment
 MIN_TRAIN = 4  # likewise RF trains
@@ -95,6 +95,10 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
     if runs:
         merge_free(flags, rec, is_rf, adc, spoil, chunk_id, stepped)
         zapped = zap_spoiled(flags, rec, is_rf, spoil, chunk_id, stepped)
+        if nufft is not None:  # a run followed by a Mz-only record: its Mxy at the end is never read
+            for a, k in nufft.runs:
+                if a + k < S and zapped[a + k]:
+                    flags[a] |= XY_DEAD
     # the free segments' cache key in moments (equal to (dt, g, b0_offset) for single segments; merged ones carry
     # their sums)
     fkey = rec["d"].copy()
@@ -274,14 +278,15 @@ def pow2_scale(bound: float) -> float:
 
 def sens_operand(coils: torch.Tensor, n: int, n_pad: int, n_tiles: int) -> torch.Tensor:
     """(N, C) complex sensitivities in the matrix-core A order: (n_pad / 64, tiles, 16 steps, 64 lanes), where lane
-    l = 16 g + i of step u holds row i (Re S_0..7, Im S_0..7 of the tile) at spin 16 g + u of the wave."""
+    l = 16 g + i of step u holds row i (Re S_0, Im S_0, Re S_1, Im S_1, ... of the tile) at spin 16 g + u of the
+    wave."""
     C = coils.shape[1]
     full = torch.zeros(n_pad, 8 * n_tiles, dtype=torch.complex128)
     full[:n, :C] = coils.detach().to(torch.complex128).cpu()
     w = n_pad // 64
     re = full.real.reshape(w, 64, n_tiles, 8)
     im = full.imag.reshape(w, 64, n_tiles, 8)
-    m = torch.cat([re, im], dim=3).permute(0, 2, 3, 1)  # (w, tiles, 16 rows i, 64 spins k)
+    m = torch.stack([re, im], dim=4).reshape(w, 64, n_tiles, 16).permute(0, 2, 3, 1)  # (w, tiles, 16 rows, 64 k)
     m = m.reshape(w, n_tiles, 16, 4, 16).permute(0, 1, 4, 3, 2)  # k = 16 g + u -> (w, tiles, u, g, i)
     return m.reshape(w, n_tiles, 16, 64).float().contiguous()
 

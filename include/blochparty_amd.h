@@ -33,10 +33,14 @@ extern "C" {
 #define HB_RUN 512        /* first of pad[3] identical free ADC segments: step Mxy per sample, Mz once        */
 #define HB_SAME_RUN 1024  /* a run with the previous run's dt and length (reuse its Mz factor)              */
 #define HB_TRAIN 2048     /* first of pad[3] consecutive RF segments without ADC or spoil: stepped in one loop */
-#define HB_MERGED 4096    /* first of pad[3] free segments without ADC merged into it (summed dt and moments)   */
-#define HB_SPOILED_FREE 8192 /* free segment spoiled at its end without ADC: Mz relaxes, Mxy is zeroed          */
+#define HB_MERGED 4096    /* first of pad[3] free segments merged into it (summed dt and moments); only the last
+                             may be sampled (HB_ADC: the sample is taken after the merged step)                 */
+#define HB_SPOILED_FREE 8192 /* free record without ADC whose Mxy never reaches a sample (spoiled at its end, or
+                             starting at Mxy = 0): Mz relaxes over its dt, Mxy is zeroed                        */
 #define HB_NUFFT 16384    /* first of pad[3] readout samples taken as a NUFFT: slot pad[1], kind pad[2]; a spoil
                              at the run's end is HB_PULSE_SPOIL                                                 */
+#define HB_XY_DEAD 32768  /* a NUFFT run whose Mxy at its end is never read (the next record is HB_SPOILED_FREE):
+                             the run's jump relaxes Mz only and zeroes Mxy                                     */
 #define HB_NUFFT_MAX_TERMS 8
 
 /* One segment of the event list, 128 bytes (the reference's segs row + flags + rows, driver.py:357-384). psi is the
@@ -58,7 +62,7 @@ typedef struct hb_segment {
 /* Device buffers of one case (driver.py:320-433). spins: (8, n_pad) px py pz pd r1 r2 db0 b1; pos_lo: (3, n_pad)
  * the float64 positions' remainders (pos = hi + lo) for the float64 k.r phase of free segments; sens_a: coil
  * sensitivities in the matrix-core operand order, (n_pad / 64, n_tiles, 16, 64) floats, n_tiles = ceil(C / 8), rows
- * of a tile Re S_0..7 then Im S_0..7. state: (6, n_pad) mx my mz and their low parts (the reference's M and L
+ * of a tile Re S_0, Im S_0, Re S_1, Im S_1, ... state: (6, n_pad) mx my mz and their low parts (the reference's M and L
  * buffers); the kernels carry a two-sum low part for mz only and fold the transverse ones in when they load. */
 typedef struct hb_case {
     const float* spins;

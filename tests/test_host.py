@@ -69,12 +69,14 @@ def test_sensitivity_operand_order_gives_the_coil_sums(n, C):
             D = sum(mfma_16x16x4(A[w, t, u], [stg[lane % 16, 16 * (lane // 16) + u] for lane in range(64)])
                     for u in range(16))
             acc = np.array([[D[4 * (lane // 16) + r, lane % 16] for r in range(4)] for lane in range(64)])
-            for lane in range(32):
+            for lane in range(64):  # rows of lane group gg: Re S_2gg, Im S_2gg, Re S_2gg+1, Im S_2gg+1
                 jj, gg = lane % 16, lane // 16
-                partner = lane + (40 if jj < 8 else 24)
-                for r in range(4):
-                    v = acc[lane, r] - acc[partner, r] if jj < 8 else acc[lane, r] + acc[partner, r]
-                    out[jj & 7, 8 * t + 4 * gg + r] += v if jj < 8 else 1j * v
+                rot = acc[16 * gg + (jj + 8) % 16]  # DPP row_ror:8
+                if jj < 8:
+                    p, q = acc[lane, 0] - rot[1], rot[0] + acc[lane, 1]
+                else:
+                    p, q = rot[2] - acc[lane, 3], acc[lane, 2] + rot[3]
+                out[jj & 7, 8 * t + 2 * gg + (jj >> 3)] += p + 1j * q
     ref = M[:n].T @ np.concatenate([coils.numpy(), np.zeros((n, 8 * nt - C))], 1)
     assert np.abs(out - ref).max() < 1e-5 * np.abs(ref).max()
 

@@ -96,6 +96,9 @@ def test_records_and_dispatches_keep_runs_whole():
     for (a, k), kind in zip(plan.runs, plan.kind):
         assert fl[a] & driver.NUFFT and rec["pad"][a, 3] == k and rec["pad"][a, 2] == kind
         assert not (fl[a + 1 : a + k] & (driver.RUN | driver.MERGED | driver.NUFFT)).any()
+    # a run whose Mxy is never read (the next record relaxes Mz only) skips the precession of its jump
+    dead = [a for a, k in plan.runs if fl[a] & driver.XY_DEAD]
+    assert dead and all(fl[a + k] & driver.SPOILED_FREE for a, k in plan.runs if fl[a] & driver.XY_DEAD)
     # slots count up within each dispatch
     for s0, s1 in zip(starts, ends):
         sl = [rec["pad"][h, 1] for h in heads if s0 <= h < s1]
