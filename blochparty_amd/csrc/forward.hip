@@ -52,7 +52,13 @@ __device__ __forceinline__ void split8(const float* x, h8& hi, h8& lo) {
 #pragma unroll
     for (int e = 0; e < 8; e += 2) {  // in pairs: one v_cvt_pk_f16_f32 per pair of hi and of lo parts
         const h2 h = __builtin_convertvector((f2){x[e], x[e + 1]}, h2);
-        const h2 l = __builtin_convertvector((f2){x[e] - (float)h[0], x[e + 1] - (float)h[1]}, h2);
+        // x - float(h) in one v_fma_mix_f32 per value, the float16 operand taken from its half of the packed pair
+        // (the compiler forms a conversion and a subtraction); exact, float16 subnormals included
+        const unsigned hu = __builtin_bit_cast(unsigned, h);
+        float r0, r1;
+        asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r0) : "v"(hu), "v"(x[e]));
+        asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r1) : "v"(hu), "v"(x[e + 1]));
+        const h2 l = __builtin_convertvector((f2){r0, r1}, h2);
         hi[e] = h[0], hi[e + 1] = h[1];
         lo[e] = l[0], lo[e + 1] = l[1];
     }

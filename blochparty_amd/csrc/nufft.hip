@@ -2,12 +2,12 @@
 // into the grid rows and its host-side FFT and recombination, src/blochparty/engines/gpu/driver.py:549-580,
 // 703-725).
 //
-// hb_nufft_grid: every grid cell of every run slot is the sum, in workgroup order, of the window columns of the
-// workgroups whose windows hold the cell (the lists are built on the host from the windows): one lane per (slot,
-// term, cell, value), float64, no atomics, so the grids are the same bit for bit on every run.
+// hb_nufft_grid: every grid cell of every run slot is the sum, in a fixed order, of the window columns of the
+// workgroups whose windows hold the cell (the lists are built on the host from the windows): float64, no atomics,
+// so the grids are the same bit for bit on every run.
 // hb_nufft_signal: the readout's samples are the grids' DFT at the n modes j = i - n / 2 (the modes the samples
 // are), divided by the kernel's transform and recombined over the decay terms by their Lagrange weights (folded
-// into lw), then demodulated by the sample's receiver phase: one lane per (slot, sample, coil), float64.
+// into lw), then demodulated by the sample's receiver phase; float64.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -15,61 +15,111 @@
 
 namespace hb {
 
-__global__ __launch_bounds__(256) void nufft_grid_kernel(const float* __restrict__ part, int runs_max, int cols,
-                                                         int vals, int n_slots, const int* __restrict__ run_of,
-                                                         const int* __restrict__ kind_of, int ng, int terms,
-                                                         const int* __restrict__ csr_off,
-                                                         const int* __restrict__ csr,
-                                                         const int* __restrict__ csr_base,
-                                                         double* __restrict__ grid) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t total = (size_t)n_slots * terms * ng * vals;
-    if (idx >= total) return;
-    const int v = (int)(idx % vals);
-    const int u = (int)((idx / vals) % ng);
-    const int m = (int)((idx / ((size_t)vals * ng)) % terms);
-    const int r = (int)(idx / ((size_t)vals * ng * terms));
+constexpr int kGridBlock = 256;
+constexpr int kBatch = 8;  // list entries per step of a lane's gather
+
+// One workgroup per (cell u, slot r): the cell's list entries (block b, local cell lc) are taken by G groups of
+// terms x vals lanes, group k the entries k, k + G, ... (each entry's terms x vals floats are contiguous in part:
+// one coalesced read), each lane summing in float64; then lane q adds the G groups' sums in order. Fixed order
+// throughout: the grid is the same bit for bit on every run.
+__global__ __launch_bounds__(kGridBlock) void nufft_grid_kernel(const float* __restrict__ part, int runs_max, int cols,
+                                                                int vals, const int* __restrict__ run_of,
+                                                                const int* __restrict__ kind_of, int ng, int terms,
+                                                                const int* __restrict__ csr_off,
+                                                                const int* __restrict__ csr,
+                                                                const int* __restrict__ csr_base,
+                                                                double* __restrict__ grid) {
+    __shared__ double sums[kGridBlock];
+    const int u = blockIdx.x, r = blockIdx.y, tid = threadIdx.x;
+    const int vt = terms * vals;
+    const int G = vt >= kGridBlock ? 1 : kGridBlock / vt;  // entry groups
     const int kind = kind_of[run_of[r]];
     const int* off = csr_off + (size_t)kind * (ng + 1);
     const int* ent = csr + 2 * (size_t)csr_base[kind];
-    double acc = 0.0;
-    for (int e = off[u]; e < off[u + 1]; ++e) {
-        const int b = ent[2 * e], lc = ent[2 * e + 1];
-        acc += (double)part[(((size_t)b * runs_max + r) * cols + lc * terms + m) * vals + v];
+    const int e0 = off[u], e1 = off[u + 1];
+    for (int q0 = 0; q0 < vt; q0 += kGridBlock) {  // value chunks (one unless terms x vals > 256)
+        const int k = tid / (G > 1 ? vt : kGridBlock), q = q0 + tid % (G > 1 ? vt : kGridBlock);
+        double acc = 0.0;
+        if (k < G && q < vt) {
+            // kBatch entries at a time: their list reads, then their value reads, are all in flight together (the
+            // sum still takes them in entry order; an entry past the end adds an exact 0)
+            for (int e = e0 + k; e < e1; e += kBatch * G) {
+                int2 be[kBatch];
+#pragma unroll
+                for (int j = 0; j < kBatch; ++j) {
+                    const int ee = e + j * G;
+                    be[j] = ee < e1 ? reinterpret_cast<const int2*>(ent)[ee] : make_int2(-1, 0);
+                }
+                float x[kBatch];
+#pragma unroll
+                for (int j = 0; j < kBatch; ++j)
+                    x[j] = be[j].x >= 0
+                               ? part[(((size_t)be[j].x * runs_max + r) * cols + be[j].y * terms) * vals + q]
+                               : 0.0f;
+#pragma unroll
+                for (int j = 0; j < kBatch; ++j) acc += (double)x[j];
+            }
+        }
+        sums[tid] = acc;
+        __syncthreads();
+        const int qq = q0 + tid;
+        if (tid < (G > 1 ? vt : kGridBlock) && qq < vt) {
+            double t = sums[tid];
+            for (int g = 1; g < G; ++g) t += sums[g * vt + tid];
+            const int m = qq / vals, v = qq % vals;
+            grid[((size_t)(r * terms + m) * ng + u) * vals + v] = t;
+        }
+        __syncthreads();
     }
-    grid[idx] = acc;
 }
 
-__global__ __launch_bounds__(256) void nufft_signal_kernel(const double* __restrict__ grid, int n_slots,
+constexpr int kSigParts = 32;  // lanes per coil, each summing the cells u = p, p + 32, ...
+
+// One workgroup per (sample i, slot r): lanes (coil c of a group of 8, part p) sum the DFT over their cells and
+// every decay term, weighted; the 32 parts are added in order; then the receiver phase is taken off.
+__global__ __launch_bounds__(256) void nufft_signal_kernel(const double* __restrict__ grid,
                                                            const int* __restrict__ run_of,
                                                            const int* __restrict__ row_of, int n, int ng, int terms,
                                                            int vals, int n_coils, const double* __restrict__ tw,
                                                            const double* __restrict__ lw,
                                                            const double* __restrict__ psi, double* __restrict__ sig) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n_slots * n * n_coils) return;
-    const int c = idx % n_coils, i = (idx / n_coils) % n, r = idx / (n_coils * n);
+    __shared__ double red[2][256];
+    const int i = blockIdx.x, r = blockIdx.y, tid = threadIdx.x;
+    const int cl = tid % 8, p = tid / 8;
     const double* twi = tw + (size_t)i * ng * 2;
-    double re = 0.0, im = 0.0;
-    for (int m = 0; m < terms; ++m) {
-        const double* g = grid + ((size_t)(r * terms + m) * ng) * vals + 2 * c;
-        double fr = 0.0, fi = 0.0;
-        for (int u = 0; u < ng; ++u) {
-            const double gr = g[(size_t)u * vals], gi = g[(size_t)u * vals + 1];
-            const double cr = twi[2 * u], ci = twi[2 * u + 1];
-            fr += gr * cr - gi * ci;
-            fi += gr * ci + gi * cr;
-        }
-        const double wgt = lw[(size_t)m * n + i];
-        re += wgt * fr;
-        im += wgt * fi;
-    }
     const int row = row_of[run_of[r]] + i;
     double sp, cp;
     sincos(psi[row], &sp, &cp);
-    double* out = sig + ((size_t)row * n_coils + c) * 2;
-    out[0] = re * cp + im * sp;
-    out[1] = im * cp - re * sp;
+    for (int c0 = 0; c0 < n_coils; c0 += 8) {
+        const int c = c0 + cl;
+        double re = 0.0, im = 0.0;
+        if (c < n_coils) {
+            for (int m = 0; m < terms; ++m) {
+                const double* g = grid + ((size_t)(r * terms + m) * ng) * vals + 2 * c;
+                double fr = 0.0, fi = 0.0;
+                for (int u = p; u < ng; u += kSigParts) {
+                    const double gr = g[(size_t)u * vals], gi = g[(size_t)u * vals + 1];
+                    const double tr = twi[2 * u], ti = twi[2 * u + 1];
+                    fr += gr * tr - gi * ti;
+                    fi += gr * ti + gi * tr;
+                }
+                const double wgt = lw[(size_t)m * n + i];
+                re += wgt * fr;
+                im += wgt * fi;
+            }
+        }
+        red[0][tid] = re;
+        red[1][tid] = im;
+        __syncthreads();
+        if (tid < 8 && c < n_coils) {
+            double sr = 0.0, si = 0.0;
+            for (int q = 0; q < kSigParts; ++q) sr += red[0][q * 8 + tid], si += red[1][q * 8 + tid];
+            double* out = sig + ((size_t)row * n_coils + c) * 2;
+            out[0] = sr * cp + si * sp;
+            out[1] = si * cp - sr * sp;
+        }
+        __syncthreads();
+    }
 }
 
 }  // namespace hb
@@ -82,10 +132,8 @@ extern "C" int hb_nufft_grid(const float* part, int32_t runs_max, int32_t cols, 
         n_tiles < 1 || n_slots < 0 || n_slots > runs_max || ng < 2 || terms < 1 || terms > HB_NUFFT_MAX_TERMS)
         return hipErrorInvalidValue;
     if (n_slots == 0) return hipSuccess;
-    const size_t total = (size_t)n_slots * terms * ng * 16 * n_tiles;
-    hipLaunchKernelGGL(hb::nufft_grid_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                       (hipStream_t)stream, part, runs_max, cols, 16 * n_tiles, n_slots, run_of, kind_of, ng, terms,
-                       csr_off, csr, csr_base, grid);
+    hipLaunchKernelGGL(hb::nufft_grid_kernel, dim3(ng, n_slots), dim3(hb::kGridBlock), 0, (hipStream_t)stream, part,
+                       runs_max, cols, 16 * n_tiles, run_of, kind_of, ng, terms, csr_off, csr, csr_base, grid);
     return hipGetLastError();
 }
 
@@ -96,8 +144,7 @@ extern "C" int hb_nufft_signal(const double* grid, int32_t n_slots, const int32_
         terms > HB_NUFFT_MAX_TERMS || n_tiles < 1 || n_coils < 1 || n_coils > 8 * n_tiles)
         return hipErrorInvalidValue;
     if (n_slots == 0) return hipSuccess;
-    const int total = n_slots * n * n_coils;
-    hipLaunchKernelGGL(hb::nufft_signal_kernel, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, grid,
-                       n_slots, run_of, row_of, n, ng, terms, 16 * n_tiles, n_coils, tw, lw, psi, sig);
+    hipLaunchKernelGGL(hb::nufft_signal_kernel, dim3(n, n_slots), dim3(256), 0, (hipStream_t)stream, grid, run_of,
+                       row_of, n, ng, terms, 16 * n_tiles, n_coils, tw, lw, psi, sig);
     return hipGetLastError();
 }
