@@ -105,13 +105,30 @@ __device__ __forceinline__ float offset(const Spin& p, const hb_segment& q) {
     return p.px * q.g_x + p.py * q.g_y + p.pz * q.g_z + p.db0 + q.b0;
 }
 
-// Phase [cycles] of a free segment in float64 from the segment's float64 gradient moment, reduced to [-1/2, 1/2]:
+// A spin's float64 position (hi + lo), formed once by a kernel that takes many free-segment phases.
+struct Pos64 {
+    double x, y, z;
+};
+
+__device__ __forceinline__ Pos64 pos64(const Spin& p) {
+    return {(double)p.px + (double)p.lx, (double)p.py + (double)p.ly, (double)p.pz + (double)p.lz};
+}
+
+// Phase [cycles] of a free segment in float64 from the segment's float64 gradient moment, unreduced:
 // cycles = k . r + db0 dt + int b0_offset dt.
-__device__ __forceinline__ float free_cycles(const Spin& p, const hb_segment& q) {
-    const double rx = (double)p.px + (double)p.lx, ry = (double)p.py + (double)p.ly, rz = (double)p.pz + (double)p.lz;
-    double cyc = rx * q.m_x + ry * q.m_y + rz * q.m_z + (double)p.db0 * q.dt64 + q.b0_area;
+__device__ __forceinline__ double free_cycles64(const Pos64& r, const Spin& p, const hb_segment& q) {
+    return r.x * q.m_x + r.y * q.m_y + r.z * q.m_z + (double)p.db0 * q.dt64 + q.b0_area;
+}
+
+// The same reduced to [-1/2, 1/2].
+__device__ __forceinline__ float free_cycles(const Pos64& r, const Spin& p, const hb_segment& q) {
+    double cyc = free_cycles64(r, p, q);
     cyc -= rint(cyc);
     return (float)cyc;
+}
+
+__device__ __forceinline__ float free_cycles(const Spin& p, const hb_segment& q) {
+    return free_cycles(pos64(p), p, q);
 }
 
 // Factors kept across runs of identical segments.
@@ -201,10 +218,11 @@ __device__ __forceinline__ void rf_step(const R& q, int f, const Spin& p, float 
     }
 }
 
-__device__ __forceinline__ void free_step(const hb_segment& q, int f, const Spin& p, State& m, Cache& k) {
+__device__ __forceinline__ void free_step(const hb_segment& q, int f, const Spin& p, const Pos64& r, State& m,
+                                          Cache& k) {
     {
         if (!(f & HB_SAME_FREE)) {
-            sincospi(-2.0f * free_cycles(p, q), &k.s, &k.c);
+            sincospi(-2.0f * free_cycles(r, p, q), &k.s, &k.c);
             k.e1 = expm1x(-q.dt * p.r1);
             k.e2 = expm1x(-q.dt * p.r2);
         }
@@ -215,11 +233,20 @@ __device__ __forceinline__ void free_step(const hb_segment& q, int f, const Spin
     }
 }
 
+__device__ __forceinline__ void step(const hb_segment& q, int f, const Spin& p, const Pos64& r, float b1sq, State& m,
+                                     Cache& k) {
+    if (f & HB_RF)
+        rf_step(q, f, p, b1sq, m, k);
+    else
+        free_step(q, f, p, r, m, k);
+}
+
+// The same for kernels that keep no float64 position (it is formed per free segment).
 __device__ __forceinline__ void step(const hb_segment& q, int f, const Spin& p, float b1sq, State& m, Cache& k) {
     if (f & HB_RF)
         rf_step(q, f, p, b1sq, m, k);
     else
-        free_step(q, f, p, m, k);
+        free_step(q, f, p, pos64(p), m, k);
 }
 
 }  // namespace hb

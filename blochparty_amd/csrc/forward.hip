@@ -181,16 +181,14 @@ __device__ __forceinline__ void flush_cols(float* stg, float* wred, int& par, co
 // Samples still staged (nb of them, rows [row, row + nb)) share the first flush with the grid columns.
 template <int NT, int NUT>
 __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu, const hb_segment& q, const Spin& p,
-                                          State& m, float* stg, float* wred, int& par, const Sens<NT>& S, int nt,
+                                          const Pos64& r64, State& m, float* stg, float* wred, int& par, const Sens<NT>& S, int nt,
                                           int lane, int& nb_io, int& row, int rows, float* __restrict__ partial) {
     const int L = q.pad[3], slot = q.pad[1], kind = q.pad[2];
     const int ng = (int)nu.params[0], terms = (int)nu.params[2], shift = (int)nu.params[4];
     const float h = nu.params[6 + HB_NUFFT_MAX_TERMS];       // 2 / width: z per grid cell
     const float bl2 = nu.params[7 + HB_NUFFT_MAX_TERMS];     // beta log2(e)
     const int vals = 16 * nt;
-    // the phase per sample in cycles, float64 (free_cycles' form without the reduction)
-    const double rx = (double)p.px + (double)p.lx, ry = (double)p.py + (double)p.ly, rz = (double)p.pz + (double)p.lz;
-    const double cyc = rx * q.m_x + ry * q.m_y + rz * q.m_z + (double)p.db0 * q.dt64 + q.b0_area;
+    const double cyc = free_cycles64(r64, p, q);
     const double fr = cyc - floor(cyc);
     const double xg = fr * ng;  // float64: at x ~ ng a float32 position is 1e-5 of a cycle off at the last sample
     double ph = fr * shift;
@@ -271,6 +269,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
     float* stg = lds + wave * kStgFloats;
     float* wred = lds + kWaves * kStgFloats;
     const Spin p = load_spin(cs.spins, cs.pos_lo, np, j);
+    const Pos64 r64 = pos64(p);
     const float b1sq = p.b1 * p.b1;
     State m = load_state(cs.state, np, j);
     Sens<NT> S;
@@ -299,7 +298,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
         const int f = q.flags;
         if constexpr (NUT > 0) {
             if (f & HB_NUFFT) {  // a readout run as a type-1 NUFFT: spread onto the grid window, jump the run
-                nufft_run<NT, NUT>(cs, nu, q, p, m, stg, wred, par, S, nt, lane, nb, row, rows, partial);
+                nufft_run<NT, NUT>(cs, nu, q, p, r64, m, stg, wred, par, S, nt, lane, nb, row, rows, partial);
                 s += q.pad[3] - 1;
                 continue;
             }
@@ -318,7 +317,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
         if (f & HB_RUN) {  // L identical free ADC segments: Mxy stepped and sampled per segment, Mz relaxed once
             const int L = q.pad[3];
             if (!(f & HB_SAME_FREE)) {
-                sincospi(-2.0f * free_cycles(p, q), &k.s, &k.c);
+                sincospi(-2.0f * free_cycles(r64, p, q), &k.s, &k.c);
                 k.e1 = expm1x(-q.dt * p.r1);
                 k.e2 = expm1x(-q.dt * p.r2);
             }
@@ -360,9 +359,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
             continue;
         }
         if constexpr (RF)
-            step(q, f, p, b1sq, m, k);
+            step(q, f, p, r64, b1sq, m, k);
         else
-            free_step(q, f, p, m, k);
+            free_step(q, f, p, r64, m, k);
         if (f & HB_MERGED) s += q.pad[3] - 1;
         if (f & HB_ADC) {
             stg[nb * kStgStride + lane] = m.mx * S.x_scale;
