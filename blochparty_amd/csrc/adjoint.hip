@@ -81,7 +81,7 @@ __device__ __forceinline__ void seg_store(const f4& d, float (*segbuf)[8], int l
 
 // Effective flags: cache hints honoured only while the cache holds the factors they refer to.
 __device__ __forceinline__ int live_flags(int f, bool free_ok, bool rf_ok, int same_free, int same_rf) {
-    if (!free_ok) f &= ~same_free;
+    if (!free_ok) f &= ~(same_free | HB_SAME_FREE_DT);
     if (!rf_ok) f &= ~same_rf;
     return f;
 }

@@ -223,8 +223,10 @@ __device__ __forceinline__ void free_step(const hb_segment& q, int f, const Spin
     {
         if (!(f & HB_SAME_FREE)) {
             sincospi(-2.0f * free_cycles(r, p, q), &k.s, &k.c);
-            k.e1 = expm1x(-q.dt * p.r1);
-            k.e2 = expm1x(-q.dt * p.r2);
+            if (!(f & HB_SAME_FREE_DT)) {
+                k.e1 = expm1x(-q.dt * p.r1);
+                k.e2 = expm1x(-q.dt * p.r2);
+            }
         }
         const float nx = k.c * m.mx - k.s * m.my, ny = k.s * m.mx + k.c * m.my;
         m.mx = nx + nx * k.e2;

@@ -181,8 +181,9 @@ __device__ __forceinline__ void flush_cols(float* stg, float* wred, int& par, co
 // Samples still staged (nb of them, rows [row, row + nb)) share the first flush with the grid columns.
 template <int NT, int NUT>
 __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu, const hb_segment& q, const Spin& p,
-                                          const Pos64& r64, State& m, float* stg, float* wred, int& par, const Sens<NT>& S, int nt,
-                                          int lane, int& nb_io, int& row, int rows, float* __restrict__ partial) {
+                                          const Pos64& r64, const float (&em)[NUT], float& e1_run, State& m,
+                                          float* stg, float* wred, int& par, const Sens<NT>& S, int nt, int lane,
+                                          int& nb_io, int& row, int rows, float* __restrict__ partial) {
     const int L = q.pad[3], slot = q.pad[1], kind = q.pad[2];
     const int ng = (int)nu.params[0], terms = (int)nu.params[2], shift = (int)nu.params[4];
     const float h = nu.params[6 + HB_NUFFT_MAX_TERMS];       // 2 / width: z per grid cell
@@ -199,11 +200,7 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
     // the weight (times the staged values' scale) times each decay term e^{-R2 tau_m}
     float wr[NUT], wi[NUT];
 #pragma unroll
-    for (int t = 0; t < NUT; ++t) {
-        const float e = t < terms ? __builtin_amdgcn_exp2f(-(nu.params[6 + t] * 1.4426950408889634f) * p.r2) : 0.0f;
-        wr[t] = e * cr;
-        wi[t] = e * ci;
-    }
+    for (int t = 0; t < NUT; ++t) wr[t] = em[t] * cr, wi[t] = em[t] * ci;
     const int* win = nu.win + ((size_t)kind * gridDim.x + blockIdx.x) * 2;
     const int wb = win[0], W = win[1];
     float* out = nu.part + ((size_t)blockIdx.x * nu.runs_max + slot) * nu.cols * vals;
@@ -252,7 +249,8 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
         m.mx = nx + nx * e2;
         m.my = ny + ny * e2;
     }
-    two_sum(m.mz, m.lz, (m.mz - p.pd) * expm1x(-T * p.r1));
+    if (!(q.flags & HB_SAME_RUN)) e1_run = expm1x(-T * p.r1);
+    two_sum(m.mz, m.lz, (m.mz - p.pd) * e1_run);
 }
 
 // NT: coil tiles (0: any, the sensitivities read from memory per flush); NUT: NUFFT runs with up to NUT decay terms
@@ -291,14 +289,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
         S.out_scale = 1.0f / S.x_scale;
     }
     Cache k = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    float e1_run = 0.0f;  // expm1(-L dt R1) of the last readout run
+    float e1_run = 0.0f;  // expm1(-L dt R1) of the last readout run (per sample or NUFFT)
+    // the NUFFT decay terms e^{-R2 tau_m}: the spin's, the same for every run
+    float em[NUT > 0 ? NUT : 1];
+    if constexpr (NUT > 0) {
+        const int terms = (int)nu.params[2];
+#pragma unroll
+        for (int t = 0; t < NUT; ++t)
+            em[t] = t < terms ? __builtin_amdgcn_exp2f(-(nu.params[6 + t] * 1.4426950408889634f) * p.r2) : 0.0f;
+    }
     int nb = 0, row = 0, par = 0;
     for (int s = s0; s < s1; ++s) {
         const hb_segment q = cs.segs[s];
         const int f = q.flags;
         if constexpr (NUT > 0) {
             if (f & HB_NUFFT) {  // a readout run as a type-1 NUFFT: spread onto the grid window, jump the run
-                nufft_run<NT, NUT>(cs, nu, q, p, r64, m, stg, wred, par, S, nt, lane, nb, row, rows, partial);
+                nufft_run<NT, NUT>(cs, nu, q, p, r64, em, e1_run, m, stg, wred, par, S, nt, lane, nb, row, rows,
+                                   partial);
                 s += q.pad[3] - 1;
                 continue;
             }
@@ -318,8 +325,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
             const int L = q.pad[3];
             if (!(f & HB_SAME_FREE)) {
                 sincospi(-2.0f * free_cycles(r64, p, q), &k.s, &k.c);
-                k.e1 = expm1x(-q.dt * p.r1);
-                k.e2 = expm1x(-q.dt * p.r2);
+                if (!(f & HB_SAME_FREE_DT)) {
+                    k.e1 = expm1x(-q.dt * p.r1);
+                    k.e2 = expm1x(-q.dt * p.r2);
+                }
             }
             if (!(f & HB_SAME_RUN)) e1_run = expm1x(-(float)L * q.dt * p.r1);
             for (int i = 0; i < L; ++i) {

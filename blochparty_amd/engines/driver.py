@@ -30,7 +30,8 @@ from . import hip, pulses
 
 BLOCK = 256
 RF, ADC, SPOIL, SAME_FREE, SAME_RF_DT, NEXT_FREE, NEXT_RF_DT, PULSE, PULSE_SPOIL, RUN, SAME_RUN, TRAIN, MERGED, \
-    SPOILED_FREE, NUFFT, XY_DEAD = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384, 32768)
+    SPOILED_FREE, NUFFT, XY_DEAD, SAME_FREE_DT = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384,
+                                                  32768, 65536)
 NUFFT_RUNS = 64  # NUFFT runs per dispatch at most (slots of the grid partials)
 MIN_RUN = 4  # readout runs shorter than this are stepped segment by segment
 MIN_TRAIN = 4  # likewise RF trains
@@ -110,8 +111,23 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
             same = _close(k[cur, cols], k[prev, cols]).all(1) & (chunk_id[cur] == chunk_id[prev])
             flags[cur[same]] |= bit  # forward order: equal to the previous one of the kind
             flags[prev[same]] |= nxt  # reverse order: equal to the next one of the kind
+    # free records with the previous free record's float32 duration (in the dispatch): their relaxation factors
+    # are reused even where the phase differs
+    idx = np.flatnonzero(~is_rf & stepped & ~zapped)
+    if len(idx) > 1:
+        prev, cur = idx[:-1], idx[1:]
+        same_dt = (rec["f"][cur, 0] == rec["f"][prev, 0]) & (chunk_id[cur] == chunk_id[prev])
+        flags[cur[same_dt]] |= SAME_FREE_DT
     if runs:
         mark_runs(flags, rec, key, is_rf, adc, spoil, chunk_id, stepped)
+        # SAME_RUN: the previous readout run in the dispatch (per-sample or NUFFT: both relax Mz once over the run
+        # and keep the factor) has the same float32 sample dt and length
+        heads = np.flatnonzero(flags & (RUN | NUFFT))
+        flags[heads] &= ~SAME_RUN
+        for prev, cur in zip(heads[:-1], heads[1:]):
+            if (chunk_id[cur] == chunk_id[prev] and rec["pad"][cur, 3] == rec["pad"][prev, 3]
+                    and rec["f"][cur, 0] == rec["f"][prev, 0]):
+                flags[cur] |= SAME_RUN
     mark_trains(flags, rec, is_rf, adc, spoil, chunk_id, stepped)
     rec["flags"] = flags
     w = -2 * math.pi * dt  # the rotation-vector factors, formed in float64

@@ -31,7 +31,8 @@ extern "C" {
 #define HB_PULSE 128      /* first segment of a reused pulse: apply map pad[1] turned by psi, skip pad[2] segs */
 #define HB_PULSE_SPOIL 256 /* the reused pulse's last segment is spoiled                                       */
 #define HB_RUN 512        /* first of pad[3] identical free ADC segments: step Mxy per sample, Mz once        */
-#define HB_SAME_RUN 1024  /* a run with the previous run's dt and length (reuse its Mz factor)              */
+#define HB_SAME_RUN 1024  /* a readout run (HB_RUN or HB_NUFFT) with the previous one's dt and length (reuse its
+                             Mz factor)                                                                         */
 #define HB_TRAIN 2048     /* first of pad[3] consecutive RF segments without ADC or spoil: stepped in one loop */
 #define HB_MERGED 4096    /* first of pad[3] free segments merged into it (summed dt and moments); only the last
                              may be sampled (HB_ADC: the sample is taken after the merged step)                 */
@@ -41,6 +42,8 @@ extern "C" {
                              at the run's end is HB_PULSE_SPOIL                                                 */
 #define HB_XY_DEAD 32768  /* a NUFFT run whose Mxy at its end is never read (the next record is HB_SPOILED_FREE):
                              the run's jump relaxes Mz only and zeroes Mxy                                     */
+#define HB_SAME_FREE_DT 65536 /* free segment with the previous free one's dt here (reuse its relaxation factors,
+                             also where the phase differs)                                                    */
 #define HB_NUFFT_MAX_TERMS 8
 
 /* One segment of the event list, 128 bytes (the reference's segs row + flags + rows, driver.py:357-384). psi is the

@@ -99,6 +99,13 @@ def test_records_and_dispatches_keep_runs_whole():
     # a run whose Mxy is never read (the next record relaxes Mz only) skips the precession of its jump
     dead = [a for a, k in plan.runs if fl[a] & driver.XY_DEAD]
     assert dead and all(fl[a + k] & driver.SPOILED_FREE for a, k in plan.runs if fl[a] & driver.XY_DEAD)
+    # identical runs after the first in a dispatch reuse its Mz factor; free records with the previous free one's
+    # duration reuse its relaxation factors (each TR's prephaser + first sample has the same duration)
+    for s0, s1 in zip(starts, ends):
+        hs = [h for h in heads if s0 <= h < s1]
+        assert [bool(fl[h] & driver.SAME_RUN) for h in hs] == [False] + [True] * (len(hs) - 1)
+    same_dt = np.flatnonzero(fl & driver.SAME_FREE_DT)
+    assert len(same_dt) > 0 and not (fl[same_dt] & (driver.RF | driver.SPOILED_FREE)).any()
     # slots count up within each dispatch
     for s0, s1 in zip(starts, ends):
         sl = [rec["pad"][h, 1] for h in heads if s0 <= h < s1]
