@@ -499,4 +499,4 @@ extern "C" int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t 
 
 extern "C" const char* hb_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 
-extern "C" int hb_abi_version(void) { return 15; }
+extern "C" int hb_abi_version(void) { return 16; }

@@ -6,7 +6,7 @@
 // composed on four states at once (0, e_x, e_y, e_z) in float64: v = m'(0) and the columns of D are
 // m'(e_c) - v - e_c. One lane per spin, one grid row per distinct pulse; all lanes read the same table rows.
 // Output maps[(p * 12 + r) * n_pad + spin]: r = 0..8 the rows of D (D[i][k] at 3 i + k), r = 9..11 v, float32.
-// With a mask, only the spins it marks are composed (the surrogate's fallback and its judge).
+// With a spin list, only the listed spins are composed, one lane each (the surrogate's fallback and its judge).
 // D is formed as a difference so that its float32 rounding is relative to the (small) change, not to m.
 #include "bloch_device.h"
 
@@ -16,9 +16,11 @@ __global__ __launch_bounds__(256) void pulse_maps_kernel(const float* __restrict
                                                          const float* __restrict__ pos_lo, int n_pad,
                                                          const double* __restrict__ table,
                                                          const int* __restrict__ index,
-                                                         const int* __restrict__ mask, float* __restrict__ maps) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (mask && !mask[j]) return;
+                                                         const int* __restrict__ list, int n_lanes,
+                                                         float* __restrict__ maps) {
+    const int lane_id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane_id >= n_lanes) return;
+    const int j = list ? list[lane_id] : lane_id;
     const int p = blockIdx.y;
     const double px = (double)spins[j] + (double)pos_lo[j];
     const double py = (double)spins[n_pad + j] + (double)pos_lo[n_pad + j];
@@ -70,11 +72,13 @@ __global__ __launch_bounds__(256) void pulse_maps_kernel(const float* __restrict
 }  // namespace hb
 
 extern "C" int hb_pulse_maps(const hb_case* cs, const double* table, const int32_t* index, int32_t n_pulses,
-                             const int32_t* mask, float* maps, void* stream) {
+                             const int32_t* spin_list, int32_t n_list, float* maps, void* stream) {
     using namespace hb;
     if (!cs || cs->n_pad <= 0 || cs->n_pad % 256 || n_pulses < 0 || n_pulses > 65535) return hipErrorInvalidValue;
-    if (n_pulses == 0) return hipSuccess;
-    hipLaunchKernelGGL(pulse_maps_kernel, dim3(cs->n_pad / 256, n_pulses), dim3(256), 0, (hipStream_t)stream,
-                       cs->spins, cs->pos_lo, cs->n_pad, table, index, mask, maps);
+    if (spin_list && (n_list < 0 || n_list > cs->n_pad)) return hipErrorInvalidValue;
+    const int n_lanes = spin_list ? n_list : cs->n_pad;
+    if (n_pulses == 0 || n_lanes == 0) return hipSuccess;
+    hipLaunchKernelGGL(pulse_maps_kernel, dim3((n_lanes + 255) / 256, n_pulses), dim3(256), 0, (hipStream_t)stream,
+                       cs->spins, cs->pos_lo, cs->n_pad, table, index, spin_list, n_lanes, maps);
     return hipGetLastError();
 }

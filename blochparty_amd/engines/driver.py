@@ -533,7 +533,7 @@ def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> l
         surrogate.compose_maps(case, hip.load())
     elif case.n_pulses:
         lib.pulse_maps(hip.ctypes.byref(st), case.pulse_table.data_ptr(), case.pulse_index.data_ptr(),
-                       case.n_pulses, None, case.maps.data_ptr())
+                       case.n_pulses, None, 0, case.maps.data_ptr())
     nu = case.nu
     nu_arg = None
     if nu is not None:

@@ -251,19 +251,19 @@ def compose_maps(case, lib) -> None:
     dev = case.spins.device
     if len(nets) < case.n_pulses:  # pulses the surrogate does not cover
         _check(lib.hb_pulse_maps(ctypes.byref(st), case.pulse_table.data_ptr(), case.pulse_index.data_ptr(),
-                                 case.n_pulses, None, case.maps.data_ptr(), stream()), lib, "hb_pulse_maps")
+                                 case.n_pulses, None, 0, case.maps.data_ptr(), stream()), lib, "hb_pulse_maps")
     outside = torch.zeros(case.n_pad, dtype=torch.int32, device=dev)
-    judge = case.extra.get("judge_mask")
-    if judge is None:
-        judge = torch.zeros(case.n_pad, dtype=torch.int32, device=dev)
-        judge[: case.n : JUDGE_STRIDE] = 1
-        case.extra["judge_mask"] = judge
+    judge = case.extra.get("judge_list")
+    if judge is None:  # every JUDGE_STRIDE-th spin, composed exactly on every run
+        judge = torch.arange(0, case.n, JUDGE_STRIDE, dtype=torch.int32, device=dev)
+        case.extra["judge_list"] = judge
         case.extra["judge_maps"] = torch.empty_like(case.maps)
     jm = case.extra["judge_maps"]
-    # the judge: exact maps of the sampled spins, every pulse
+    # the judge: exact maps of the sampled spins, every pulse (one lane per judged spin)
     _check(lib.hb_pulse_maps(ctypes.byref(st), case.pulse_table.data_ptr(), case.pulse_index.data_ptr(),
-                             case.n_pulses, judge.data_ptr(), jm.data_ptr(), stream()), lib, "hb_pulse_maps")
-    sel = judge.bool()
+                             case.n_pulses, judge.data_ptr(), len(judge), jm.data_ptr(), stream()), lib,
+           "hb_pulse_maps")
+    sel = judge.long()
     report = []
     for pid, shape, net in nets:
         blocks = torch.from_numpy(shape.blocks).to(dev)
@@ -273,17 +273,19 @@ def compose_maps(case, lib) -> None:
         mine = case.maps.view(case.n_pulses, 12, case.n_pad)[pid][:, sel]
         exact = jm.view(case.n_pulses, 12, case.n_pad)[pid][:, sel]
         err = (mine - exact).abs().max()
-        # the fallback: spins outside the box, composed exactly
-        _check(lib.hb_pulse_maps(ctypes.byref(st), case.pulse_table.data_ptr(), case.pulse_index.data_ptr(),
-                                 case.n_pulses, outside.data_ptr(), case.maps.data_ptr(), stream()), lib,
-               "hb_pulse_maps")
-        report.append((pid, err, outside.sum(), net))
+        # the fallback: spins outside the box, composed exactly (listed on the device; nothing to do for none)
+        out_list = torch.nonzero(outside).flatten().to(torch.int32)
+        if len(out_list):
+            _check(lib.hb_pulse_maps(ctypes.byref(st), case.pulse_table.data_ptr(), case.pulse_index.data_ptr(),
+                                     case.n_pulses, out_list.data_ptr(), len(out_list), case.maps.data_ptr(),
+                                     stream()), lib, "hb_pulse_maps")
+        report.append((pid, err, len(out_list), net))
         del blocks
     case.extra["surrogate_report"] = report
     fallback = [r for r in report if float(r[1]) > JUDGE_TOL]
     if fallback:  # the judge rejects a network: exact maps for every spin of every pulse
         _check(lib.hb_pulse_maps(ctypes.byref(st), case.pulse_table.data_ptr(), case.pulse_index.data_ptr(),
-                                 case.n_pulses, None, case.maps.data_ptr(), stream()), lib, "hb_pulse_maps")
+                                 case.n_pulses, None, 0, case.maps.data_ptr(), stream()), lib, "hb_pulse_maps")
         case.extra["surrogate_rejected"] = True
 
 
@@ -292,7 +294,7 @@ def report(case) -> dict:
     the box, the training time and held-out loss; and whether the exact maps replaced the network."""
     rows = []
     for pid, err, n_out, net in case.extra.get("surrogate_report", []):
-        rows.append({"pulse": pid, "judge_max_map_err": float(err), "judge_spins": int(case.extra["judge_mask"].sum()),
+        rows.append({"pulse": pid, "judge_max_map_err": float(err), "judge_spins": len(case.extra["judge_list"]),
                      "outside_box": int(n_out), "train_s": net.train_s, "held_out_loss": net.final_loss})
     return {"pulses": rows, "covered": len(rows), "reused_pulses": case.n_pulses,
             "rejected": bool(case.extra.get("surrogate_rejected", False)), "tolerance": JUDGE_TOL}

@@ -137,10 +137,11 @@ int hb_nufft_signal(const double* grid, int32_t n_slots, const int32_t* run_of, 
 /* Per-spin affine maps of reused pulses (replaces pulse_maps, cuda/pulse.cu.in:3-32, launched at driver.py:600-613):
  * pulse p's K substeps composed in float64 into m -> m + D m + v; maps[(p * 12 + r) * n_pad + spin] holds D's rows
  * (r = 0..8, D[i][k] at 3 i + k) and v (r = 9..11). table: (rows, 8) float64 dt, rf_x, rf_y, g_x, g_y, g_z,
- * b0_offset, 0 of every pulse's first occurrence; index: (n_pulses, 2) its (first row, length). mask: (n_pad,) or
- * NULL; when given, only spins with mask[spin] != 0 are composed (the others' maps are left as they are). */
+ * b0_offset, 0 of every pulse's first occurrence; index: (n_pulses, 2) its (first row, length). spin_list: NULL
+ * (every spin), or the n_list spins (< n_pad) to compose, one lane each (the others' maps are left as they are:
+ * the surrogate's judge and fallback compose a few spins without paying for the rest). */
 int hb_pulse_maps(const hb_case* cs, const double* table, const int32_t* index, int32_t n_pulses,
-                  const int32_t* mask, float* maps, void* stream);
+                  const int32_t* spin_list, int32_t n_list, float* maps, void* stream);
 
 /* ---- The learned RF surrogate (no counterpart in the reference: it stands where the reference composes a reused
  * pulse per spin, cuda/pulse.cu.in:3-32). A pulse is cut into HB_SUR_BLOCKS blocks; a network 3 -> 64 -> 64 -> 4 M
@@ -197,8 +198,8 @@ int hb_mlp_train_steps(const hb_mlp* net, const float* x, const float* y, int32_
 int hb_mlp_eval(const hb_mlp* net, const float* x, int32_t n, float* y, void* stream);
 
 /* Maps of pulse sp->pid for every spin from the network (the maps format of hb_pulse_maps), and outside[spin] = 1
- * for spins (< n) whose inputs lie outside the trained box (their maps are to be composed exactly with
- * hb_pulse_maps' mask), else 0. */
+ * for spins (< n) whose inputs lie outside the trained box (their maps are to be composed exactly, hb_pulse_maps
+ * with their list), else 0. */
 int hb_surrogate_maps(const hb_case* cs, const hb_mlp* net, const hb_sur_pulse* sp, float* maps, int32_t* outside,
                       void* stream);
 

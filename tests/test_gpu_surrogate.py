@@ -198,7 +198,7 @@ def test_ablation_training_and_cost(gpu, trained, slab):
         return a.elapsed_time(b) / 3
 
     t_exact = timed(lambda: lib.hb_pulse_maps(ctypes.byref(st), big.pulse_table.data_ptr(), big.pulse_index.data_ptr(),
-                                              big.n_pulses, None, big.maps.data_ptr(), sur.stream()))
+                                              big.n_pulses, None, 0, big.maps.data_ptr(), sur.stream()))
     t_net = timed(lambda: lib.hb_surrogate_maps(ctypes.byref(st), ctypes.byref(net.struct()), ctypes.byref(sp),
                                                 big.maps.data_ptr(), outside.data_ptr(), sur.stream()))
     print(f"maps of {big.n} spins: exact composition {t_exact:.3f} ms, network {t_net:.3f} ms "
