@@ -86,6 +86,45 @@ __device__ __forceinline__ int live_flags(int f, bool free_ok, bool rf_ok, int s
     return f;
 }
 
+// Reverse through a reused pulse's occurrence (head record q): m_after = Rz(psi) ((I + D) Rz(-psi) m + v). With
+// (lx, ly, lz) = dL/dm_after in, dL/dm out: lambda <- Rz(psi) (I + D)^T Rz(-psi) lambda. The seeds of the pulse's
+// parameter gradients, in its frame (t = Rz(-psi) lambda_after, u = Rz(-psi) m_before): t u^T on D's entries and t
+// on v's, added in float64. With SEG: sg[G_RX] = t^T (Dd u + vd) (the map's derivative along the pulse's amplitude,
+// maps_d), sg[G_RY] = dL/dpsi = (lambda_after x m_after)_z - (lambda_before x m_before)_z (the frame's turn).
+template <bool SEG>
+__device__ __forceinline__ void pulse_reverse(const hb_segment& q, const hb_case& cs, const hb_adjoint& ad, int j,
+                                              const Stash& mb, const StashXY& ma, float& lx, float& ly, float& lz,
+                                              float (&sg)[N_SEG_GRADS]) {
+    const int np = cs.n_pad;
+    const float c = q.w_rf_x, sn = q.w_rf_y;
+    const size_t base = (size_t)q.pad[1] * 12 * np + j;
+    const float* U = cs.maps + base;
+    const float tx = c * lx + sn * ly, ty = c * ly - sn * lx, tz = lz;
+    const float ux = c * mb.x + sn * mb.y, uy = c * mb.y - sn * mb.x, uz = mb.z;
+    double* sd = ad.seeds + base;
+    const float t3[3] = {tx, ty, tz}, u3[3] = {ux, uy, uz};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) sd[(3 * i + k) * (size_t)np] += (double)t3[i] * (double)u3[k];
+        sd[(9 + i) * (size_t)np] += (double)t3[i];
+    }
+    const float bx = tx + U[0] * tx + U[3 * np] * ty + U[6 * np] * tz;
+    const float by = ty + U[np] * tx + U[4 * np] * ty + U[7 * np] * tz;
+    const float bz = tz + U[2 * np] * tx + U[5 * np] * ty + U[8 * np] * tz;
+    const float nlx = c * bx - sn * by, nly = sn * bx + c * by;
+    if (SEG) {
+#pragma unroll
+        for (int g = 0; g < N_SEG_GRADS; ++g) sg[g] = 0.0f;
+        const float* M = ad.maps_d + base;
+        sg[G_RX] = tx * (M[0] * ux + M[np] * uy + M[2 * np] * uz + M[9 * np]) +
+                   ty * (M[3 * np] * ux + M[4 * np] * uy + M[5 * np] * uz + M[10 * np]) +
+                   tz * (M[6 * np] * ux + M[7 * np] * uy + M[8 * np] * uz + M[11 * np]);
+        sg[G_RY] = (ly * ma.x - lx * ma.y) - (nly * mb.x - nlx * mb.y);
+    }
+    lx = nlx, ly = nly, lz = bz;
+}
+
 template <int NC, bool SEG>
 __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint ad, int s0, int s1) {
     __shared__ float segbuf[kWavesA][8][8];
@@ -106,6 +145,18 @@ __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint
                 r[j] = m.mx, r[np + j] = m.my, r[2 * np + j] = m.mz + m.lz;
             }
             const hb_segment q = cs.segs[s];
+            if (q.flags & HB_PULSE_BODY) continue;  // inside an occurrence begun in an earlier interval
+            if (q.flags & HB_PULSE) {  // a reused pulse: its map, then the state stands over the occurrence
+                apply_map(cs.maps + (size_t)q.pad[1] * 12 * np + j, np, q.w_rf_x, q.w_rf_y, m);
+                if (q.flags & HB_PULSE_SPOIL) m.mx = m.my = 0.0f;
+                const int e = min(s + q.pad[2], s1);
+                for (int t = s0 + (((s + 1 - s0) + 7) & ~7); t < e; t += 8) {
+                    float* r = ad.r8 + (size_t)((t - s0) >> 3) * 3 * np;
+                    r[j] = m.mx, r[np + j] = m.my, r[2 * np + j] = m.mz + m.lz;
+                }
+                s = e - 1;
+                continue;
+            }
             if (q.flags & HB_TRAIN) {  // an RF train: its own loop, the next record in flight (forward_kernel's)
                 const int L = min(q.pad[3], s1 - s);
                 RfRec cur = load_rf(&cs.segs[s]);
@@ -147,6 +198,12 @@ __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint
     for (int b = n8 - 1; b >= 0; --b) {
         const int sb = s0 + 8 * b;
         const int nseg = min(8, s1 - sb);
+        // a block inside one reused pulse's occurrence is the identity: only its (zero) segment gradients
+        const hb_segment& qf = cs.segs[sb];
+        const hb_segment& ql = cs.segs[sb + nseg - 1];
+        if ((qf.flags & ql.flags & HB_PULSE_BODY) && qf.pad[3] == ql.pad[3]) {
+            if (SEG && tid < 8 * nseg) ad.gseg[((size_t)blockIdx.x * ad.k_max + (sb - s0)) * 8 + tid] = 0.0f;
+        } else {
         // replay the block with its states in registers: before[i] before segment sb + i, after[i] after its step
         Stash before[8];
         StashXY after[8];
@@ -161,6 +218,16 @@ __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint
                 if (i < nseg) {
                     before[i] = {m.mx, m.my, m.mz + m.lz};
                     const hb_segment q = cs.segs[sb + i];
+                    if (q.flags & HB_PULSE_BODY) {
+                        after[i] = {m.mx, m.my};
+                        continue;
+                    }
+                    if (q.flags & HB_PULSE) {
+                        apply_map(cs.maps + (size_t)q.pad[1] * 12 * np + j, np, q.w_rf_x, q.w_rf_y, m);
+                        after[i] = {m.mx, m.my};
+                        if (q.flags & HB_PULSE_SPOIL) m.mx = m.my = 0.0f;
+                        continue;
+                    }
                     const int f = live_flags(q.flags, free_ok, rf_ok, HB_SAME_FREE, HB_SAME_RF_DT);
                     step(q, f, p, b1sq, m, k);
                     if (f & HB_RF) rf_ok = true; else free_ok = true;
@@ -176,6 +243,13 @@ __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint
             if (i < nseg) {
                 const int s = sb + i;
                 const hb_segment q = cs.segs[s];
+                if (q.flags & HB_PULSE_BODY) continue;  // the identity, no gradients
+                if (q.flags & HB_PULSE) {
+                    if (q.flags & HB_PULSE_SPOIL) lx = ly = 0.0f;
+                    pulse_reverse<SEG>(q, cs, ad, j, before[i], after[i], lx, ly, lz, sg);
+                    if (SEG) seg_add(segd, sg, i, lane);
+                    continue;
+                }
                 const int f = live_flags(q.flags, rfree_ok, rrf_ok, HB_NEXT_FREE, HB_NEXT_RF_DT);
                 if (f & HB_SPOIL) lx = ly = 0.0f;
                 if (NC > 0 && (f & HB_ADC)) {
@@ -214,6 +288,7 @@ __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint
                 ad.gseg[((size_t)blockIdx.x * ad.k_max + (sb - s0 + i)) * 8 + g] = v;
             }
             __syncthreads();
+        }
         }
         if ((sb & 63) == 0) {  // every 64 segments: into the float64 accumulators
 #pragma unroll

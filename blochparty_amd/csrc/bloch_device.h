@@ -172,6 +172,18 @@ __device__ __forceinline__ void store_state(float* __restrict__ st, int n_pad, i
     st[3 * n_pad + j] = 0.0f, st[4 * n_pad + j] = 0.0f, st[5 * n_pad + j] = m.lz;
 }
 
+// A reused pulse's occurrence with RF phase psi (c, sn its cosine and sine): m += Rz(psi) (D Rz(-psi) m + v), U the
+// spin's map (rows n_pad apart: D[i][k] at 3 i + k, then v), read per occurrence so that no registers hold it.
+__device__ __forceinline__ void apply_map(const float* __restrict__ U, int np, float c, float sn, State& m) {
+    const float ux = c * m.mx + sn * m.my, uy = c * m.my - sn * m.mx, uz = m.mz;
+    const float dux = U[0] * ux + U[np] * uy + U[2 * np] * uz + U[9 * np];
+    const float duy = U[3 * np] * ux + U[4 * np] * uy + U[5 * np] * uz + U[10 * np];
+    const float duz = U[6 * np] * ux + U[7 * np] * uy + U[8 * np] * uz + U[11 * np];
+    m.mx += c * dux - sn * duy;
+    m.my += sn * dux + c * duy;
+    two_sum(m.mz, m.lz, duz);
+}
+
 // One segment (not including the spoil).
 //
 // RF: the three Strang increments (relax dt/2, rotate, relax dt/2) are formed from the running state in float32 and

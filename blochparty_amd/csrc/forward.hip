@@ -203,7 +203,7 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
     for (int t = 0; t < NUT; ++t) wr[t] = em[t] * cr, wi[t] = em[t] * ci;
     const int* win = nu.win + ((size_t)kind * gridDim.x + blockIdx.x) * 2;
     const int wb = win[0], W = win[1];
-    float* out = nu.part + ((size_t)blockIdx.x * nu.runs_max + slot) * nu.cols * vals;
+    float* out = nu.part + ((size_t)slot * gridDim.x + blockIdx.x) * nu.cols * vals;
     int nb = nb_io, col = 0, pending = nb_io;  // the staged samples come first in the first flush
     nb_io = 0;
     // z = (cell - x) 2 / width: at the window's first cell from the float64 position (x ~ ng would leave a float32
@@ -299,7 +299,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
             em[t] = t < terms ? __builtin_amdgcn_exp2f(-(nu.params[6 + t] * 1.4426950408889634f) * p.r2) : 0.0f;
     }
     int nb = 0, row = 0, par = 0;
-    for (int s = s0; s < s1; ++s) {
+    // a dispatch that starts inside a reused pulse's occurrence (checkpoint intervals do not avoid them): the
+    // occurrence's map was applied at its first segment, in the dispatch before
+    int s_start = s0;
+    if (cs.segs[s0].flags & HB_PULSE_BODY) {
+        const int head = cs.segs[s0].pad[3];
+        s_start = head + cs.segs[head].pad[2];
+    }
+    for (int s = s_start; s < s1; ++s) {
         const hb_segment q = cs.segs[s];
         const int f = q.flags;
         if constexpr (NUT > 0) {
@@ -348,15 +355,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
             continue;
         }
         if (f & HB_PULSE) {  // a reused pulse: m += Rz(psi) (D Rz(-psi) m + v), then skip its segments
-            const float* U = cs.maps + (size_t)q.pad[1] * 12 * np + j;  // loaded per occurrence: no registers held
-            const float cph = q.w_rf_x, sph = q.w_rf_y;  // cos, sin of the occurrence's RF phase (host float64)
-            const float ux = cph * m.mx + sph * m.my, uy = cph * m.my - sph * m.mx, uz = m.mz;
-            const float dux = U[0] * ux + U[np] * uy + U[2 * np] * uz + U[9 * np];
-            const float duy = U[3 * np] * ux + U[4 * np] * uy + U[5 * np] * uz + U[10 * np];
-            const float duz = U[6 * np] * ux + U[7 * np] * uy + U[8 * np] * uz + U[11 * np];
-            m.mx += cph * dux - sph * duy;
-            m.my += sph * dux + cph * duy;
-            two_sum(m.mz, m.lz, duz);
+            // cos, sin of the occurrence's RF phase (host float64)
+            apply_map(cs.maps + (size_t)q.pad[1] * 12 * np + j, np, q.w_rf_x, q.w_rf_y, m);
             if (f & HB_PULSE_SPOIL) m.mx = m.my = 0.0f;
             s += q.pad[2] - 1;
             continue;
@@ -499,4 +499,4 @@ extern "C" int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t 
 
 extern "C" const char* hb_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 
-extern "C" int hb_abi_version(void) { return 16; }
+extern "C" int hb_abi_version(void) { return 18; }

@@ -22,7 +22,9 @@ constexpr int kBatch = 8;  // list entries per step of a lane's gather
 // terms x vals lanes, group k the entries k, k + G, ... (each entry's terms x vals floats are contiguous in part:
 // one coalesced read), each lane summing in float64; then lane q adds the G groups' sums in order. Fixed order
 // throughout: the grid is the same bit for bit on every run.
-__global__ __launch_bounds__(kGridBlock) void nufft_grid_kernel(const float* __restrict__ part, int runs_max, int cols,
+// Workgroups go round robin over the 8 XCDs, so workgroup i works on item (i mod 8) * ~T/8 + i / 8 of the T (slot,
+// cell) items: each XCD takes one contiguous stretch of cells, whose entries share cache lines of part in its L2.
+__global__ __launch_bounds__(kGridBlock) void nufft_grid_kernel(const float* __restrict__ part, int n_blocks, int cols,
                                                                 int vals, const int* __restrict__ run_of,
                                                                 const int* __restrict__ kind_of, int ng, int terms,
                                                                 const int* __restrict__ csr_off,
@@ -30,7 +32,9 @@ __global__ __launch_bounds__(kGridBlock) void nufft_grid_kernel(const float* __r
                                                                 const int* __restrict__ csr_base,
                                                                 double* __restrict__ grid) {
     __shared__ double sums[kGridBlock];
-    const int u = blockIdx.x, r = blockIdx.y, tid = threadIdx.x;
+    const int T = gridDim.x, xcd = blockIdx.x % 8, per = T / 8, extra = T % 8;
+    const int w = xcd * per + min(xcd, extra) + blockIdx.x / 8;
+    const int u = w % ng, r = w / ng, tid = threadIdx.x;
     const int vt = terms * vals;
     const int G = vt >= kGridBlock ? 1 : kGridBlock / vt;  // entry groups
     const int kind = kind_of[run_of[r]];
@@ -54,7 +58,7 @@ __global__ __launch_bounds__(kGridBlock) void nufft_grid_kernel(const float* __r
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j)
                     x[j] = be[j].x >= 0
-                               ? part[(((size_t)be[j].x * runs_max + r) * cols + be[j].y * terms) * vals + q]
+                               ? part[(((size_t)r * n_blocks + be[j].x) * cols + be[j].y * terms) * vals + q]
                                : 0.0f;
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) acc += (double)x[j];
@@ -124,16 +128,17 @@ __global__ __launch_bounds__(256) void nufft_signal_kernel(const double* __restr
 
 }  // namespace hb
 
-extern "C" int hb_nufft_grid(const float* part, int32_t runs_max, int32_t cols, int32_t n_tiles, int32_t n_slots,
-                             const int32_t* run_of, const int32_t* kind_of, int32_t ng, int32_t terms,
-                             const int32_t* csr_off, const int32_t* csr, const int32_t* csr_base, double* grid,
-                             void* stream) {
-    if (!part || !run_of || !kind_of || !csr_off || !csr || !csr_base || !grid || runs_max < 1 || cols < 1 ||
-        n_tiles < 1 || n_slots < 0 || n_slots > runs_max || ng < 2 || terms < 1 || terms > HB_NUFFT_MAX_TERMS)
+extern "C" int hb_nufft_grid(const float* part, int32_t n_blocks, int32_t runs_max, int32_t cols, int32_t n_tiles,
+                             int32_t n_slots, const int32_t* run_of, const int32_t* kind_of, int32_t ng,
+                             int32_t terms, const int32_t* csr_off, const int32_t* csr, const int32_t* csr_base,
+                             double* grid, void* stream) {
+    if (!part || !run_of || !kind_of || !csr_off || !csr || !csr_base || !grid || n_blocks < 1 || runs_max < 1 ||
+        cols < 1 || n_tiles < 1 || n_slots < 0 || n_slots > runs_max || ng < 2 || terms < 1 ||
+        terms > HB_NUFFT_MAX_TERMS || (int64_t)ng * n_slots > INT32_MAX)
         return hipErrorInvalidValue;
     if (n_slots == 0) return hipSuccess;
-    hipLaunchKernelGGL(hb::nufft_grid_kernel, dim3(ng, n_slots), dim3(hb::kGridBlock), 0, (hipStream_t)stream, part,
-                       runs_max, cols, 16 * n_tiles, run_of, kind_of, ng, terms, csr_off, csr, csr_base, grid);
+    hipLaunchKernelGGL(hb::nufft_grid_kernel, dim3(ng * n_slots), dim3(hb::kGridBlock), 0, (hipStream_t)stream, part,
+                       n_blocks, cols, 16 * n_tiles, run_of, kind_of, ng, terms, csr_off, csr, csr_base, grid);
     return hipGetLastError();
 }
 

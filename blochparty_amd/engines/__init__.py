@@ -115,13 +115,11 @@ def simulate(
     grad = torch.is_grad_enabled() and requires_grad(phantom, seq, coils, m_init)
     check(name, features(phantom, seq, m_init, scanner, maps), grad=grad)
     if grad:
-        if pulses != "step":
-            raise ValueError(f"pulses={pulses!r} is forward only")
         if nufft is not None:
             raise ValueError("readouts='nufft' is forward only")
         from .adjoint import simulate_grad
 
-        return simulate_grad(phantom, seq, coils, m_init, checkpoint_every)
+        return simulate_grad(phantom, seq, coils, m_init, checkpoint_every, pulses)
     from .driver import forward
 
     with torch.no_grad():

@@ -42,6 +42,7 @@ class Opts(NamedTuple):
     adc: list
     spoil: list
     every: int
+    pulses: str = "step"  # "reuse" / "surrogate": reused pulses as maps in the forward and the reverse sweep
 
 
 class GPUSimulate(torch.autograd.Function):
@@ -52,12 +53,14 @@ class GPUSimulate(torch.autograd.Function):
         ph = Phantom(d(pos), d(pd), d(t1), d(t2), d(db0), b1=d(b1))
         sq = Sequence(d(dt), d(rf), d(grad), d(adc_phase), opts.is_rf, opts.adc, opts.spoil, {}, d(b0_offset))
         coils = torch.complex(coils_ri[..., 0].detach(), coils_ri[..., 1].detach())
-        case = driver.prepare_case(ph, sq, coils, d(m0), chunk=opts.every, aligned=True)
+        case = driver.prepare_case(ph, sq, coils, d(m0), chunk=opts.every, aligned=True,
+                                   reuse=opts.pulses == "reuse", surrogate=opts.pulses == "surrogate")
         ckpts = driver.run_forward(case, keep=True)
         hip.HIP.synchronize()
         res = driver.read_result(case)
         case.partial = None  # the forward's work buffer is not needed any more
         ctx.opts, ctx.case, ctx.ckpts = opts, case, ckpts
+        ctx.rf = rf.detach().double().cpu().numpy() if case.pulses else None
         ctx.coils = coils.cpu()
         ctx.like = [None if t is None else (t.dtype, t.device)
                     for t in (pos, pd, t1, t2, db0, coils_ri, dt, rf, grad, adc_phase, m0, b1, b0_offset)]
@@ -117,6 +120,8 @@ class GPUSimulate(torch.autograd.Function):
             out[5] = torch.stack([G[0::2].T, G[1::2].T], -1)
         if need_seg:
             sgr = seg.cpu().view(S, 8)
+            if case.pulses:
+                project_pulse_rf(sgr, ctx.rf, case.pulses)
             out[6], out[7], out[8], out[12] = sgr[:, 0], sgr[:, 1:3], sgr[:, 3:6], sgr[:, 6]
         if need[9]:
             g = torch.zeros(S, dtype=torch.float64)
@@ -141,6 +146,15 @@ def run_backward(case: driver.DeviceCase, ckpts: list, every: int, h: torch.Tens
     dev = case.spins.device
     n_pad, S = case.n_pad, case.n_segments
     n_blocks = n_pad // driver.BLOCK
+    P = case.n_pulses
+    cs = case.struct()
+    seeds = torch.zeros(P * 12 * n_pad, dtype=torch.float64, device=dev) if P else None
+    maps_d = None
+    if P and need_seg:  # the maps' derivative along each pulse's amplitude, for its occurrences' rf gradients
+        maps_d = torch.empty(P * 12 * n_pad, dtype=torch.float32, device=dev)
+        steps = map_fd_steps(case.pulse_table.cpu().numpy(), case.pulse_index.cpu().numpy())
+        lib.pulse_map_dlnb1(ctypes.byref(cs), case.pulse_table.data_ptr(), case.pulse_index.data_ptr(), P,
+                            float(steps[:, 8].min()), maps_d.data_ptr())
     gs = torch.zeros(8, n_pad, dtype=torch.float64, device=dev)
     gc = torch.zeros(2 * max(nc, 1), n_pad, dtype=torch.float64, device=dev)
     gseg = torch.zeros(n_blocks * every * 8 if need_seg else 1, dtype=torch.float32, device=dev)
@@ -148,15 +162,63 @@ def run_backward(case: driver.DeviceCase, ckpts: list, every: int, h: torch.Tens
     r8 = torch.empty((every + 7) // 8 * 3 * n_pad, dtype=torch.float32, device=dev)
     red = torch.empty(max(1, -(-n_blocks // SEG_CHUNK) * every * 8 if need_seg else 1), dtype=torch.float64,
                       device=dev)
-    cs = case.struct()
+    ptr = lambda t: None if t is None else t.data_ptr()
     for c in reversed(range(len(ckpts))):
         s0, s1 = case.starts[c], case.ends[c]
         ad = hip.HbAdjoint(ckpts[c].data_ptr(), r8.data_ptr(), lam.data_ptr(), gs.data_ptr(), gc.data_ptr(),
-                           gseg.data_ptr(), h.data_ptr(), sens.data_ptr(), nc, int(need_seg), int(need_coil), every)
+                           gseg.data_ptr(), h.data_ptr(), sens.data_ptr(), ptr(maps_d), ptr(seeds), nc,
+                           int(need_seg), int(need_coil), every)
         lib.adjoint_interval(ctypes.byref(cs), ctypes.byref(ad), s0, s1)
         if need_seg:
             lib.seg_reduce(gseg.data_ptr(), n_blocks, every, s0, s1 - s0, red.data_ptr(), seg.data_ptr())
+    if P:  # the spin parameters' gradients through the maps, from the seeds of the sweep
+        steps = map_fd_steps(case.pulse_table.cpu().numpy(), case.pulse_index.cpu().numpy())
+        st = torch.from_numpy(np.ascontiguousarray(steps[:, :8])).to(dev)
+        lib.pulse_map_grads(ctypes.byref(cs), case.pulse_table.data_ptr(), case.pulse_index.data_ptr(), P,
+                            st.data_ptr(), seeds.data_ptr(), gs.data_ptr())
     return gs, gc, seg, lam
+
+
+FD_TURN = 1e-4  # rad: how far a central-difference step turns a reused pulse's rotation
+
+
+def map_fd_steps(table: np.ndarray, index: np.ndarray) -> np.ndarray:
+    """(P, 9) central-difference steps per reused pulse: for px, py, pz, db0, pd, r1, r2, b1 (hb_pulse_map_grads'
+    order; 0 where the map does not depend on the parameter), then the relative step along the amplitude. Each turns
+    the pulse's rotation (or moves its relaxation exponent) by about FD_TURN: truncation ~FD_TURN^2 / 6 relative,
+    rounding of the float64 composition ~1e-16 / FD_TURN."""
+    steps = np.zeros((len(index), 9))
+    for p, (off, k) in enumerate(index):
+        rows = table[off : off + k]
+        T = max(float(rows[:, 0].sum()), 1e-300)
+        angle = 2 * math.pi * float((rows[:, 0] * np.hypot(rows[:, 1], rows[:, 2])).sum())
+        gmax = np.abs(rows[:, 3:6]).max(0)
+        for a in range(3):
+            steps[p, a] = FD_TURN / (2 * math.pi * T * gmax[a]) if gmax[a] > 0 else 0.0
+        steps[p, 3] = FD_TURN / (2 * math.pi * T)
+        steps[p, 4] = 0.5  # the map is affine in pd: any step is exact
+        steps[p, 5] = steps[p, 6] = FD_TURN / T
+        rel = min(FD_TURN / max(angle, FD_TURN), 1e-3)
+        steps[p, 7] = rel  # b1 ~ 1
+        steps[p, 8] = rel
+    return steps
+
+
+def project_pulse_rf(sgr: torch.Tensor, rf: np.ndarray, pulses) -> None:
+    """A reused pulse's occurrence has, from the sweep, dL/d(ln amplitude) c and dL/d(RF phase) d in its first
+    segment's rf slots. Its segments' rf gradients become the least-norm ones with those two components:
+    (c rf_k + d i rf_k) / sum |rf|^2, exact for a loss through each occurrence's complex amplitude (flip angle, RF
+    phase); the pulse's shape, dt, gradients and b0_offset get none (simulate refuses those gradients)."""
+    g = sgr.numpy()
+    for _, k, occ in pulses:
+        for a, _ in occ:
+            c, d = float(g[a, 1]), float(g[a, 2])
+            z = rf[a : a + k]
+            nrm = float((z * z).sum())
+            g[a : a + k] = 0.0
+            if nrm > 0:
+                g[a : a + k, 1] = (c * z[:, 0] - d * z[:, 1]) / nrm
+                g[a : a + k, 2] = (c * z[:, 1] + d * z[:, 0]) / nrm
 
 
 def adjoint_sens(coils: torch.Tensor, n_pad: int, nc: int) -> torch.Tensor:
@@ -169,8 +231,10 @@ def adjoint_sens(coils: torch.Tensor, n_pad: int, nc: int) -> torch.Tensor:
     return sens
 
 
-def simulate_grad(phantom: Phantom, seq: Sequence, coils, m_init, checkpoint_every) -> SimResult:
-    """simulate(engine="gpu") with gradients: the inputs as tensors of ``GPUSimulate`` (adjoint.py:264-323)."""
+def simulate_grad(phantom: Phantom, seq: Sequence, coils, m_init, checkpoint_every, pulses: str = "step") -> SimResult:
+    """simulate(engine="gpu") with gradients: the inputs as tensors of ``GPUSimulate`` (adjoint.py:264-323).
+    ``pulses``: "step", or "reuse" / "surrogate" (reused pulses as maps in both sweeps; the reference's reused pulses
+    are forward only, driver.py:221-225)."""
     n, pd = len(phantom), phantom.pd
     if coils is None:
         coils = torch.ones(n, 1, dtype=torch.complex64 if pd.dtype == torch.float32 else torch.complex128)
@@ -181,7 +245,13 @@ def simulate_grad(phantom: Phantom, seq: Sequence, coils, m_init, checkpoint_eve
     every = checkpoint_every or default_checkpoint_every(len(seq))
     if every % 64:
         raise ValueError(f"checkpoint_every must be a multiple of 64, got {every}")
-    opts = Opts(list(seq.is_rf), list(seq.adc), list(seq.spoil), every)
+    if pulses != "step":
+        fixed = [n for n, t in (("dt", seq.dt), ("grad", seq.grad), ("b0_offset", seq.b0_offset))
+                 if t is not None and t.requires_grad]
+        if fixed:
+            raise ValueError(f"pulses={pulses!r} differentiates spin parameters, coils and each reused pulse's "
+                             f"amplitude and phase, not {', '.join(fixed)}: use pulses='step'")
+    opts = Opts(list(seq.is_rf), list(seq.adc), list(seq.spoil), every, pulses)
     sig, m = GPUSimulate.apply(opts, phantom.pos, pd, phantom.t1, phantom.t2, phantom.db0, torch.view_as_real(coils),
                                seq.dt, seq.rf, seq.grad, seq.adc_phase, m0, phantom.b1, seq.b0_offset)
     return SimResult(torch.view_as_complex(sig.contiguous()).T, m)

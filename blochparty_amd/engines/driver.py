@@ -30,8 +30,8 @@ from . import hip, pulses
 
 BLOCK = 256
 RF, ADC, SPOIL, SAME_FREE, SAME_RF_DT, NEXT_FREE, NEXT_RF_DT, PULSE, PULSE_SPOIL, RUN, SAME_RUN, TRAIN, MERGED, \
-    SPOILED_FREE, NUFFT, XY_DEAD, SAME_FREE_DT = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384,
-                                                  32768, 65536)
+    SPOILED_FREE, NUFFT, XY_DEAD, SAME_FREE_DT, PULSE_BODY = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096,
+                                                              8192, 16384, 32768, 65536, 131072)
 NUFFT_RUNS = 64  # NUFFT runs per dispatch at most (slots of the grid partials)
 MIN_RUN = 4  # readout runs shorter than this are stepped segment by segment
 MIN_TRAIN = 4  # likewise RF trains
@@ -129,6 +129,10 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
                     and rec["f"][cur, 0] == rec["f"][prev, 0]):
                 flags[cur] |= SAME_RUN
     mark_trains(flags, rec, is_rf, adc, spoil, chunk_id, stepped)
+    for _, k, occ in pulses:  # an occurrence's other segments: the identity (its map is applied at its first)
+        for a, _ in occ:
+            flags[a + 1 : a + k] = PULSE_BODY
+            rec["pad"][a + 1 : a + k, 3] = a
     rec["flags"] = flags
     w = -2 * math.pi * dt  # the rotation-vector factors, formed in float64
     rec["w"] = np.stack([w, w * rf[:, 0], w * rf[:, 1], w * g[:, 0], w * g[:, 1], w * g[:, 2], w * b0,
@@ -424,8 +428,6 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
     vals = 16 * n_tiles
     rows_cap = max(8, (partial_bytes // (4 * vals * n_blocks)) // 8 * 8)
     reused = pulses.reused_pulses(seq) if reuse else []
-    if reused and aligned:
-        raise ValueError("pulses='reuse' is forward only")
     spans = [(a, k) for _, k, occ in reused for a, _ in occ]
     adc_np = np.asarray(seq.adc, dtype=np.int64)
     rows_w = adc_np.copy()  # samples with a partial row (not in NUFFT runs)
@@ -556,7 +558,7 @@ def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> l
                               case.sig.data_ptr())
         if nu is not None and nu["n_slots"][d]:
             pl = nu["plan"]
-            lib.nufft_grid(nu["part"].data_ptr(), nu["runs_max"], nu["cols"], case.n_tiles, nu["n_slots"][d],
+            lib.nufft_grid(nu["part"].data_ptr(), n_blocks, nu["runs_max"], nu["cols"], case.n_tiles, nu["n_slots"][d],
                            nu["slots"][d].data_ptr(), nu["kind_of"].data_ptr(), pl.ng, pl.terms,
                            nu["nu_csr_off"].data_ptr(), nu["nu_csr"].data_ptr(), nu["nu_csr_base"].data_ptr(),
                            nu["grid"].data_ptr())

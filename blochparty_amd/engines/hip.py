@@ -15,7 +15,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent.parent / "libblochparty_amd.so"
-ABI_VERSION = 16
+ABI_VERSION = 18
 MAX_COILS = 128  # 16 tiles of 8 coils (the forward kernel's LDS budget)
 
 
@@ -54,6 +54,8 @@ class HbAdjoint(ctypes.Structure):
         ("gseg", ctypes.c_void_p),
         ("h", ctypes.c_void_p),
         ("sens", ctypes.c_void_p),
+        ("maps_d", ctypes.c_void_p),
+        ("seeds", ctypes.c_void_p),
         ("nc", ctypes.c_int32),
         ("seg_grads", ctypes.c_int32),
         ("coil_grads", ctypes.c_int32),
@@ -82,11 +84,13 @@ _P, _I, _V = ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p
 SIGNATURES = {
     "hb_forward": [ctypes.POINTER(HbCase), _I, _I, _P, _I, _P, _V],
     "hb_signal_reduce": [_P, _I, _I, _I, _I, _P, _I, _P, _P, _P, _V],
-    "hb_nufft_grid": [_P, _I, _I, _I, _I, _P, _P, _I, _I, _P, _P, _P, _P, _V],
+    "hb_nufft_grid": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _P, _P, _P, _P, _V],
     "hb_nufft_signal": [_P, _I, _P, _P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _V],
     "hb_adjoint_interval": [ctypes.POINTER(HbCase), ctypes.POINTER(HbAdjoint), _I, _I, _V],
     "hb_seg_reduce": [_P, _I, _I, _I, _I, _P, _P, _V],
     "hb_pulse_maps": [ctypes.POINTER(HbCase), _P, _P, _I, _P, _I, _P, _V],
+    "hb_pulse_map_grads": [ctypes.POINTER(HbCase), _P, _P, _I, _P, _P, _P, _V],
+    "hb_pulse_map_dlnb1": [ctypes.POINTER(HbCase), _P, _P, _I, ctypes.c_double, _P, _V],
     "hb_spinor_blocks": [_P, _P, _P, _I, _P, _V],
     "hb_mlp_grad": [_P, _P, _P, _I, _P, _P, _V],
     "hb_mlp_train_step": [_P, _P, _P, _I, _P, _I, ctypes.c_float, _V],

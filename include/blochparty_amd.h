@@ -44,6 +44,9 @@ extern "C" {
                              the run's jump relaxes Mz only and zeroes Mxy                                     */
 #define HB_SAME_FREE_DT 65536 /* free segment with the previous free one's dt here (reuse its relaxation factors,
                              also where the phase differs)                                                    */
+#define HB_PULSE_BODY 131072 /* a segment of a reused pulse's occurrence after its first (the map at the first one
+                             covers it): the identity, in tables with checkpoints (the adjoint's); pad[3] the
+                             occurrence's first segment                                                       */
 #define HB_NUFFT_MAX_TERMS 8
 
 /* One segment of the event list, 128 bytes (the reference's segs row + flags + rows, driver.py:357-384). psi is the
@@ -89,8 +92,8 @@ typedef struct hb_case {
 /* NUFFT readouts of a dispatch (readouts="nufft"; the reference's nu buffer and grid rows, driver.py:549-580).
  * params: (8 + HB_NUFFT_MAX_TERMS,) float32 ng, width, terms, beta, shift (= n / 2 + 1), cols, then the decay nodes
  * tau_m [s], then 2 / width and beta log2(e); win: (kinds, n_pad / 256, 2) every workgroup's window start (grid cell, may be negative) and width
- * (cells) per run kind; part: (n_pad / 256, runs_max, cols, 16 n_tiles) the workgroups' window columns (column =
- * local cell * terms + term) of the dispatch's runs, by slot. */
+ * (cells) per run kind; part: (runs_max, n_pad / 256, cols, 16 n_tiles) the workgroups' window columns (column =
+ * local cell * terms + term) of the dispatch's runs, by slot (each slot's columns one dense stretch). */
 typedef struct hb_nufft {
     const float* params;
     const int32_t* win;
@@ -120,11 +123,11 @@ int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t rows, int32
 /* The NUFFT grids of a dispatch's runs, float64 (replaces the float atomics of the reference's grid rows): for slot
  * r < n_slots (run run_of[r], kind kind_of[run]), term m, cell u < ng and value v < 16 n_tiles,
  * grid[((r * terms + m) * ng + u) * vals + v] = sum over the entries (block b, local cell lc) of cell u's list, in
- * order, of part[((b * runs_max + r) * cols + lc * terms + m) * vals + v]. csr_off: (kinds, ng + 1) list offsets
- * (from csr_base[kind]); csr: (entries, 2). */
-int hb_nufft_grid(const float* part, int32_t runs_max, int32_t cols, int32_t n_tiles, int32_t n_slots,
-                  const int32_t* run_of, const int32_t* kind_of, int32_t ng, int32_t terms, const int32_t* csr_off,
-                  const int32_t* csr, const int32_t* csr_base, double* grid, void* stream);
+ * order, of part[((r * n_blocks + b) * cols + lc * terms + m) * vals + v]. csr_off: (kinds, ng + 1) list offsets
+ * (from csr_base[kind]); csr: (entries, 2); n_blocks = n_pad / 256. */
+int hb_nufft_grid(const float* part, int32_t n_blocks, int32_t runs_max, int32_t cols, int32_t n_tiles,
+                  int32_t n_slots, const int32_t* run_of, const int32_t* kind_of, int32_t ng, int32_t terms,
+                  const int32_t* csr_off, const int32_t* csr, const int32_t* csr_base, double* grid, void* stream);
 
 /* The NUFFT readouts' samples from the grids (replaces driver.py:703-725's FFT and recombination): for slot r,
  * sample i < n and coil c, sig[(row_of[run] + i) * n_coils + c] = exp(-i psi) sum_m lw[m][i] sum_u
@@ -142,6 +145,20 @@ int hb_nufft_signal(const double* grid, int32_t n_slots, const int32_t* run_of, 
  * the surrogate's judge and fallback compose a few spins without paying for the rest). */
 int hb_pulse_maps(const hb_case* cs, const double* table, const int32_t* index, int32_t n_pulses,
                   const int32_t* spin_list, int32_t n_list, float* maps, void* stream);
+
+/* Gradients through reused pulses' maps (no counterpart in the reference, whose reused pulses are forward only,
+ * driver.py:221-225). A map's dependence on the spin's parameters (px, py, pz, db0, pd, r1, r2, b1) is taken by
+ * central differences of the float64 composition of hb_pulse_maps (steps: (n_pulses, 8) float64 step per parameter
+ * and pulse, chosen by the caller so that the pulse's rotation moves by ~1e-4 rad: truncation ~1e-9 relative).
+ * hb_pulse_map_grads: gs[a * n_pad + spin] += sum over pulses p of <seeds_p, d map_p / d theta_a>, seeds: (n_pulses,
+ * 12, n_pad) float64 in the maps' row order (the adjoint's sums of lambda m^T over D's entries and of lambda over
+ * v's, in the pulse's frame); gs as hb_adjoint's. hb_pulse_map_dlnb1: maps_d (the maps' layout) = b1 d map / d b1
+ * (the derivative along the pulse's amplitude: a pulse scaled by (1 + e) maps as one with b1 (1 + e)), relative
+ * step rel. */
+int hb_pulse_map_grads(const hb_case* cs, const double* table, const int32_t* index, int32_t n_pulses,
+                       const double* steps, const double* seeds, double* gs, void* stream);
+int hb_pulse_map_dlnb1(const hb_case* cs, const double* table, const int32_t* index, int32_t n_pulses, double rel,
+                       float* maps_d, void* stream);
 
 /* ---- The learned RF surrogate (no counterpart in the reference: it stands where the reference composes a reused
  * pulse per spin, cuda/pulse.cu.in:3-32). A pulse is cut into HB_SUR_BLOCKS blocks; a network 3 -> 64 -> 64 -> 4 M
@@ -210,7 +227,13 @@ int hb_surrogate_maps(const hb_case* cs, const hb_mlp* net, const hb_sur_pulse* 
  * db0, pd, r1, r2, b1); gc: (2 nc, n_pad) float64 accumulators of dL/d(Re S_c, Im S_c) (when coil_grads); gseg:
  * (n_pad / 256, k_max, 8) per-workgroup sums of dL/d(dt, rf_x, rf_y, g_x, g_y, g_z, b0_offset, -) of the interval's
  * segments (when seg_grads); h: (n_adc, nc, 2) dL/d(signal) turned by exp(+i psi); sens: (2 nc, n_pad) Re S_c,
- * Im S_c. nc in {0, 1, 2, 4, 8, 16} (coils padded with zeros). The interval length is at most k_max. */
+ * Im S_c. nc in {0, 1, 2, 4, 8, 16} (coils padded with zeros). The interval length is at most k_max.
+ * Reused pulses (HB_PULSE records, cs->maps; HB_PULSE_BODY records are the identity): lambda goes back through the
+ * occurrence's map, seeds (n_pulses, 12, n_pad) float64 accumulate per spin, in the pulse's frame (turned by -psi),
+ * lambda m^T (rows 0..8, as D's entries) and lambda (rows 9..11, as v's) with lambda after the map and m before it
+ * (their parameter gradients follow by hb_pulse_map_grads); with seg_grads, the occurrence's first segment's rf_x,
+ * rf_y slots get the sums over spins of dL/d(ln amplitude) (from maps_d, hb_pulse_map_dlnb1) and dL/d(RF phase) of
+ * the occurrence, its other slots 0. maps_d and seeds may be NULL when the table has no HB_PULSE record. */
 typedef struct hb_adjoint {
     const float* ckpt;
     float* r8;
@@ -220,6 +243,8 @@ typedef struct hb_adjoint {
     float* gseg;
     const float* h;
     const float* sens;
+    const float* maps_d;
+    double* seeds;
     int32_t nc, seg_grads, coil_grads, k_max;
 } hb_adjoint;
 

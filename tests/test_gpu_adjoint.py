@@ -160,3 +160,79 @@ def test_ernst_angle_by_gradient_descent(gpu):
     ernst = math.degrees(math.acos(math.exp(-tr / t1)))
     # |signal| is even in the flip angle: the descent may end at -ernst (as the reference's test allows)
     assert abs(abs(float(flip.detach())) - ernst) < 0.01, (float(flip.detach()), ernst)
+
+
+# ---- reused pulses as maps in the adjoint (pulses="reuse"; the reference's reused pulses are forward only)
+
+
+def pulse_case(n=8, seed=3):
+    """A 2D GRE with a sinc slice pulse (ramps, RF spoiling) on spins spread through the slice, B1 varying."""
+    ph = bp.shepp_logan(n)
+    g = torch.Generator().manual_seed(seed)
+    pos = ph.pos.clone()
+    pos[:, 2] = (torch.rand(len(ph), generator=g, dtype=f64) - 0.5) * 4e-3
+    ph = replace(ph, pos=pos, b1=torch.linspace(0.9, 1.1, len(ph), dtype=f64),
+                 db0=(torch.rand(len(ph), generator=g, dtype=f64) - 0.5) * 60)
+    seq = bp.gre_2d(n, ramps=True, spoiling="rf", slice_thickness=5e-3, n_dummy=3)
+    m0 = torch.randn(len(ph), 3, generator=g, dtype=f64) * 0.3
+    return ph, seq, bp.birdcage_sensitivities(ph.pos, 3), m0
+
+
+def occurrence_index(seq) -> torch.Tensor:
+    """Per segment, the number of RF runs begun up to it (an occurrence's segments share it)."""
+    is_rf = torch.tensor(seq.is_rf)
+    start = is_rf & ~torch.cat([torch.tensor([False]), is_rf[:-1]])
+    return torch.cumsum(start.to(f64), 0)
+
+
+def map_gradients(sim, ph, seq, coils, m_init, **kw):
+    """d loss / d(spin inputs, coils, m_init) and d loss / d(flip, theta) with rf = flip rf0 e^{i theta n}, n the
+    occurrence's index: a common amplitude and a per-occurrence phase, both through rf."""
+    leaves = {}
+
+    def leaf(name, t):
+        t = t.detach().clone().requires_grad_()
+        leaves[name] = t
+        return t
+
+    p = replace(ph, **{k: leaf(k, getattr(ph, k)) for k in ("pos", "pd", "t1", "t2", "db0", "b1")})
+    flip, theta = leaf("flip", torch.tensor(1.0, dtype=f64)), leaf("theta", torch.tensor(0.0, dtype=f64))
+    rf0 = torch.complex(seq.rf[:, 0], seq.rf[:, 1])
+    rf = rf0 * flip * torch.exp(1j * theta * occurrence_index(seq))
+    s = replace(seq, rf=torch.view_as_real(rf))
+    sig, m = sim(p, s, leaf("coils", coils), leaf("m_init", m_init), **kw)
+    grads = torch.autograd.grad(weighted_loss(sig, m), list(leaves.values()))
+    return dict(zip(leaves, grads))
+
+
+def test_reused_pulse_maps_in_the_adjoint(gpu):
+    ph, seq, coils, m0 = pulse_case()
+    ref = map_gradients(oracle_sim, ph, seq, coils, m0)
+    out = map_gradients(gpu_sim, ph, seq, coils, m0, pulses="reuse", checkpoint_every=64)
+    step = map_gradients(gpu_sim, ph, seq, coils, m0, checkpoint_every=64)
+    errors = {k: rel(out[k], ref[k]) for k in ref if ref[k].abs().max() > 0}
+    errors_step = {k: rel(step[k], ref[k]) for k in ref if ref[k].abs().max() > 0}
+    print("maps", {k: f"{v:.1e}" for k, v in errors.items()}, flush=True)
+    print("steps", {k: f"{v:.1e}" for k, v in errors_step.items()}, flush=True)
+    assert set(errors) == set(ref)  # every gradient is exercised (none is zero in the oracle)
+    assert max(errors.values()) < 1e-4, errors
+
+
+def test_reused_pulse_gradients_bit_identical_for_any_checkpoint_interval(gpu):
+    """Occurrences straddle the intervals differently for each interval length (the map is applied at an
+    occurrence's first segment, the rest is the identity)."""
+    ph, seq, coils, m0 = pulse_case(6, seed=4)
+    runs = [map_gradients(gpu_sim, ph, seq, coils, m0, pulses="reuse", checkpoint_every=k) for k in (64, 192)]
+    for k in runs[0]:
+        assert torch.equal(runs[0][k], runs[1][k]), k
+
+
+def test_reused_pulse_gradients_refuse_timing_and_gradient_inputs(gpu):
+    ph, seq, coils, m0 = pulse_case(4)
+    for name in ("dt", "grad"):
+        s = replace(seq, **{name: getattr(seq, name).clone().requires_grad_()})
+        with pytest.raises(ValueError, match="pulses='step'"):
+            bp.simulate(ph, s, coils, engine="gpu", pulses="reuse")
+    t1 = ph.t1.clone().requires_grad_()
+    with pytest.raises(ValueError, match="forward only"):
+        bp.simulate(replace(ph, t1=t1), seq, coils, engine="gpu", readouts="nufft")

@@ -158,9 +158,13 @@ def test_reused_pulses_match_stepping_and_oracle(gpu, make):
     assert torch.equal(split.signal, reuse.signal) or rel(split.signal, reuse.signal) < 1e-6
 
 
-def test_reused_pulses_are_forward_only(gpu):
+def test_reused_pulses_under_autograd_keep_the_forward(gpu):
+    """With gradients, reused pulses run as maps on checkpoint intervals (occurrences may straddle them): the signal
+    is the forward's."""
     ph = bp.shepp_logan(8)
     t1 = ph.t1.clone().requires_grad_()
     seq = bp.gre_2d(8, ramps=True, spoiling="rf", slice_thickness=5e-3)
-    with pytest.raises(ValueError, match="forward only"):
-        bp.simulate(bp.Phantom(ph.pos, ph.pd, t1, ph.t2, ph.db0), seq, pulses="reuse")
+    fwd = bp.simulate(ph, seq, pulses="reuse")
+    for every in (64, 320):
+        res = bp.simulate(bp.Phantom(ph.pos, ph.pd, t1, ph.t2, ph.db0), seq, pulses="reuse", checkpoint_every=every)
+        assert rel(res.signal.detach(), fwd.signal) < 1e-6, every
