@@ -15,9 +15,11 @@ Legs, K timed steps each after W warmup steps:
   trained once before the timed steps (its training time is reported, not timed);
 - forward with reused exact maps (pulses="reuse": the sinc pulse composed once per spin in float64), same readouts;
 - forward stepping every RF substep (the exact-RF baseline);
-- forward + adjoint of L = sum |signal|^2 w.r.t. the flip angle (through the per-segment RF gradients), the T1 map
-  and the coil maps (the reference's --grad case, benchmarks/bench.py:69-95), exact RF: the forward with checkpoints,
-  dL/dsignal on the device, and the reverse sweep.
+- forward + adjoint of L = sum |signal|^2 w.r.t. the flip angle (through the RF gradients), the T1 map and the coil
+  maps (the reference's --grad case, benchmarks/bench.py:69-95): the forward with checkpoints, dL/dsignal on the
+  device, and the reverse sweep; with exact RF (every substep stepped and reversed), then with the learned
+  surrogate's maps (each occurrence one map in both sweeps, the maps' parameter gradients from float64 central
+  differences of the exact composition once per step). Both report dL/dflip, which should agree.
 Every forward-kernel launch of the value's leg is bracketed by HIP events on the stream it runs on; the roofline's
 ``achieved`` is the algorithmic flops per launch over their mean duration. ``traffic`` is the HBM bytes per launch
 of the same kernel from the committed rocprofv3 PMC summary (profiles/), when present.
@@ -40,6 +42,7 @@ import torch
 
 METRIC = "spin·samples/s fwd & fwd+adjoint, 3D GRE 64³; RF surrogate vs exact @1e-4"
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, published
+FLIP_DEG = 10.0  # the workload's flip angle: rf = FLIP_DEG x (the pulse's shape per degree)
 
 # counting convention of the algorithmic work (per spin)
 FLOPS_RF = 48  # exact RF substep
@@ -140,6 +143,47 @@ def forward_leg(case, lib, driver, steps: int, warmup: int, work: int, label: st
     return out
 
 
+def fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, steps: int, warmup: int, work: int,
+                    label: str) -> dict:
+    """Forward with checkpoints + reverse sweep of L = sum |s|^2 w.r.t. the flip angle, the T1 map and the coils.
+    rf = flip x shape: dL/dflip = sum_s rf_s . dL/drf_s / flip for stepped RF; a reused pulse's occurrence carries
+    dL/d(ln amplitude) in its first segment's rf_x slot."""
+    S, n_adc = len(seq), seq.n_adc
+    nc = next(c for c in adjoint.ADJOINT_COILS if c >= case_a.n_coils)
+    dev = case_a.spins.device
+    sens = adjoint.adjoint_sens(coils, case_a.n_pad, nc).to(dev)
+    rf64 = seq.rf.double().to(dev)
+    heads = torch.tensor([a for _, _, occ in case_a.pulses for a, _ in occ], dtype=torch.int64, device=dev)
+    out = {}
+
+    def step():
+        ckpts = driver.run_forward(case_a, lib, keep=True)
+        sig = case_a.sig[:n_adc]  # L = sum |s|^2: dL/dsignal = 2 s, turned by exp(+i psi)
+        gr, gi = 2 * sig[..., 0], 2 * sig[..., 1]
+        cp, sp = torch.cos(case_a.psi_rows)[:, None], torch.sin(case_a.psi_rows)[:, None]
+        h = torch.zeros(n_adc, nc, 2, dtype=torch.float32, device=dev)
+        h[:, : case_a.n_coils, 0] = (gr * cp - gi * sp).float()
+        h[:, : case_a.n_coils, 1] = (gr * sp + gi * cp).float()
+        lam = torch.zeros(3, case_a.n_pad, dtype=torch.float32, device=dev)
+        gs, gc, segg, _ = adjoint.run_backward(case_a, ckpts, every, h, lam, sens, nc, 2, True, lib)
+        sg = segg.view(S, 8)
+        if len(heads):
+            out["flip"] = sg[heads, 1].sum() / FLIP_DEG
+        else:
+            out["flip"] = (sg[:, 1:3] * rf64).sum() / FLIP_DEG
+        out["t1"] = gs[5]
+        out["coils"] = gc
+
+    t_a = timed(step, steps, warmup)
+    assert torch.isfinite(out["flip"]) and torch.isfinite(out["t1"]).all() and torch.isfinite(out["coils"]).all()
+    leg = {"value": work * steps / t_a, "ms_per_step": 1e3 * t_a / steps, "rf": label, "checkpoint_every": every,
+           "loss": "sum |signal|^2", "wrt": ["flip angle", "T1 map", "coil maps"],
+           "dL_dflip_per_deg": float(out["flip"])}
+    log(f"forward + adjoint ({label}): {leg['ms_per_step']:.1f} ms/step, {leg['value']:.3e} spin-samples/s, "
+        f"dL/dflip {leg['dL_dflip_per_deg']:.6e}")
+    return leg
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,39 +251,21 @@ def main() -> None:
         del case_e
         torch.cuda.empty_cache()
 
-    # -- forward + adjoint, exact RF
-    fwd_adj = None
+    # -- forward + adjoint: exact RF, then the learned surrogate's maps
+    fwd_adj = fwd_adj_sur = None
     if not args.no_adjoint:
-        n_adc = seq.n_adc
         every = adjoint.default_checkpoint_every(S)
         case_a = driver.prepare_case(ph, seq, coils, chunk=every, aligned=True)
-        nc = next(c for c in adjoint.ADJOINT_COILS if c >= case_a.n_coils)
-        sens = adjoint.adjoint_sens(coils, case_a.n_pad, nc).to(case_a.spins.device)
-        rf64 = seq.rf.double().to(case_a.spins.device)
-        out = {}
-
-        def fwd_adj_step():
-            ckpts = driver.run_forward(case_a, lib, keep=True)
-            sig = case_a.sig[:n_adc]  # L = sum |s|^2: dL/dsignal = 2 s, turned by exp(+i psi)
-            gr, gi = 2 * sig[..., 0], 2 * sig[..., 1]
-            cp, sp = torch.cos(case_a.psi_rows)[:, None], torch.sin(case_a.psi_rows)[:, None]
-            h = torch.zeros(n_adc, nc, 2, dtype=torch.float32, device=sig.device)
-            h[:, : case_a.n_coils, 0] = (gr * cp - gi * sp).float()
-            h[:, : case_a.n_coils, 1] = (gr * sp + gi * cp).float()
-            lam = torch.zeros(3, case_a.n_pad, dtype=torch.float32, device=sig.device)
-            gs, gc, segg, _ = adjoint.run_backward(case_a, ckpts, every, h, lam, sens, nc, True, True, lib)
-            out["flip"] = (segg.view(S, 8)[:, 1:3] * rf64).sum() / 10.0  # rf = flip * shape: dL/dflip
-            out["t1"] = gs[5]
-            out["coils"] = gc
-
-        t_a = timed(fwd_adj_step, args.steps, args.warmup)
-        assert torch.isfinite(out["flip"]) and torch.isfinite(out["t1"]).all() and torch.isfinite(out["coils"]).all()
-        fwd_adj = {"value": work * args.steps / t_a, "ms_per_step": 1e3 * t_a / args.steps, "rf": "exact",
-                   "checkpoint_every": every, "loss": "sum |signal|^2", "wrt": ["flip angle", "T1 map", "coil maps"]}
-        log(f"forward + adjoint (exact RF): {fwd_adj['ms_per_step']:.1f} ms/step, {fwd_adj['value']:.3e} "
-            f"spin-samples/s")
+        fwd_adj = fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, args.steps, args.warmup, work,
+                                  "exact RF")
         del case_a
         torch.cuda.empty_cache()
+        if not args.no_surrogate:
+            case_a = driver.prepare_case(ph, seq, coils, chunk=every, aligned=True, surrogate=True)
+            fwd_adj_sur = fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, args.steps, args.warmup,
+                                          work, "learned RF surrogate")
+            del case_a
+            torch.cuda.empty_cache()
 
     if args.copy_probe:  # a plain copy of known size: x * 1 into y reads and writes copy_probe MiB each
         x = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
@@ -317,6 +343,9 @@ def main() -> None:
         "exact_rf_forward": exact,
         "speedup_vs_exact_rf": None if exact is None else lead["value"] / exact["value"],
         "fwd_adjoint": fwd_adj,
+        "fwd_adjoint_surrogate": fwd_adj_sur,
+        "fwd_adjoint_speedup_vs_exact_rf": (None if fwd_adj is None or fwd_adj_sur is None
+                                            else fwd_adj_sur["value"] / fwd_adj["value"]),
         "cpu_baseline": cpu,
         "parity": parity,
     }

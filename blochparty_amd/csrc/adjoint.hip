@@ -125,14 +125,18 @@ __device__ __forceinline__ void pulse_reverse(const hb_segment& q, const hb_case
     lx = nlx, ly = nly, lz = bz;
 }
 
+// Two waves per SIMD need at most 256 registers each, VGPRs and AGPRs together: the attribute holds the allocation
+// there (one wave per SIMD halves the sweep's speed); 16 coils do not fit without spilling and keep one wave.
 template <int NC, bool SEG>
-__global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint ad, int s0, int s1) {
+__global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8 ? 2 : 1))) void adjoint_kernel(
+    hb_case cs, hb_adjoint ad, int s0, int s1) {
     __shared__ float segbuf[kWavesA][8][8];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = blockIdx.x * kBlockA + tid;
     const int np = cs.n_pad;
     const Spin p = load_spin(cs.spins, cs.pos_lo, np, j);
     const float b1sq = p.b1 * p.b1;
+    const bool rf_only = SEG && ad.seg_grads == 2;
 
     // 1. replay the interval, r8[b] = state before segment s0 + 8 b (low parts folded)
     {
@@ -238,6 +242,7 @@ __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint
         }
         float sg[N_SEG_GRADS];
         f4 segd = {0.0f, 0.0f, 0.0f, 0.0f};
+        bool seg_block = !rf_only;  // the block's segment gradients are formed (rf_only: it holds an RF record)
 #pragma unroll
         for (int i = 7; i >= 0; --i) {
             if (i < nseg) {
@@ -248,6 +253,7 @@ __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint
                     if (q.flags & HB_PULSE_SPOIL) lx = ly = 0.0f;
                     pulse_reverse<SEG>(q, cs, ad, j, before[i], after[i], lx, ly, lz, sg);
                     if (SEG) seg_add(segd, sg, i, lane);
+                    seg_block = true;
                     continue;
                 }
                 const int f = live_flags(q.flags, rfree_ok, rrf_ok, HB_NEXT_FREE, HB_NEXT_RF_DT);
@@ -271,10 +277,15 @@ __global__ __launch_bounds__(kBlockA) void adjoint_kernel(hb_case cs, hb_adjoint
                                ((f & HB_NEXT_RF_DT) ? HB_SAME_RF_DT : 0);
                 reverse_step<SEG>(q, fr, p, before[i].x, before[i].y, before[i].z, lx, ly, lz, acc, sg, rc);
                 if (f & HB_RF) rrf_ok = true; else rfree_ok = true;
-                if (SEG) seg_add(segd, sg, i, lane);
+                if (SEG && (!rf_only || (f & HB_RF))) {
+                    seg_add(segd, sg, i, lane);
+                    seg_block = true;
+                }
             }
         }
-        if (SEG) {
+        if (SEG && !seg_block) {  // rf_only and no RF record here: zeros, without the workgroup sum
+            if (tid < 8 * nseg) ad.gseg[((size_t)blockIdx.x * ad.k_max + (sb - s0)) * 8 + tid] = 0.0f;
+        } else if (SEG) {
             seg_store(segd, segbuf[wave], lane);
             __syncthreads();
             if (tid < 8 * nseg) {

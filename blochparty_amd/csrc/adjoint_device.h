@@ -68,14 +68,18 @@ __device__ __forceinline__ void reverse_step(const hb_segment& q, int f, const S
         const float nx = c * m1x + s1 * (wy * m1z - wz * m1y) + s2 * wx * dot;
         const float ny = c * m1y + s1 * (wz * m1x - wx * m1z) + s2 * wy * dot;
         const float nz = c * m1z + s1 * (wx * m1y - wy * m1x) + s2 * wz * dot;
-        // second half relaxation, reversed
+        // second half relaxation, reversed (lambda + lambda d: one rounding, where a factor 1 + d rounded to float32
+        // would bias every substep the same way)
         float gE2 = lx * nx + ly * ny, gE1 = lz * (nz - p.pd), gpd = -lz * d1;
-        const float ax = lx * e2, ay = ly * e2, az = lz * e1;
-        // rotation, reversed
+        const float ax = lx + lx * d2, ay = ly + ly * d2, az = lz + lz * d1;
+        // rotation, reversed: R^T a = a - s1 (w x a) + s2 w x (w x a), an increment to a (c = 1 - s2 |w|^2 is not
+        // formed: rounded near 1 it would bias the norm of lambda at every substep)
         const float wa = wx * ax + wy * ay + wz * az;
-        const float bx = c * ax - s1 * (wy * az - wz * ay) + s2 * wx * wa;
-        const float by = c * ay - s1 * (wz * ax - wx * az) + s2 * wy * wa;
-        const float bzz = c * az - s1 * (wx * ay - wy * ax) + s2 * wz * wa;
+        const float cxa = wy * az - wz * ay, cya = wz * ax - wx * az, cza = wx * ay - wy * ax;
+        const float exa = wy * cza - wz * cya, eya = wz * cxa - wx * cza, eza = wx * cya - wy * cxa;
+        const float bx = ax + (s2 * exa - s1 * cxa);
+        const float by = ay + (s2 * eya - s1 * cya);
+        const float bzz = az + (s2 * eza - s1 * cza);
         const float am = ax * m1x + ay * m1y + az * m1z;
         const float awm = ax * (wy * m1z - wz * m1y) + ay * (wz * m1x - wx * m1z) + az * (wx * m1y - wy * m1x);
         const float kk = -s1 * am + f1 * awm + f2 * wa * dot;
@@ -86,9 +90,9 @@ __device__ __forceinline__ void reverse_step(const hb_segment& q, int f, const S
         gE2 += bx * x0 + by * y0;
         gE1 += bzz * (z0 - p.pd);
         gpd -= bzz * d1;
-        lx = bx * e2;
-        ly = by * e2;
-        lz = bzz * e1;
+        lx = bx + bx * d2;
+        ly = by + by * d2;
+        lz = bzz + bzz * d1;
         acc.v[A_R2] -= gE2 * h * e2;
         acc.v[A_R1] -= gE1 * h * e1;
         acc.v[A_PD] += gpd;
@@ -113,9 +117,9 @@ __device__ __forceinline__ void reverse_step(const hb_segment& q, int f, const S
         const float gE1 = lz * (z0 - p.pd);
         const float ax = c * lx + sn * ly, ay = c * ly - sn * lx;
         acc.v[A_PD] -= lz * rc.d1;
-        lx = ax * e2;
-        ly = ay * e2;
-        lz = lz * e1;
+        lx = ax + ax * rc.d2;
+        ly = ay + ay * rc.d2;
+        lz = lz + lz * rc.d1;
         acc.v[A_R2] -= gE2 * dt * e2;
         acc.v[A_R1] -= gE1 * dt * e1;
         gb = gphi * w;

@@ -98,9 +98,11 @@ class GPUSimulate(torch.autograd.Function):
         if g_m is not None:
             lam[:, :n] = g_m.detach().float().cpu().T
         need_seg = any(need[i] for i in (6, 7, 8, 12))
+        # 2: only rf's gradients (the sweep forms segment gradients only where an RF record is)
+        seg_mode = 0 if not need_seg else 1 if any(need[i] for i in (6, 8, 12)) else 2
         need_coil = bool(nc) and need[5]
         gs, gc, seg, lam_d = run_backward(case, ckpts, every, h.float().to(dev).contiguous(), lam.to(dev).contiguous(),
-                                          sens.to(dev).contiguous(), nc, need_seg, need_coil)
+                                          sens.to(dev).contiguous(), nc, seg_mode, need_coil)
         hip.HIP.synchronize()
         GS = gs.cpu()[:, :n]
         lam = lam_d.cpu().double()[:, :n]
@@ -137,11 +139,12 @@ class GPUSimulate(torch.autograd.Function):
 
 
 def run_backward(case: driver.DeviceCase, ckpts: list, every: int, h: torch.Tensor, lam: torch.Tensor,
-                 sens: torch.Tensor, nc: int, need_seg: bool, need_coil: bool, lib=None):
+                 sens: torch.Tensor, nc: int, need_seg: int, need_coil: bool, lib=None):
     """The adjoint kernels on the current stream over ``case`` (aligned dispatches, forward run with ``keep``):
     ``h`` (n_adc, nc, 2) float32 dL/dsignal turned by exp(+i psi), ``lam`` (3, n_pad) float32 dL/dm at the end (it
-    ends as dL/dm0), ``sens`` (2 nc, n_pad). Returns the float64 accumulators (gs (8, n_pad), gc (2 nc, n_pad),
-    seg (S, 8)) and lam."""
+    ends as dL/dm0), ``sens`` (2 nc, n_pad). ``need_seg``: 0 no segment gradients, 1 all, 2 only rf's (the other
+    slots come back 0). Returns the float64 accumulators (gs (8, n_pad), gc (2 nc, n_pad), seg (S, 8)) and lam."""
+    need_seg = int(need_seg)
     lib = lib or hip.HIP.lib()
     dev = case.spins.device
     n_pad, S = case.n_pad, case.n_segments

@@ -226,7 +226,9 @@ int hb_surrogate_maps(const hb_case* cs, const hb_mlp* net, const hb_sur_pulse* 
  * lam: (3, n_pad) dL/dm after the interval in, before it out; gs: (8, n_pad) float64 accumulators of dL/d(px, py, pz,
  * db0, pd, r1, r2, b1); gc: (2 nc, n_pad) float64 accumulators of dL/d(Re S_c, Im S_c) (when coil_grads); gseg:
  * (n_pad / 256, k_max, 8) per-workgroup sums of dL/d(dt, rf_x, rf_y, g_x, g_y, g_z, b0_offset, -) of the interval's
- * segments (when seg_grads); h: (n_adc, nc, 2) dL/d(signal) turned by exp(+i psi); sens: (2 nc, n_pad) Re S_c,
+ * segments (seg_grads 1; seg_grads 2: only the rf_x, rf_y slots of RF and HB_PULSE segments are wanted, every other
+ * value is written as 0 without being formed); h: (n_adc, nc, 2) dL/d(signal) turned by exp(+i psi); sens: (2 nc,
+ * n_pad) Re S_c,
  * Im S_c. nc in {0, 1, 2, 4, 8, 16} (coils padded with zeros). The interval length is at most k_max.
  * Reused pulses (HB_PULSE records, cs->maps; HB_PULSE_BODY records are the identity): lambda goes back through the
  * occurrence's map, seeds (n_pulses, 12, n_pad) float64 accumulate per spin, in the pulse's frame (turned by -psi),

@@ -8,6 +8,7 @@ tests/cases.py:151-198): shaped RF with gradients and off-resonance, readouts sp
 from __future__ import annotations
 
 import math
+from dataclasses import replace
 
 import torch
 
@@ -75,3 +76,39 @@ def c1_case(n: int = 64, n_dummy: int = 200):
     ph = bp.shepp_logan(n)
     seq = bp.gre_2d(n, n_dummy=n_dummy, te=5e-3, flip_deg=15.0, tr=10e-3)
     return ph, seq, bp.birdcage_sensitivities(ph.pos, 8)
+
+
+def weighted_loss(signal, m):
+    """Linear in the outputs with fixed seeded float64 weights (the same for the GPU and the oracle)."""
+    g = torch.Generator().manual_seed(1)
+    w = lambda x: torch.randn(x.shape, generator=g, dtype=f64)
+    return ((signal.real.double() * w(signal.real)).sum() + (signal.imag.double() * w(signal.imag)).sum()
+            + (m.double() * w(m)).sum())
+
+
+def occurrence_index(seq) -> torch.Tensor:
+    """Per segment, the number of RF runs begun up to it (an occurrence's segments share it)."""
+    is_rf = torch.tensor(seq.is_rf)
+    start = is_rf & ~torch.cat([torch.tensor([False]), is_rf[:-1]])
+    return torch.cumsum(start.to(f64), 0)
+
+
+def map_gradients(sim, ph, seq, coils, m_init, **kw):
+    """d loss / d(spin inputs, coils, m_init) and d loss / d(flip, theta) with rf = flip rf0 e^{i theta n}, n the
+    occurrence's index: a common amplitude and a per-occurrence phase, both through rf."""
+    leaves = {}
+
+    def leaf(name, t):
+        t = t.detach().clone().requires_grad_()
+        leaves[name] = t
+        return t
+
+    p = replace(ph, **{k: leaf(k, getattr(ph, k)) for k in ("pos", "pd", "t1", "t2", "db0", "b1")
+                       if getattr(ph, k) is not None})
+    flip, theta = leaf("flip", torch.tensor(1.0, dtype=f64)), leaf("theta", torch.tensor(0.0, dtype=f64))
+    rf0 = torch.complex(seq.rf[:, 0], seq.rf[:, 1])
+    rf = rf0 * flip * torch.exp(1j * theta * occurrence_index(seq))
+    s = replace(seq, rf=torch.view_as_real(rf))
+    sig, m = sim(p, s, leaf("coils", coils), None if m_init is None else leaf("m_init", m_init), **kw)
+    grads = torch.autograd.grad(weighted_loss(sig, m), list(leaves.values()))
+    return dict(zip(leaves, grads))

@@ -9,7 +9,7 @@ import torch
 import blochparty_amd as bp
 from blochparty_amd import workloads as W
 from blochparty_amd.engines import driver, pulses, surrogate as sur
-from cases import rel
+from cases import map_gradients, rel
 from oracle import core
 from oracle import surrogate as S
 
@@ -137,6 +137,29 @@ def test_surrogate_simulation_matches_the_oracle(gpu, trained, slab):
     print(f"k-space error vs oracle: surrogate {e_sig:.2e}, exact maps {rel(exact.signal, ref_sig):.2e}")
     assert e_sig < 1e-4
     assert rel(res.m, ref_m) < 1e-3
+
+
+def test_gradients_through_the_surrogate_maps(gpu, trained, slab):
+    """Forward + adjoint with the network's maps (each occurrence one map in both sweeps; the maps' parameter
+    derivatives from the exact float64 composition): every gradient against the oracle's float64 autograd through
+    all 200 substeps, on a seeded subset of the slab's spins over 4 TRs."""
+    ph, seq, coils, _ = slab
+    ph_s, c_s = W.subset(ph, coils, 1024)
+    seq4 = W.first_trs(seq, 4)
+
+    def gpu(p, s, c, m0, **kw):
+        r = bp.simulate(p, s, c, m_init=m0, engine="gpu", **kw)
+        return r.signal, r.m
+
+    ref = map_gradients(lambda p, s, c, m0: core.simulate(p, s, c, m_init=m0), ph_s, seq4, c_s, None)
+    sur_g = map_gradients(gpu, ph_s, seq4, c_s, None, pulses="surrogate", checkpoint_every=128)
+    exact = map_gradients(gpu, ph_s, seq4, c_s, None, pulses="reuse", checkpoint_every=128)
+    err = {k: rel(sur_g[k], ref[k]) for k in ref if ref[k].abs().max() > 0}
+    err_x = {k: rel(exact[k], ref[k]) for k in ref if ref[k].abs().max() > 0}
+    print("surrogate", {k: f"{v:.1e}" for k, v in err.items()}, flush=True)
+    print("exact maps", {k: f"{v:.1e}" for k, v in err_x.items()}, flush=True)
+    assert max(err_x.values()) < 1e-4, err_x
+    assert max(err.values()) < 1e-3, err
 
 
 def test_spins_outside_the_box_fall_back_to_the_exact_maps(gpu, trained, slab):
