@@ -89,7 +89,9 @@ def simulate(
     only), ``pulses`` ("step"; "reuse": a pulse that recurs up to a global RF phase is composed
     once per spin into its affine map, in float64, and applied at every occurrence; "surrogate": those maps from
     the learned RF surrogate, trained on the GPU on first use, with the exact composition as the fallback outside its
-    trained box and as the judge of its error on every run, reported in ``SimResult.surrogate``; both forward only).
+    trained box and as the judge of its error on every run, reported in ``SimResult.surrogate``; under autograd both
+    run as maps in the adjoint too, with gradients for the spin inputs, coils, m_init and each occurrence's RF
+    amplitude and phase, and ValueError when dt, grad or b0_offset require a gradient).
     Other keywords raise TypeError.
     """
     name = resolve(engine)

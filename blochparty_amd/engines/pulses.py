@@ -6,7 +6,7 @@ occurrence maps m to m + Rz(phi) (D Rz(-phi) m + v).
 Reference: ``src/blochparty/engines/gpu/driver.py:109-164`` (grouping: runs of >= 2 consecutive RF segments without
 ADC, a spoil only on the last, with an earlier run's dt, gradient, b0_offset and flags and its RF times e^{i phase}
 to ``tol`` of the peak, phase by least squares) and ``cuda/pulse.cu.in`` + ``kernels.cu.in:141-165`` (maps and their
-application). Forward only, as there.
+application). The reference's are forward only; here the adjoint applies them too (``engines/adjoint.py``).
 """
 
 from __future__ import annotations

@@ -107,7 +107,8 @@ typedef struct hb_nufft {
  * dispatch of driver.py:658-698). Every ADC sample's coil sums per workgroup of 256 spins go to
  * partial[(block * rows + r) * 16 * n_tiles + 2 c + {0: re, 1: im}], r the sample's index among this dispatch's
  * samples that are not in NUFFT runs (< rows), block = spin / 256; no atomics. nu: NULL, or the dispatch's NUFFT
- * runs (HB_NUFFT records), whose window columns go to nu->part. */
+ * runs (HB_NUFFT records), whose window columns go to nu->part. A dispatch whose first segment is HB_PULSE_BODY starts
+ * after that occurrence (its map was applied at the occurrence's first segment, in an earlier dispatch). */
 int hb_forward(const hb_case* cs, int32_t s0, int32_t s1, float* partial, int32_t rows, const hb_nufft* nu,
                void* stream);
 
