@@ -19,7 +19,9 @@ Legs, K timed steps each after W warmup steps:
   maps (the reference's --grad case, benchmarks/bench.py:69-95): the forward with checkpoints, dL/dsignal on the
   device, and the reverse sweep; with exact RF (every substep stepped and reversed), then with the learned
   surrogate's maps (each occurrence one map in both sweeps, the maps' parameter gradients from float64 central
-  differences of the exact composition once per step). Both report dL/dflip, which should agree.
+  differences of the exact composition once per step), and with the surrogate's maps and NUFFT readouts (each
+  readout run reversed as the transpose of its NUFFT: the headline forward + adjoint, whose ratio to the exact-RF leg
+  is ``fwd_adjoint_speedup_vs_exact_rf``). Each reports dL/dflip, which should agree.
 Every forward-kernel launch of the value's leg is bracketed by HIP events on the stream it runs on; the roofline's
 ``achieved`` is the algorithmic flops per launch over their mean duration. ``traffic`` is the HBM bytes per launch
 of the same kernel from the committed rocprofv3 PMC summary (profiles/), when present.
@@ -151,6 +153,8 @@ def fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, steps: int,
     S, n_adc = len(seq), seq.n_adc
     nc = next(c for c in adjoint.ADJOINT_COILS if c >= case_a.n_coils)
     dev = case_a.spins.device
+    if case_a.perm is not None:  # NUFFT readouts: the spins are sorted on the device
+        coils = coils[torch.from_numpy(case_a.perm)]
     sens = adjoint.adjoint_sens(coils, case_a.n_pad, nc).to(dev)
     rf64 = seq.rf.double().to(dev)
     heads = torch.tensor([a for _, _, occ in case_a.pulses for a, _ in occ], dtype=torch.int64, device=dev)
@@ -251,8 +255,8 @@ def main() -> None:
         del case_e
         torch.cuda.empty_cache()
 
-    # -- forward + adjoint: exact RF, then the learned surrogate's maps
-    fwd_adj = fwd_adj_sur = None
+    # -- forward + adjoint: exact RF, then the learned surrogate's maps (per-sample readouts, then NUFFT readouts)
+    fwd_adj = fwd_adj_sur = fwd_adj_nu = None
     if not args.no_adjoint:
         every = adjoint.default_checkpoint_every(S)
         case_a = driver.prepare_case(ph, seq, coils, chunk=every, aligned=True)
@@ -266,6 +270,12 @@ def main() -> None:
                                           work, "learned RF surrogate")
             del case_a
             torch.cuda.empty_cache()
+            if args.readouts == "nufft":
+                case_a = driver.prepare_case(ph, seq, coils, chunk=every, aligned=True, surrogate=True, nufft=(6, 3))
+                fwd_adj_nu = fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, args.steps,
+                                             args.warmup, work, "learned RF surrogate, NUFFT readouts")
+                del case_a
+                torch.cuda.empty_cache()
 
     if args.copy_probe:  # a plain copy of known size: x * 1 into y reads and writes copy_probe MiB each
         x = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
@@ -344,8 +354,9 @@ def main() -> None:
         "speedup_vs_exact_rf": None if exact is None else lead["value"] / exact["value"],
         "fwd_adjoint": fwd_adj,
         "fwd_adjoint_surrogate": fwd_adj_sur,
-        "fwd_adjoint_speedup_vs_exact_rf": (None if fwd_adj is None or fwd_adj_sur is None
-                                            else fwd_adj_sur["value"] / fwd_adj["value"]),
+        "fwd_adjoint_surrogate_nufft": fwd_adj_nu,
+        "fwd_adjoint_speedup_vs_exact_rf": (None if fwd_adj is None or (fwd_adj_nu or fwd_adj_sur) is None
+                                            else (fwd_adj_nu or fwd_adj_sur)["value"] / fwd_adj["value"]),
         "cpu_baseline": cpu,
         "parity": parity,
     }

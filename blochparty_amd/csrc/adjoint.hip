@@ -127,7 +127,113 @@ __device__ __forceinline__ void pulse_reverse(const hb_segment& q, const hb_case
 
 // Two waves per SIMD need at most 256 registers each, VGPRs and AGPRs together: the attribute holds the allocation
 // there (one wave per SIMD halves the sweep's speed); 16 coils do not fit without spilling and keep one wave.
-template <int NC, bool SEG>
+// Reverse through a NUFFT readout run (head record q, m0 the state at the run's start; (lx, ly, lz) = dL/dm after the
+// run in, at its start out). The forward takes the samples s_ic = sum over spins of K_ic S_c m0 (the spreading, the
+// grid's DFT, the decay terms, the demodulation), then jumps over the run. Reversed:
+//   1. the jump (an exact free step of L dt, phase L cyc), with its R2, R1, PD and phase gradients;
+//   2. A_c = e^{2 pi i ph} sum_m e^{-R2 tau_m} sum_u psi(z_u) G[m][u][c] over the workgroup's window (the transpose of
+//      the spreading, G from hb_nufft_adjoint_grid): lambda += sum_c conj(S_c) A_c, the coil gradients += conj(m0)
+//      A_c; for a parameter theta of K, dL/dtheta = Re(m0 conj(sum_c conj(S_c) dA_c/dtheta)): R2 through the decay
+//      terms (-tau_m e^{-R2 tau_m}), the phase per sample through ph and the cells' z (psi', dz/dfr = -ng 2 / width).
+// The phase per sample cyc = bz dt (bz = g.r + db0 + b0_offset) gives the position and db0 gradients.
+template <int NC>
+__device__ __forceinline__ void nufft_reverse(const hb_segment& q, const hb_adjoint& ad, const Spin& p,
+                                              const Stash& m0, float& lx, float& ly, float& lz, Acc& acc,
+                                              const float (&sr)[NC > 0 ? NC : 1], const float (&si)[NC > 0 ? NC : 1],
+                                              float (&gr)[NC > 0 ? NC : 1], float (&gi)[NC > 0 ? NC : 1]) {
+    const float* par = ad.nu_params;
+    const int L = q.pad[3], kind = q.pad[2], run = q.pad[4];
+    const int ng = (int)par[0], terms = (int)par[2], shift = (int)par[4];
+    const float hz = par[6 + HB_NUFFT_MAX_TERMS], bl2 = par[7 + HB_NUFFT_MAX_TERMS];
+    const double cyc = free_cycles64(pos64(p), p, q);
+    const float T = (float)L * q.dt;
+    float dcyc = 0.0f;  // dL/dcyc
+    // 1. the jump
+    if (q.flags & (HB_XY_DEAD | HB_PULSE_SPOIL)) {
+        lx = ly = 0.0f;
+    } else {
+        double cl = cyc * (double)L;
+        cl -= rint(cl);
+        float s2, c2;
+        sincospi(-2.0f * (float)cl, &s2, &c2);
+        const float d2 = expm1x(-T * p.r2), e2 = 1.0f + d2;
+        const float nx = c2 * m0.x - s2 * m0.y, ny = s2 * m0.x + c2 * m0.y;
+        acc.v[A_R2] -= T * e2 * (lx * nx + ly * ny);
+        dcyc -= HB_TWO_PI * (float)L * e2 * (ly * nx - lx * ny);
+        const float ax = lx + lx * d2, ay = ly + ly * d2;
+        lx = c2 * ax + s2 * ay;
+        ly = c2 * ay - s2 * ax;
+    }
+    {
+        const float d1 = expm1x(-T * p.r1);
+        acc.v[A_PD] -= lz * d1;
+        acc.v[A_R1] -= lz * (m0.z - p.pd) * T * (1.0f + d1);
+        lz = lz + lz * d1;
+    }
+    // 2. the samples
+    if constexpr (NC > 0) {
+        const double fr = cyc - floor(cyc);
+        const double xg = fr * ng;
+        double ph = fr * shift;
+        ph -= rint(ph);
+        float sp, cp;
+        sincospi(2.0f * (float)ph, &sp, &cp);  // e^{+2 pi i ph}
+        const float fre = m0.x * cp + m0.y * sp, fim = m0.x * sp - m0.y * cp;  // conj(m0) e^{2 pi i ph}
+        const int* win = ad.nu_win + ((size_t)kind * gridDim.x + blockIdx.x) * 2;
+        const int wb = win[0], W = win[1];
+        const float z0 = (float)(((double)wb - xg) * (double)hz);
+        const float* G = ad.nu_grid + (size_t)run * terms * ng * (2 * NC);
+        const float beta = bl2 * 0.6931471805599453f;
+        float Qr = 0.0f, Qi = 0.0f, Rr = 0.0f, Ri = 0.0f, Xr = 0.0f, Xi = 0.0f;
+        float lcf = 0.0f;
+        for (int lc = 0; lc < W; ++lc, lcf += 1.0f) {
+            const float z = fmaf(lcf, hz, z0);
+            const float u = fmaf(-z, z, 1.0f);
+            const bool in = u > 0.0f;
+            const float su = __builtin_amdgcn_sqrtf(in ? u : 1.0f);
+            const float kv = in ? __builtin_amdgcn_exp2f(fmaf(bl2, su, -bl2)) : 0.0f;
+            const float kd = in ? -kv * beta * z / su : 0.0f;  // d psi / dz
+            int cell = (wb + lc) % ng;
+            if (cell < 0) cell += ng;
+            for (int m = 0; m < terms; ++m) {
+                const float tau = par[6 + m];
+                const float e = __builtin_amdgcn_exp2f(-(tau * 1.4426950408889634f) * p.r2);
+                const float* g = G + ((size_t)m * ng + cell) * (2 * NC);
+                const float w = kv * e;
+                float Pr = 0.0f, Pi = 0.0f;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    const float g_r = g[2 * c], g_i = g[2 * c + 1];
+                    Pr += sr[c] * g_r + si[c] * g_i;  // conj(S_c) g
+                    Pi += sr[c] * g_i - si[c] * g_r;
+                    const float wgr = w * g_r, wgi = w * g_i;
+                    gr[c] += fre * wgr - fim * wgi;
+                    gi[c] += fre * wgi + fim * wgr;
+                }
+                Qr += w * Pr, Qi += w * Pi;
+                const float wt = -tau * w;
+                Rr += wt * Pr, Ri += wt * Pi;
+                const float wd = kd * e;
+                Xr += wd * Pr, Xi += wd * Pi;
+            }
+        }
+        lx += cp * Qr - sp * Qi;
+        ly += cp * Qi + sp * Qr;
+        acc.v[A_R2] += m0.x * (cp * Rr - sp * Ri) + m0.y * (cp * Ri + sp * Rr);
+        const float a2 = HB_TWO_PI * (float)shift, b2 = -(float)ng * hz;  // d/dfr: 2 pi i shift Q + (-ng hz) X
+        const float Fr = -a2 * Qi + b2 * Xr, Fi = a2 * Qr + b2 * Xi;
+        dcyc += m0.x * (cp * Fr - sp * Fi) + m0.y * (cp * Fi + sp * Fr);
+    }
+    const float gb = dcyc * (float)q.dt64;  // dL/dbz
+    acc.v[A_PX] += gb * q.g_x;
+    acc.v[A_PY] += gb * q.g_y;
+    acc.v[A_PZ] += gb * q.g_z;
+    acc.v[A_DB0] += gb;
+}
+
+// NU: the table has NUFFT runs (their reverse raises the register pressure of the unrolled block loop, so the other
+// tables take the variant without it).
+template <int NC, bool SEG, bool NU>
 __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8 ? 2 : 1))) void adjoint_kernel(
     hb_case cs, hb_adjoint ad, int s0, int s1) {
     __shared__ float segbuf[kWavesA][8][8];
@@ -154,6 +260,16 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
                 apply_map(cs.maps + (size_t)q.pad[1] * 12 * np + j, np, q.w_rf_x, q.w_rf_y, m);
                 if (q.flags & HB_PULSE_SPOIL) m.mx = m.my = 0.0f;
                 const int e = min(s + q.pad[2], s1);
+                for (int t = s0 + (((s + 1 - s0) + 7) & ~7); t < e; t += 8) {
+                    float* r = ad.r8 + (size_t)((t - s0) >> 3) * 3 * np;
+                    r[j] = m.mx, r[np + j] = m.my, r[2 * np + j] = m.mz + m.lz;
+                }
+                s = e - 1;
+                continue;
+            }
+            if (NU && (q.flags & HB_NUFFT)) {  // a readout run: its jump (its samples add nothing to the state)
+                run_jump(q, p, free_cycles64(pos64(p), p, q), expm1x(-((float)q.pad[3] * q.dt) * p.r1), m);
+                const int e = min(s + q.pad[3], s1);
                 for (int t = s0 + (((s + 1 - s0) + 7) & ~7); t < e; t += 8) {
                     float* r = ad.r8 + (size_t)((t - s0) >> 3) * 3 * np;
                     r[j] = m.mx, r[np + j] = m.my, r[2 * np + j] = m.mz + m.lz;
@@ -232,6 +348,11 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
                         if (q.flags & HB_PULSE_SPOIL) m.mx = m.my = 0.0f;
                         continue;
                     }
+                    if (NU && (q.flags & HB_NUFFT)) {
+                        run_jump(q, p, free_cycles64(pos64(p), p, q), expm1x(-((float)q.pad[3] * q.dt) * p.r1), m);
+                        after[i] = {m.mx, m.my};
+                        continue;
+                    }
                     const int f = live_flags(q.flags, free_ok, rf_ok, HB_SAME_FREE, HB_SAME_RF_DT);
                     step(q, f, p, b1sq, m, k);
                     if (f & HB_RF) rf_ok = true; else free_ok = true;
@@ -249,6 +370,12 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
                 const int s = sb + i;
                 const hb_segment q = cs.segs[s];
                 if (q.flags & HB_PULSE_BODY) continue;  // the identity, no gradients
+                if constexpr (NU) {
+                    if (q.flags & HB_NUFFT) {
+                        nufft_reverse<NC>(q, ad, p, before[i], lx, ly, lz, acc, sr, si, gr, gi);
+                        continue;
+                    }
+                }
                 if (q.flags & HB_PULSE) {
                     if (q.flags & HB_PULSE_SPOIL) lx = ly = 0.0f;
                     pulse_reverse<SEG>(q, cs, ad, j, before[i], after[i], lx, ly, lz, sg);
@@ -359,10 +486,15 @@ __global__ __launch_bounds__(256) void seg_final_kernel(const double* __restrict
 template <int NC>
 static hipError_t launch_adjoint(const hb_case* cs, const hb_adjoint* ad, int s0, int s1, hipStream_t st) {
     const dim3 grid(cs->n_pad / kBlockA), block(kBlockA);
-    if (ad->seg_grads)
-        hipLaunchKernelGGL((adjoint_kernel<NC, true>), grid, block, 0, st, *cs, *ad, s0, s1);
+    const bool nu = ad->nu_params != nullptr;
+    if (ad->seg_grads && nu)
+        hipLaunchKernelGGL((adjoint_kernel<NC, true, true>), grid, block, 0, st, *cs, *ad, s0, s1);
+    else if (ad->seg_grads)
+        hipLaunchKernelGGL((adjoint_kernel<NC, true, false>), grid, block, 0, st, *cs, *ad, s0, s1);
+    else if (nu)
+        hipLaunchKernelGGL((adjoint_kernel<NC, false, true>), grid, block, 0, st, *cs, *ad, s0, s1);
     else
-        hipLaunchKernelGGL((adjoint_kernel<NC, false>), grid, block, 0, st, *cs, *ad, s0, s1);
+        hipLaunchKernelGGL((adjoint_kernel<NC, false, false>), grid, block, 0, st, *cs, *ad, s0, s1);
     return hipGetLastError();
 }
 
@@ -372,6 +504,7 @@ extern "C" int hb_adjoint_interval(const hb_case* cs, const hb_adjoint* ad, int3
     using namespace hb;
     if (!cs || !ad || cs->n_pad <= 0 || cs->n_pad % kBlockA) return hipErrorInvalidValue;
     if (s0 < 0 || s1 < s0 || s1 > cs->n_segments || s1 - s0 > ad->k_max) return hipErrorInvalidValue;
+    if (ad->nu_params && (!ad->nu_win || (ad->nc > 0 && !ad->nu_grid))) return hipErrorInvalidValue;
     if (s1 == s0) return hipSuccess;
     hipStream_t st = (hipStream_t)stream;
     switch (ad->nc) {

@@ -172,6 +172,26 @@ __device__ __forceinline__ void store_state(float* __restrict__ st, int n_pad, i
     st[3 * n_pad + j] = 0.0f, st[4 * n_pad + j] = 0.0f, st[5 * n_pad + j] = m.lz;
 }
 
+// The jump over a readout run of L = q.pad[3] samples (q its first segment): one exact free step of L dt, phase
+// L cyc (cyc the segment's float64 phase in cycles), Mxy zeroed instead when the run ends spoiled or its Mxy is never
+// read; Mz relaxes by e1 = expm1(-L dt R1).
+__device__ __forceinline__ void run_jump(const hb_segment& q, const Spin& p, double cyc, float e1, State& m) {
+    if (q.flags & (HB_XY_DEAD | HB_PULSE_SPOIL)) {
+        m.mx = m.my = 0.0f;
+    } else {
+        const float T = (float)q.pad[3] * q.dt;
+        double cl = cyc * (double)q.pad[3];
+        cl -= rint(cl);
+        float s2, c2;
+        sincospi(-2.0f * (float)cl, &s2, &c2);
+        const float e2 = expm1x(-T * p.r2);
+        const float nx = c2 * m.mx - s2 * m.my, ny = s2 * m.mx + c2 * m.my;
+        m.mx = nx + nx * e2;
+        m.my = ny + ny * e2;
+    }
+    two_sum(m.mz, m.lz, (m.mz - p.pd) * e1);
+}
+
 // A reused pulse's occurrence with RF phase psi (c, sn its cosine and sine): m += Rz(psi) (D Rz(-psi) m + v), U the
 // spin's map (rows n_pad apart: D[i][k] at 3 i + k, then v), read per occurrence so that no registers hold it.
 __device__ __forceinline__ void apply_map(const float* __restrict__ U, int np, float c, float sn, State& m) {

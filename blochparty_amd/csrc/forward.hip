@@ -236,21 +236,8 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
         pending = 0;
     }
     // the jump over the run: phase L cyc, relaxation over L dt (Mz only when the run's Mxy is never read)
-    const float T = (float)L * q.dt;
-    if (q.flags & (HB_XY_DEAD | HB_PULSE_SPOIL)) {
-        m.mx = m.my = 0.0f;
-    } else {
-        double cl = cyc * (double)L;
-        cl -= rint(cl);
-        float s2, c2;
-        sincospi(-2.0f * (float)cl, &s2, &c2);
-        const float e2 = expm1x(-T * p.r2);
-        const float nx = c2 * m.mx - s2 * m.my, ny = s2 * m.mx + c2 * m.my;
-        m.mx = nx + nx * e2;
-        m.my = ny + ny * e2;
-    }
-    if (!(q.flags & HB_SAME_RUN)) e1_run = expm1x(-T * p.r1);
-    two_sum(m.mz, m.lz, (m.mz - p.pd) * e1_run);
+    if (!(q.flags & HB_SAME_RUN)) e1_run = expm1x(-((float)L * q.dt) * p.r1);
+    run_jump(q, p, cyc, e1_run, m);
 }
 
 // NT: coil tiles (0: any, the sensitivities read from memory per flush); NUT: NUFFT runs with up to NUT decay terms
@@ -299,12 +286,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 
             em[t] = t < terms ? __builtin_amdgcn_exp2f(-(nu.params[6 + t] * 1.4426950408889634f) * p.r2) : 0.0f;
     }
     int nb = 0, row = 0, par = 0;
-    // a dispatch that starts inside a reused pulse's occurrence (checkpoint intervals do not avoid them): the
-    // occurrence's map was applied at its first segment, in the dispatch before
+    // a dispatch that starts inside a reused pulse's occurrence or a NUFFT run (checkpoint intervals do not avoid
+    // them): the record was applied at its first segment, in the dispatch before
     int s_start = s0;
     if (cs.segs[s0].flags & HB_PULSE_BODY) {
         const int head = cs.segs[s0].pad[3];
-        s_start = head + cs.segs[head].pad[2];
+        const hb_segment& hq = cs.segs[head];
+        s_start = head + ((hq.flags & HB_PULSE) ? hq.pad[2] : hq.pad[3]);
     }
     for (int s = s_start; s < s1; ++s) {
         const hb_segment q = cs.segs[s];
@@ -499,4 +487,4 @@ extern "C" int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t 
 
 extern "C" const char* hb_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 
-extern "C" int hb_abi_version(void) { return 18; }
+extern "C" int hb_abi_version(void) { return 19; }

@@ -126,7 +126,42 @@ __global__ __launch_bounds__(256) void nufft_signal_kernel(const double* __restr
     }
 }
 
+// One lane per (cell u, coil c) of run blockIdx.z, term blockIdx.y: the sum over the run's samples in float64.
+__global__ __launch_bounds__(256) void nufft_adjoint_grid_kernel(const float* __restrict__ h, int nc,
+                                                                 const int* __restrict__ row_of, int n, int ng,
+                                                                 int terms, const double* __restrict__ tw,
+                                                                 const double* __restrict__ lw,
+                                                                 float* __restrict__ G) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ng * nc) return;
+    const int u = k / nc, c = k % nc, m = blockIdx.y, r = blockIdx.z;
+    const float* hr = h + (size_t)row_of[r] * nc * 2 + 2 * c;
+    const double* lm = lw + (size_t)m * n;
+    double re = 0.0, im = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double a = lm[i], tr = tw[((size_t)i * ng + u) * 2], ti = tw[((size_t)i * ng + u) * 2 + 1];
+        const double xr = hr[(size_t)i * nc * 2], xi = hr[(size_t)i * nc * 2 + 1];
+        re += a * (tr * xr + ti * xi);  // conj(tw) h
+        im += a * (tr * xi - ti * xr);
+    }
+    float* out = G + (((size_t)r * terms + m) * ng + u) * 2 * nc + 2 * c;
+    out[0] = (float)re;
+    out[1] = (float)im;
+}
+
 }  // namespace hb
+
+extern "C" int hb_nufft_adjoint_grid(const float* h, int32_t nc, const int32_t* row_of, int32_t n_runs, int32_t n,
+                                     int32_t ng, int32_t terms, const double* tw, const double* lw, float* G,
+                                     void* stream) {
+    if (!h || !row_of || !tw || !lw || !G || nc < 1 || n_runs < 0 || n_runs > 65535 || n < 1 || ng < 2 ||
+        terms < 1 || terms > HB_NUFFT_MAX_TERMS)
+        return hipErrorInvalidValue;
+    if (n_runs == 0) return hipSuccess;
+    hipLaunchKernelGGL(hb::nufft_adjoint_grid_kernel, dim3((ng * nc + 255) / 256, terms, n_runs), dim3(256), 0,
+                       (hipStream_t)stream, h, nc, row_of, n, ng, terms, tw, lw, G);
+    return hipGetLastError();
+}
 
 extern "C" int hb_nufft_grid(const float* part, int32_t n_blocks, int32_t runs_max, int32_t cols, int32_t n_tiles,
                              int32_t n_slots, const int32_t* run_of, const int32_t* kind_of, int32_t ng,

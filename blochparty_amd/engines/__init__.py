@@ -85,8 +85,8 @@ def simulate(
     (NotImplementedError) unless None. seed: only meaningful for diffusion, which is refused.
     options: ``chunk`` (segments per dispatch, default 32768), ``group`` (256), ``mma`` (the matrix cores always
     contract the coils here; any value is accepted), ``readouts`` ("direct", or "nufft": readouts of >= 16 equal
-    samples as a type-1 NUFFT with ``nufft_width`` kernel cells (6) and ``nufft_terms`` decay terms (3); forward
-    only), ``pulses`` ("step"; "reuse": a pulse that recurs up to a global RF phase is composed
+    samples as a type-1 NUFFT with ``nufft_width`` kernel cells (6) and ``nufft_terms`` decay terms (3); under
+    autograd the reverse sweep takes each run as the transpose of its NUFFT), ``pulses`` ("step"; "reuse": a pulse that recurs up to a global RF phase is composed
     once per spin into its affine map, in float64, and applied at every occurrence; "surrogate": those maps from
     the learned RF surrogate, trained on the GPU on first use, with the exact composition as the fallback outside its
     trained box and as the judge of its error on every run, reported in ``SimResult.surrogate``; under autograd both
@@ -117,11 +117,9 @@ def simulate(
     grad = torch.is_grad_enabled() and requires_grad(phantom, seq, coils, m_init)
     check(name, features(phantom, seq, m_init, scanner, maps), grad=grad)
     if grad:
-        if nufft is not None:
-            raise ValueError("readouts='nufft' is forward only")
         from .adjoint import simulate_grad
 
-        return simulate_grad(phantom, seq, coils, m_init, checkpoint_every, pulses)
+        return simulate_grad(phantom, seq, coils, m_init, checkpoint_every, pulses, nufft)
     from .driver import forward
 
     with torch.no_grad():

@@ -43,6 +43,7 @@ class Opts(NamedTuple):
     spoil: list
     every: int
     pulses: str = "step"  # "reuse" / "surrogate": reused pulses as maps in the forward and the reverse sweep
+    nufft: tuple | None = None  # (width, terms): readout runs as NUFFTs in the forward and the reverse sweep
 
 
 class GPUSimulate(torch.autograd.Function):
@@ -54,7 +55,8 @@ class GPUSimulate(torch.autograd.Function):
         sq = Sequence(d(dt), d(rf), d(grad), d(adc_phase), opts.is_rf, opts.adc, opts.spoil, {}, d(b0_offset))
         coils = torch.complex(coils_ri[..., 0].detach(), coils_ri[..., 1].detach())
         case = driver.prepare_case(ph, sq, coils, d(m0), chunk=opts.every, aligned=True,
-                                   reuse=opts.pulses == "reuse", surrogate=opts.pulses == "surrogate")
+                                   reuse=opts.pulses == "reuse", surrogate=opts.pulses == "surrogate",
+                                   nufft=opts.nufft)
         ckpts = driver.run_forward(case, keep=True)
         hip.HIP.synchronize()
         res = driver.read_result(case)
@@ -93,10 +95,18 @@ class GPUSimulate(torch.autograd.Function):
             gr, gi = g_sig[..., 0].double().cpu(), g_sig[..., 1].double().cpu()
             h[:, :C, 0] = gr * torch.cos(psi) - gi * torch.sin(psi)
             h[:, :C, 1] = gr * torch.sin(psi) + gi * torch.cos(psi)
-        sens = adjoint_sens(ctx.coils, n_pad, nc)
+        # with NUFFT readouts the device holds the spins sorted (case.perm: device k <- caller spin perm[k])
+        perm = None if case.perm is None else torch.from_numpy(np.ascontiguousarray(case.perm))
+        inv = None
+        if perm is not None:
+            inv = torch.empty_like(perm)
+            inv[perm] = torch.arange(len(perm))
+        to_caller = (lambda t: t) if inv is None else (lambda t: t[:, inv])
+        sens = adjoint_sens(ctx.coils if perm is None else ctx.coils[perm], n_pad, nc)
         lam = torch.zeros(3, n_pad, dtype=torch.float32)
         if g_m is not None:
-            lam[:, :n] = g_m.detach().float().cpu().T
+            gm = g_m.detach().float().cpu()
+            lam[:, :n] = (gm if perm is None else gm[perm]).T
         need_seg = any(need[i] for i in (6, 7, 8, 12))
         # 2: only rf's gradients (the sweep forms segment gradients only where an RF record is)
         seg_mode = 0 if not need_seg else 1 if any(need[i] for i in (6, 8, 12)) else 2
@@ -104,8 +114,8 @@ class GPUSimulate(torch.autograd.Function):
         gs, gc, seg, lam_d = run_backward(case, ckpts, every, h.float().to(dev).contiguous(), lam.to(dev).contiguous(),
                                           sens.to(dev).contiguous(), nc, seg_mode, need_coil)
         hip.HIP.synchronize()
-        GS = gs.cpu()[:, :n]
-        lam = lam_d.cpu().double()[:, :n]
+        GS = to_caller(gs.cpu()[:, :n])
+        lam = to_caller(lam_d.cpu().double()[:, :n])
         out = [None] * 13
         if need[0]:
             out[0] = GS[0:3].T.contiguous()
@@ -118,7 +128,7 @@ class GPUSimulate(torch.autograd.Function):
         if need[4]:
             out[4] = GS[3]
         if need_coil:
-            G = gc.cpu()[: 2 * C, :n]
+            G = to_caller(gc.cpu()[: 2 * C, :n])
             out[5] = torch.stack([G[0::2].T, G[1::2].T], -1)
         if need_seg:
             sgr = seg.cpu().view(S, 8)
@@ -166,11 +176,18 @@ def run_backward(case: driver.DeviceCase, ckpts: list, every: int, h: torch.Tens
     red = torch.empty(max(1, -(-n_blocks // SEG_CHUNK) * every * 8 if need_seg else 1), dtype=torch.float64,
                       device=dev)
     ptr = lambda t: None if t is None else t.data_ptr()
+    nu, grid = case.nu, None
+    if nu is not None and nc:  # the runs' adjoint grids from dL/dsignal (the transpose of hb_nufft_signal)
+        pl = nu["plan"]
+        grid = torch.empty(len(pl.runs) * pl.terms * pl.ng * 2 * nc, dtype=torch.float32, device=dev)
+        lib.nufft_adjoint_grid(h.data_ptr(), nc, nu["row_of"].data_ptr(), len(pl.runs), pl.n, pl.ng, pl.terms,
+                               nu["nu_tw"].data_ptr(), nu["nu_lw"].data_ptr(), grid.data_ptr())
     for c in reversed(range(len(ckpts))):
         s0, s1 = case.starts[c], case.ends[c]
         ad = hip.HbAdjoint(ckpts[c].data_ptr(), r8.data_ptr(), lam.data_ptr(), gs.data_ptr(), gc.data_ptr(),
-                           gseg.data_ptr(), h.data_ptr(), sens.data_ptr(), ptr(maps_d), ptr(seeds), nc,
-                           int(need_seg), int(need_coil), every)
+                           gseg.data_ptr(), h.data_ptr(), sens.data_ptr(), ptr(maps_d), ptr(seeds),
+                           None if nu is None else nu["nu"].data_ptr(), None if nu is None else nu["nu_win"].data_ptr(),
+                           ptr(grid), nc, int(need_seg), int(need_coil), every)
         lib.adjoint_interval(ctypes.byref(cs), ctypes.byref(ad), s0, s1)
         if need_seg:
             lib.seg_reduce(gseg.data_ptr(), n_blocks, every, s0, s1 - s0, red.data_ptr(), seg.data_ptr())
@@ -234,10 +251,12 @@ def adjoint_sens(coils: torch.Tensor, n_pad: int, nc: int) -> torch.Tensor:
     return sens
 
 
-def simulate_grad(phantom: Phantom, seq: Sequence, coils, m_init, checkpoint_every, pulses: str = "step") -> SimResult:
+def simulate_grad(phantom: Phantom, seq: Sequence, coils, m_init, checkpoint_every, pulses: str = "step",
+                  nufft=None) -> SimResult:
     """simulate(engine="gpu") with gradients: the inputs as tensors of ``GPUSimulate`` (adjoint.py:264-323).
-    ``pulses``: "step", or "reuse" / "surrogate" (reused pulses as maps in both sweeps; the reference's reused pulses
-    are forward only, driver.py:221-225)."""
+    ``pulses``: "step", or "reuse" / "surrogate" (reused pulses as maps in both sweeps); ``nufft``: None or (width,
+    terms), readout runs as NUFFTs in both sweeps (the reference's reused pulses and NUFFT readouts are forward only,
+    driver.py:221-225)."""
     n, pd = len(phantom), phantom.pd
     if coils is None:
         coils = torch.ones(n, 1, dtype=torch.complex64 if pd.dtype == torch.float32 else torch.complex128)
@@ -248,13 +267,14 @@ def simulate_grad(phantom: Phantom, seq: Sequence, coils, m_init, checkpoint_eve
     every = checkpoint_every or default_checkpoint_every(len(seq))
     if every % 64:
         raise ValueError(f"checkpoint_every must be a multiple of 64, got {every}")
-    if pulses != "step":
+    if pulses != "step" or nufft is not None:
         fixed = [n for n, t in (("dt", seq.dt), ("grad", seq.grad), ("b0_offset", seq.b0_offset))
                  if t is not None and t.requires_grad]
         if fixed:
-            raise ValueError(f"pulses={pulses!r} differentiates spin parameters, coils and each reused pulse's "
-                             f"amplitude and phase, not {', '.join(fixed)}: use pulses='step'")
-    opts = Opts(list(seq.is_rf), list(seq.adc), list(seq.spoil), every, pulses)
+            what = f"pulses={pulses!r}" if pulses != "step" else "readouts='nufft'"
+            raise ValueError(f"{what} differentiates spin parameters, coils, m_init and RF amplitudes and phases, not "
+                             f"{', '.join(fixed)}: use pulses='step' and readouts='direct'")
+    opts = Opts(list(seq.is_rf), list(seq.adc), list(seq.spoil), every, pulses, nufft)
     sig, m = GPUSimulate.apply(opts, phantom.pos, pd, phantom.t1, phantom.t2, phantom.db0, torch.view_as_real(coils),
                                seq.dt, seq.rf, seq.grad, seq.adc_phase, m0, phantom.b1, seq.b0_offset)
     return SimResult(torch.view_as_complex(sig.contiguous()).T, m)

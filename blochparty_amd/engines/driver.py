@@ -129,10 +129,14 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
                     and rec["f"][cur, 0] == rec["f"][prev, 0]):
                 flags[cur] |= SAME_RUN
     mark_trains(flags, rec, is_rf, adc, spoil, chunk_id, stepped)
-    for _, k, occ in pulses:  # an occurrence's other segments: the identity (its map is applied at its first)
-        for a, _ in occ:
-            flags[a + 1 : a + k] = PULSE_BODY
-            rec["pad"][a + 1 : a + k, 3] = a
+    # an occurrence's or a NUFFT run's other segments: the identity (the record is applied at its first segment)
+    spans = [(a, k) for _, k, occ in pulses for a, _ in occ] + (list(nufft.runs) if nufft is not None else [])
+    for a, k in spans:
+        flags[a + 1 : a + k] = PULSE_BODY
+        rec["pad"][a + 1 : a + k, 3] = a
+    if nufft is not None:
+        for r, (a, _) in enumerate(nufft.runs):
+            rec["pad"][a, 4] = r  # the run's index (the adjoint's grid)
     rec["flags"] = flags
     w = -2 * math.pi * dt  # the rotation-vector factors, formed in float64
     rec["w"] = np.stack([w, w * rf[:, 0], w * rf[:, 1], w * g[:, 0], w * g[:, 1], w * g[:, 2], w * b0,
@@ -385,8 +389,6 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
         coils = torch.ones(n, 1, dtype=torch.complex128)
     plan, order = None, None
     if nufft is not None:
-        if aligned:
-            raise ValueError("readouts='nufft' is forward only")
         plan = nu_mod.make_plan(seq, *nufft)
         if plan is not None:
             pos_in = phantom.pos.detach().double().cpu()

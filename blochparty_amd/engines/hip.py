@@ -15,7 +15,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent.parent / "libblochparty_amd.so"
-ABI_VERSION = 18
+ABI_VERSION = 19
 MAX_COILS = 128  # 16 tiles of 8 coils (the forward kernel's LDS budget)
 
 
@@ -56,6 +56,9 @@ class HbAdjoint(ctypes.Structure):
         ("sens", ctypes.c_void_p),
         ("maps_d", ctypes.c_void_p),
         ("seeds", ctypes.c_void_p),
+        ("nu_params", ctypes.c_void_p),
+        ("nu_win", ctypes.c_void_p),
+        ("nu_grid", ctypes.c_void_p),
         ("nc", ctypes.c_int32),
         ("seg_grads", ctypes.c_int32),
         ("coil_grads", ctypes.c_int32),
@@ -86,6 +89,7 @@ SIGNATURES = {
     "hb_signal_reduce": [_P, _I, _I, _I, _I, _P, _I, _P, _P, _P, _V],
     "hb_nufft_grid": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _P, _P, _P, _P, _V],
     "hb_nufft_signal": [_P, _I, _P, _P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _V],
+    "hb_nufft_adjoint_grid": [_P, _I, _P, _I, _I, _I, _I, _P, _P, _P, _V],
     "hb_adjoint_interval": [ctypes.POINTER(HbCase), ctypes.POINTER(HbAdjoint), _I, _I, _V],
     "hb_seg_reduce": [_P, _I, _I, _I, _I, _P, _P, _V],
     "hb_pulse_maps": [ctypes.POINTER(HbCase), _P, _P, _I, _P, _I, _P, _V],

@@ -130,6 +130,13 @@ int hb_nufft_grid(const float* part, int32_t n_blocks, int32_t runs_max, int32_t
                   int32_t n_slots, const int32_t* run_of, const int32_t* kind_of, int32_t ng, int32_t terms,
                   const int32_t* csr_off, const int32_t* csr, const int32_t* csr_base, double* grid, void* stream);
 
+/* The adjoint of hb_nufft_signal for every run (the reference's NUFFT readouts are forward only): for run r < n_runs,
+ * term m, cell u < ng and coil c < nc, G[((r * terms + m) * ng + u) * 2 nc + 2 c + {re, im}] = sum over samples
+ * i < n of lw[m][i] conj(tw[i][u]) h[(row_of[r] + i) * nc + c] (h: (n_adc, nc, 2) dL/dsignal turned by e^{+i psi},
+ * as hb_adjoint's), float64 sums stored as float32. */
+int hb_nufft_adjoint_grid(const float* h, int32_t nc, const int32_t* row_of, int32_t n_runs, int32_t n, int32_t ng,
+                          int32_t terms, const double* tw, const double* lw, float* G, void* stream);
+
 /* The NUFFT readouts' samples from the grids (replaces driver.py:703-725's FFT and recombination): for slot r,
  * sample i < n and coil c, sig[(row_of[run] + i) * n_coils + c] = exp(-i psi) sum_m lw[m][i] sum_u
  * grid[r][m][u][c] tw[i][u], float64. tw: (n, ng, 2) e^{-2 pi i j u / ng} at the modes j = i - n / 2; lw: (terms, n)
@@ -236,7 +243,14 @@ int hb_surrogate_maps(const hb_case* cs, const hb_mlp* net, const hb_sur_pulse* 
  * lambda m^T (rows 0..8, as D's entries) and lambda (rows 9..11, as v's) with lambda after the map and m before it
  * (their parameter gradients follow by hb_pulse_map_grads); with seg_grads, the occurrence's first segment's rf_x,
  * rf_y slots get the sums over spins of dL/d(ln amplitude) (from maps_d, hb_pulse_map_dlnb1) and dL/d(RF phase) of
- * the occurrence, its other slots 0. maps_d and seeds may be NULL when the table has no HB_PULSE record. */
+ * the occurrence, its other slots 0. maps_d and seeds may be NULL when the table has no HB_PULSE record.
+ * NUFFT readout runs (HB_NUFFT records: pad[2] the run's kind, pad[3] its length L, pad[4] its index r among the
+ * runs; its other segments HB_PULSE_BODY): the run's jump reversed, then the transpose of its spreading: each spin
+ * interpolates nu_grid[((r * terms + m) * ng + u) * 2 nc + 2 c + {re, im}] (hb_nufft_adjoint_grid) over its
+ * workgroup's window of cells (nu_win, as hb_nufft's win) with the forward's kernel and decay terms (nu_params, as
+ * hb_nufft's params), for lambda, the coil gradients and the R2, position and db0 gradients; no segment gradients
+ * (the caller asks for none of a run's dt, gradient or b0_offset). nu_params, nu_win, nu_grid: NULL without runs;
+ * nu_params and nu_win (and nu_grid when nc > 0) must be set when the table has HB_NUFFT records. */
 typedef struct hb_adjoint {
     const float* ckpt;
     float* r8;
@@ -248,6 +262,9 @@ typedef struct hb_adjoint {
     const float* sens;
     const float* maps_d;
     double* seeds;
+    const float* nu_params;
+    const int32_t* nu_win;
+    const float* nu_grid;
     int32_t nc, seg_grads, coil_grads, k_max;
 } hb_adjoint;
 

@@ -56,14 +56,14 @@ def test_nufft_with_reused_pulses_and_b0_offset(gpu):
     assert rel(nu.signal, direct.signal) < 1e-4
 
 
-def test_nufft_is_forward_only_and_falls_back_without_runs(gpu):
+def test_nufft_refuses_timing_gradients_and_falls_back_without_runs(gpu):
     ph, seq, coils = bp.Phantom(pos=torch.zeros(4, 3, dtype=torch.float64), pd=torch.ones(4, dtype=torch.float64),
                                 t1=torch.ones(4, dtype=torch.float64), t2=torch.full((4,), 0.1, dtype=torch.float64),
                                 db0=torch.zeros(4, dtype=torch.float64)), W.first_trs(W.gre3d_sequence(32), 2), None
-    ph.t1.requires_grad_(True)
-    with pytest.raises(ValueError, match="forward only"):
+    seq.dt.requires_grad_(True)
+    with pytest.raises(ValueError, match="readouts='direct'"):
         bp.simulate(ph, seq, engine="gpu", readouts="nufft")
-    ph.t1.requires_grad_(False)
+    seq.dt.requires_grad_(False)
     short = W.first_trs(W.gre3d_sequence(16), 2)  # readouts too short: the per-sample path, same result
     case = driver.prepare_case(ph, short, None, nufft=(6, 3))
     assert case.nu is None
