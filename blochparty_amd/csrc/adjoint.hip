@@ -79,6 +79,17 @@ __device__ __forceinline__ void seg_store(const f4& d, float (*segbuf)[8], int l
     }
 }
 
+// r8 after a record at s whose segments [s, e) it covers (a reused pulse's occurrence, a NUFFT run): the reverse sweep
+// skips every 8-block wholly inside them, so only the block that starts inside and ends past e needs its state.
+__device__ __forceinline__ void r8_after_span(float* __restrict__ r8, int np, int j, int s0, int s1, int s, int e,
+                                              const State& m) {
+    const int tl = s0 + ((e - 1 - s0) & ~7);  // the last block start before e
+    if (tl > s && e < s1 && e - tl < 8) {
+        float* r = r8 + (size_t)((tl - s0) >> 3) * 3 * np;
+        r[j] = m.mx, r[np + j] = m.my, r[2 * np + j] = m.mz + m.lz;
+    }
+}
+
 // Effective flags: cache hints honoured only while the cache holds the factors they refer to.
 __device__ __forceinline__ int live_flags(int f, bool free_ok, bool rf_ok, int same_free, int same_rf) {
     if (!free_ok) f &= ~(same_free | HB_SAME_FREE_DT);
@@ -260,20 +271,14 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
                 apply_map(cs.maps + (size_t)q.pad[1] * 12 * np + j, np, q.w_rf_x, q.w_rf_y, m);
                 if (q.flags & HB_PULSE_SPOIL) m.mx = m.my = 0.0f;
                 const int e = min(s + q.pad[2], s1);
-                for (int t = s0 + (((s + 1 - s0) + 7) & ~7); t < e; t += 8) {
-                    float* r = ad.r8 + (size_t)((t - s0) >> 3) * 3 * np;
-                    r[j] = m.mx, r[np + j] = m.my, r[2 * np + j] = m.mz + m.lz;
-                }
+                r8_after_span(ad.r8, np, j, s0, s1, s, e, m);
                 s = e - 1;
                 continue;
             }
             if (NU && (q.flags & HB_NUFFT)) {  // a readout run: its jump (its samples add nothing to the state)
                 run_jump(q, p, free_cycles64(pos64(p), p, q), expm1x(-((float)q.pad[3] * q.dt) * p.r1), m);
                 const int e = min(s + q.pad[3], s1);
-                for (int t = s0 + (((s + 1 - s0) + 7) & ~7); t < e; t += 8) {
-                    float* r = ad.r8 + (size_t)((t - s0) >> 3) * 3 * np;
-                    r[j] = m.mx, r[np + j] = m.my, r[2 * np + j] = m.mz + m.lz;
-                }
+                r8_after_span(ad.r8, np, j, s0, s1, s, e, m);
                 s = e - 1;
                 continue;
             }
@@ -314,6 +319,9 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
     for (int a = 0; a < 8; ++a) acc.v[a] = 0.0f;
     RevCache rc = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     bool rfree_ok = false, rrf_ok = false;  // reverse caches (hints: same as the next segment of the kind)
+    // a record was reversed since the last flush (else the accumulators hold exact zeros and the flush is skipped:
+    // the float64 sums are the same bit for bit, and spans inside pulses and runs cost no HBM traffic)
+    bool dirty = false;
     const int n8 = (s1 - s0 + 7) >> 3;
     for (int b = n8 - 1; b >= 0; --b) {
         const int sb = s0 + 8 * b;
@@ -324,6 +332,7 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
         if ((qf.flags & ql.flags & HB_PULSE_BODY) && qf.pad[3] == ql.pad[3]) {
             if (SEG && tid < 8 * nseg) ad.gseg[((size_t)blockIdx.x * ad.k_max + (sb - s0)) * 8 + tid] = 0.0f;
         } else {
+        dirty = true;
         // replay the block with its states in registers: before[i] before segment sb + i, after[i] after its step
         Stash before[8];
         StashXY after[8];
@@ -428,7 +437,8 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
             __syncthreads();
         }
         }
-        if ((sb & 63) == 0) {  // every 64 segments: into the float64 accumulators
+        if ((sb & 63) == 0 && dirty) {  // every 64 segments: into the float64 accumulators
+            dirty = false;
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
                 ad.gs[(size_t)a * np + j] += (double)acc.v[a];
