@@ -260,13 +260,19 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
         State m = load_state(ad.ckpt, np, j);
         Cache k = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         bool free_ok = false, rf_ok = false;
-        for (int s = s0; s < s1; ++s) {
+        int s_start = s0;
+        if (cs.segs[s0].flags & HB_PULSE_BODY) {  // inside an occurrence or a run begun in an earlier interval
+            const hb_segment& hq = cs.segs[cs.segs[s0].pad[3]];
+            const int e = min(cs.segs[s0].pad[3] + ((hq.flags & HB_PULSE) ? hq.pad[2] : hq.pad[3]), s1);
+            r8_after_span(ad.r8, np, j, s0, s1, s0 - 1, e, m);
+            s_start = e;
+        }
+        for (int s = s_start; s < s1; ++s) {
             if (((s - s0) & 7) == 0) {
                 float* r = ad.r8 + (size_t)((s - s0) >> 3) * 3 * np;
                 r[j] = m.mx, r[np + j] = m.my, r[2 * np + j] = m.mz + m.lz;
             }
             const hb_segment q = cs.segs[s];
-            if (q.flags & HB_PULSE_BODY) continue;  // inside an occurrence begun in an earlier interval
             if (q.flags & HB_PULSE) {  // a reused pulse: its map, then the state stands over the occurrence
                 apply_map(cs.maps + (size_t)q.pad[1] * 12 * np + j, np, q.w_rf_x, q.w_rf_y, m);
                 if (q.flags & HB_PULSE_SPOIL) m.mx = m.my = 0.0f;
@@ -329,8 +335,18 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
         // a block inside one reused pulse's occurrence is the identity: only its (zero) segment gradients
         const hb_segment& qf = cs.segs[sb];
         const hb_segment& ql = cs.segs[sb + nseg - 1];
+        bool flush_point = (sb & 63) == 0;
         if ((qf.flags & ql.flags & HB_PULSE_BODY) && qf.pad[3] == ql.pad[3]) {
-            if (SEG && tid < 8 * nseg) ad.gseg[((size_t)blockIdx.x * ad.k_max + (sb - s0)) * 8 + tid] = 0.0f;
+            // so is every block from the first one after the record's head down to this one: all skipped at once
+            const int head = qf.pad[3];
+            const int b_min = head < s0 ? 0 : (head + 1 - s0 + 7) >> 3;
+            const int lo = s0 + 8 * b_min;
+            if (SEG) {
+                float* gz = ad.gseg + ((size_t)blockIdx.x * ad.k_max + (lo - s0)) * 8;
+                for (int t = tid; t < 8 * (sb + nseg - lo); t += kBlockA) gz[t] = 0.0f;
+            }
+            flush_point = (sb & ~63) >= lo;  // one of the skipped blocks starts at a multiple of 64
+            b = b_min;
         } else {
         dirty = true;
         // replay the block with its states in registers: before[i] before segment sb + i, after[i] after its step
@@ -437,7 +453,7 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
             __syncthreads();
         }
         }
-        if ((sb & 63) == 0 && dirty) {  // every 64 segments: into the float64 accumulators
+        if (flush_point && dirty) {  // every 64 segments: into the float64 accumulators
             dirty = false;
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
