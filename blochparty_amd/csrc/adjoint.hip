@@ -247,7 +247,8 @@ __device__ __forceinline__ void nufft_reverse(const hb_segment& q, const hb_adjo
 template <int NC, bool SEG, bool NU>
 __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8 ? 2 : 1))) void adjoint_kernel(
     hb_case cs, hb_adjoint ad, int s0, int s1) {
-    __shared__ float segbuf[kWavesA][8][8];
+    __shared__ float segbuf[2][kWavesA][8][8];  // alternate buffers: one barrier per workgroup sum
+    int sbuf = 0;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = blockIdx.x * kBlockA + tid;
     const int np = cs.n_pad;
@@ -325,8 +326,8 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
     for (int a = 0; a < 8; ++a) acc.v[a] = 0.0f;
     RevCache rc = {1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     bool rfree_ok = false, rrf_ok = false;  // reverse caches (hints: same as the next segment of the kind)
-    // a record was reversed since the last flush (else the accumulators hold exact zeros and the flush is skipped:
-    // the float64 sums are the same bit for bit, and spans inside pulses and runs cost no HBM traffic)
+    // a record that adds to the accumulators was reversed since the last flush (else they hold exact zeros and the
+    // flush is skipped: the float64 sums are the same bit for bit, and spans of pulses and runs cost no HBM traffic)
     bool dirty = false;
     const int n8 = (s1 - s0 + 7) >> 3;
     for (int b = n8 - 1; b >= 0; --b) {
@@ -348,7 +349,6 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
             flush_point = (sb & ~63) >= lo;  // one of the skipped blocks starts at a multiple of 64
             b = b_min;
         } else {
-        dirty = true;
         // replay the block with its states in registers: before[i] before segment sb + i, after[i] after its step
         Stash before[8];
         StashXY after[8];
@@ -398,6 +398,7 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
                 if constexpr (NU) {
                     if (q.flags & HB_NUFFT) {
                         nufft_reverse<NC>(q, ad, p, before[i], lx, ly, lz, acc, sr, si, gr, gi);
+                        dirty = true;
                         continue;
                     }
                 }
@@ -409,6 +410,7 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
                     continue;
                 }
                 const int f = live_flags(q.flags, rfree_ok, rrf_ok, HB_NEXT_FREE, HB_NEXT_RF_DT);
+                dirty = true;  // a stepped record adds to the accumulators (a pulse's map adds only to the seeds)
                 if (f & HB_SPOIL) lx = ly = 0.0f;
                 if (NC > 0 && (f & HB_ADC)) {
                     const float* hrow = ad.h + (size_t)q.pad[0] * 2 * NC;
@@ -438,19 +440,21 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
         if (SEG && !seg_block) {  // rf_only and no RF record here: zeros, without the workgroup sum
             if (tid < 8 * nseg) ad.gseg[((size_t)blockIdx.x * ad.k_max + (sb - s0)) * 8 + tid] = 0.0f;
         } else if (SEG) {
-            seg_store(segd, segbuf[wave], lane);
+            // the other buffer than the last sum's: a wave may still be reading that one (a wave reaches the next
+            // write to this one only after every wave has passed the next sum's barrier, i.e. done reading)
+            seg_store(segd, segbuf[sbuf][wave], lane);
             __syncthreads();
             if (tid < 8 * nseg) {
                 const int i = tid >> 3, g = tid & 7;
                 float v = 0.0f;
                 if (g < N_SEG_GRADS) {
-                    v = segbuf[0][i][g];
+                    v = segbuf[sbuf][0][i][g];
 #pragma unroll
-                    for (int w = 1; w < kWavesA; ++w) v += segbuf[w][i][g];
+                    for (int w = 1; w < kWavesA; ++w) v += segbuf[sbuf][w][i][g];
                 }
                 ad.gseg[((size_t)blockIdx.x * ad.k_max + (sb - s0 + i)) * 8 + g] = v;
             }
-            __syncthreads();
+            sbuf ^= 1;
         }
         }
         if (flush_point && dirty) {  // every 64 segments: into the float64 accumulators
