@@ -110,3 +110,85 @@ def test_records_and_dispatches_keep_runs_whole():
     for s0, s1 in zip(starts, ends):
         sl = [rec["pad"][h, 1] for h in heads if s0 <= h < s1]
         assert sl == list(range(len(sl)))
+
+
+def test_checkpoint_tables_mark_occurrence_and_run_bodies():
+    """With checkpoint intervals (fixed multiples, which may split a reused pulse or a readout run), an occurrence's
+    and a run's first segment carries the record and the rest are identity bodies naming it; the kernels' start rule
+    (a dispatch starting on a body starts after its record) lands on the record's end."""
+    from blochparty_amd.engines import pulses
+
+    seq = W.first_trs(W.gre3d_sequence(32), 5)
+    plan = nufft.make_plan(seq)
+    reused = pulses.reused_pulses(seq)
+    every = 128
+    starts = list(range(0, len(seq), every))
+    rec = driver.segment_table(seq, starts, reused, runs=False, nufft=plan)
+    fl, pad = rec["flags"], rec["pad"]
+    spans = [(a, k, "pulse") for _, k, occ in reused for a, _ in occ] + [(a, k, "run") for a, k in plan.runs]
+    assert reused and plan.runs
+    covered = np.zeros(len(seq), dtype=bool)
+    for a, k, what in spans:
+        assert fl[a] & (driver.PULSE if what == "pulse" else driver.NUFFT)
+        assert (fl[a + 1 : a + k] == driver.PULSE_BODY).all() and (pad[a + 1 : a + k, 3] == a).all()
+        assert not covered[a : a + k].any()
+        covered[a : a + k] = True
+    for r, (a, k) in enumerate(plan.runs):
+        assert pad[a, 4] == r and pad[a, 3] == k
+    bodies = np.flatnonzero(fl & driver.PULSE_BODY)
+    assert len(bodies) == sum(k - 1 for _, k, _ in spans)
+    split = 0
+    for s0 in starts:
+        if fl[s0] & driver.PULSE_BODY:
+            head = pad[s0, 3]
+            length = pad[head, 2] if fl[head] & driver.PULSE else pad[head, 3]
+            assert head < s0 < head + length and (head, length) in [(a, k) for a, k, _ in spans]
+            split += 1
+    assert split > 0  # the intervals do split records here
+
+
+def test_pulse_rf_projection_and_difference_steps():
+    """The rf gradient of a reused pulse's occurrence is the least-norm one with its two sums: its inner products with
+    the occurrence's rf and with i rf are dL/d(ln amplitude) and dL/d(phase), it lies in their span, the other slots
+    are zero. The central-difference steps turn the pulse by the stated angle and skip axes without a gradient."""
+    import math
+
+    import torch
+
+    from blochparty_amd.engines import adjoint
+
+    rng = np.random.default_rng(4)
+    S = 40
+    rf = rng.normal(size=(S, 2))
+    g = torch.from_numpy(rng.normal(size=(S, 8)))
+    pulses_ = [(5, 10, [(5, 0.0), (25, 1.3)])]
+    c = [float(g[5, 1]), float(g[25, 1])]
+    d = [float(g[5, 2]), float(g[25, 2])]
+    before = g.clone()
+    adjoint.project_pulse_rf(g, rf, pulses_)
+    G = g.numpy()
+    for (a, _), ci, di in zip(pulses_[0][2], c, d):
+        z = rf[a : a + 10]
+        gz = G[a : a + 10, 1:3]
+        assert abs((gz * z).sum() - ci) < 1e-12 * max(1, abs(ci))
+        jz = np.stack([-z[:, 1], z[:, 0]], 1)
+        assert abs((gz * jz).sum() - di) < 1e-12 * max(1, abs(di))
+        # in the span of rf and i rf
+        coef, *_ = np.linalg.lstsq(np.stack([z.ravel(), jz.ravel()], 1), gz.ravel(), rcond=None)
+        assert np.allclose(coef[0] * z + coef[1] * jz, gz, atol=1e-12)
+        assert (G[a : a + 10, [0, 3, 4, 5, 6, 7]] == 0).all()
+    untouched = np.r_[0:5, 15:25, 35:40]
+    assert torch.equal(g[untouched], before[untouched])
+    # steps: a pulse with a z gradient only, T = 1 ms, angle pi / 2 (rf in Hz: angle = 2 pi sum dt |rf|)
+    K = 100
+    table = np.zeros((K, 8))
+    table[:, 0] = 1e-5
+    table[:, 1] = 0.25 / (K * 1e-5)
+    table[:, 5] = 2000.0
+    st = adjoint.map_fd_steps(table, np.array([[0, K]], dtype=np.int32))[0]
+    T = K * 1e-5
+    assert st[0] == 0 and st[1] == 0
+    assert math.isclose(2 * math.pi * T * 2000.0 * st[2], adjoint.FD_TURN)  # z: the phase over the pulse
+    assert math.isclose(2 * math.pi * T * st[3], adjoint.FD_TURN)  # db0
+    assert math.isclose(st[5] * T, adjoint.FD_TURN) and st[5] == st[6]  # r1, r2: the exponent
+    assert math.isclose(st[7] * math.pi / 2, adjoint.FD_TURN) and st[8] == st[7]  # b1: the rotation angle
