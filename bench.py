@@ -204,6 +204,10 @@ def main() -> None:
                          "byte counters of a profiler run")
     ap.add_argument("--readouts", choices=("direct", "nufft"), default="nufft",
                     help="readouts of the surrogate and reused-maps legs (nufft: type-1 NUFFT per readout run)")
+    ap.add_argument("--checkpoint-every", type=int, default=4352,
+                    help="segments per checkpoint interval of the forward + adjoint legs (a multiple of 64; 4 x the "
+                         "library's sqrt-of-segments default: 256 checkpoints of 24 B per spin and a 6.5 GB replay "
+                         "scratch fit one GPU's HBM many times, and fewer, longer launches shorten the tails)")
     ap.add_argument("--cpu-trs", type=int, default=32)
     ap.add_argument("--cpu-spins", type=int, default=65536)
     args = ap.parse_args()
@@ -258,7 +262,9 @@ def main() -> None:
     # -- forward + adjoint: exact RF, then the learned surrogate's maps (per-sample readouts, then NUFFT readouts)
     fwd_adj = fwd_adj_sur = fwd_adj_nu = None
     if not args.no_adjoint:
-        every = adjoint.default_checkpoint_every(S)
+        every = args.checkpoint_every
+        if every % 64:
+            raise SystemExit("--checkpoint-every must be a multiple of 64")
         case_a = driver.prepare_case(ph, seq, coils, chunk=every, aligned=True)
         fwd_adj = fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, args.steps, args.warmup, work,
                                   "exact RF")

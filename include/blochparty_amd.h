@@ -38,8 +38,8 @@ extern "C" {
                              may be sampled (HB_ADC: the sample is taken after the merged step)                 */
 #define HB_SPOILED_FREE 8192 /* free record without ADC whose Mxy never reaches a sample (spoiled at its end, or
                              starting at Mxy = 0): Mz relaxes over its dt, Mxy is zeroed                        */
-#define HB_NUFFT 16384    /* first of pad[3] readout samples taken as a NUFFT: slot pad[1], kind pad[2]; a spoil
-                             at the run's end is HB_PULSE_SPOIL                                                 */
+#define HB_NUFFT 16384    /* first of pad[3] readout samples taken as a NUFFT: slot pad[1], kind pad[2], index
+                             among all runs pad[4]; a spoil at the run's end is HB_PULSE_SPOIL                  */
 #define HB_XY_DEAD 32768  /* a NUFFT run whose Mxy at its end is never read (the next record is HB_SPOILED_FREE):
                              the run's jump relaxes Mz only and zeroes Mxy                                     */
 #define HB_SAME_FREE_DT 65536 /* free segment with the previous free one's dt here (reuse its relaxation factors,
