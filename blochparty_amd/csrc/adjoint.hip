@@ -9,10 +9,12 @@
 //      unspoil, add the ADC sample's source term sum_c h_c conj(S_c) and the coil gradients h_c conj(Mxy), reverse
 //      the step (adjoint_device.h).
 // Per-spin gradients are summed in registers and added to float64 accumulators at every segment index that is a
-// multiple of 64; per-segment gradients are summed over each wave (DPP + one MFMA per segment), then over the workgroup's 4 waves in
-// LDS in a fixed order, one partial per workgroup and segment, and over workgroups in a fixed order by seg_reduce. Nothing
-// depends on the interval length (a multiple of 64) or on scheduling: the gradients are the same bit for bit for
-// any checkpoint_every and every run.
+// multiple of 64 (when something was added since the last time); per-segment gradients are summed over each wave (DPP
+// + one MFMA per segment), then over the workgroup's 4 waves in LDS in a fixed order, one partial per workgroup and
+// segment, and over workgroups in a fixed order by seg_reduce. Nothing depends on the interval length (a multiple of
+// 64) or on scheduling: the gradients are the same bit for bit for any checkpoint_every and every run.
+// Reused pulses (maps) and NUFFT readout runs are one record each, their other segments identity bodies: the replay
+// jumps over them, the reverse skips the 8-blocks inside them (pulse_reverse, nufft_reverse below).
 #include "adjoint_device.h"
 
 namespace hb {

@@ -11,6 +11,12 @@ Forward: the dispatches are exactly the checkpoint intervals, the state before e
 ``hb_adjoint_interval`` per interval, last first, and ``hb_seg_reduce`` for the per-segment gradients. Parameter
 gradients map as the reference's (adjoint.py:222-261): t1 = 1 / r1 (-dL/dr1 / t1^2), t2 likewise, d/dpsi from the
 signal itself (dsignal/dpsi = -i signal), dL/dm0 = lambda at the start.
+
+Beyond the reference (whose reused pulses and NUFFT readouts are forward only, driver.py:221-225): with
+``pulses="reuse"`` / ``"surrogate"`` each occurrence of a reused pulse is one map in both sweeps (the sweep sums seeds
+for the maps' parameter gradients, ``hb_pulse_map_grads``, and each occurrence's amplitude and RF-phase gradient,
+``project_pulse_rf``); with ``readouts="nufft"`` each readout run is reversed as the transpose of its NUFFT
+(``hb_nufft_adjoint_grid`` and the sweep's interpolation). Either refuses gradients for dt, grad and b0_offset.
 """
 
 from __future__ import annotations
@@ -163,9 +169,9 @@ def run_backward(case: driver.DeviceCase, ckpts: list, every: int, h: torch.Tens
     cs = case.struct()
     seeds = torch.zeros(P * 12 * n_pad, dtype=torch.float64, device=dev) if P else None
     maps_d = None
+    steps = map_fd_steps(case.pulse_table.cpu().numpy(), case.pulse_index.cpu().numpy()) if P else None
     if P and need_seg:  # the maps' derivative along each pulse's amplitude, for its occurrences' rf gradients
         maps_d = torch.empty(P * 12 * n_pad, dtype=torch.float32, device=dev)
-        steps = map_fd_steps(case.pulse_table.cpu().numpy(), case.pulse_index.cpu().numpy())
         lib.pulse_map_dlnb1(ctypes.byref(cs), case.pulse_table.data_ptr(), case.pulse_index.data_ptr(), P,
                             float(steps[:, 8].min()), maps_d.data_ptr())
     gs = torch.zeros(8, n_pad, dtype=torch.float64, device=dev)
@@ -192,7 +198,6 @@ def run_backward(case: driver.DeviceCase, ckpts: list, every: int, h: torch.Tens
         if need_seg:
             lib.seg_reduce(gseg.data_ptr(), n_blocks, every, s0, s1 - s0, red.data_ptr(), seg.data_ptr())
     if P:  # the spin parameters' gradients through the maps, from the seeds of the sweep
-        steps = map_fd_steps(case.pulse_table.cpu().numpy(), case.pulse_index.cpu().numpy())
         st = torch.from_numpy(np.ascontiguousarray(steps[:, :8])).to(dev)
         lib.pulse_map_grads(ctypes.byref(cs), case.pulse_table.data_ptr(), case.pulse_index.data_ptr(), P,
                             st.data_ptr(), seeds.data_ptr(), gs.data_ptr())
