@@ -215,3 +215,16 @@ def test_api_refusals_without_gpu():
         capabilities.check("hip", capabilities.features(bp.Phantom(ph.pos, ph.pd, ph.t1, ph.t2, ph.db0,
                                                                    diffusion=torch.ones(10)), seq))
     capabilities.check("hip", frozenset({"b1", "m_init"}), grad=True)
+
+
+def test_default_checkpoint_interval():
+    """About sqrt(segments) in multiples of 64; 4x that when reused pulses or NUFFT runs make most segments identity
+    records (the 64^3 GRE: 1,110,016 segments -> 1,088 and 4,352)."""
+    from blochparty_amd.engines import adjoint
+
+    assert adjoint.default_checkpoint_every(1_110_016) == 1088
+    assert adjoint.default_checkpoint_every(1_110_016, bodies=True) == 4352
+    assert adjoint.default_checkpoint_every(10) == 64 and adjoint.default_checkpoint_every(0) == 64
+    for n in (100, 5000, 123457):
+        e = adjoint.default_checkpoint_every(n)
+        assert e % 64 == 0 and e >= n**0.5 and e - 64 < max(n**0.5, 64)
