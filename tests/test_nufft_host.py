@@ -192,3 +192,75 @@ def test_pulse_rf_projection_and_difference_steps():
     assert math.isclose(2 * math.pi * T * st[3], adjoint.FD_TURN)  # db0
     assert math.isclose(st[5] * T, adjoint.FD_TURN) and st[5] == st[6]  # r1, r2: the exponent
     assert math.isclose(st[7] * math.pi / 2, adjoint.FD_TURN) and st[8] == st[7]  # b1: the rotation angle
+
+
+def emulate_adjoint(plan, q, r2, h):
+    """The reverse sweep's arithmetic for a run (nufft_adjoint_grid_kernel, then nufft_reverse's window loop; float64,
+    one coil, unit sensitivity): per spin A = dL/dc for L = Re sum_i conj(h_i) s_i, and its derivatives in R2 and in
+    the fractional phase fr, as the kernel forms them."""
+    w, ng = plan.width, plan.ng
+    tw, lw = nufft.dft_tables(plan)
+    T = tw[..., 0] + 1j * tw[..., 1]  # (n, ng)
+    G = lw @ (np.conj(T) * h[:, None])  # (terms, ng): sum_i lw[m][i] conj(tw[i][u]) h_i
+    fr = q - np.floor(q)
+    x = fr * ng
+    E = np.exp(2j * np.pi * fr * plan.shift)
+    em = np.exp(-np.outer(r2, plan.nodes))  # (N, M)
+    hz = 2.0 / w
+    A = np.zeros(len(q), complex)
+    dR = np.zeros(len(q), complex)
+    dF = np.zeros(len(q), complex)
+    for j in range(len(q)):
+        b = j // 256
+        wb, Wd = plan.win[0, b]
+        Q = R = X = 0j
+        for lc in range(Wd):
+            z = (wb + lc - x[j]) * hz
+            u = 1 - z * z
+            if u <= 0:
+                continue
+            kv = np.exp(plan.beta * (np.sqrt(u) - 1))
+            kd = -kv * plan.beta * z / np.sqrt(u)
+            g = G[:, (wb + lc) % ng]
+            Q += kv * (em[j] * g).sum()
+            R += kv * (-plan.nodes * em[j] * g).sum()
+            X += kd * (em[j] * g).sum()
+        A[j] = E[j] * Q
+        dR[j] = E[j] * R
+        dF[j] = E[j] * (2j * np.pi * plan.shift * Q - ng * hz * X)
+    return A, dR, dF
+
+
+def test_nufft_adjoint_is_the_transpose_of_the_forward():
+    """The reverse sweep's run arithmetic against the forward's (emulate): <s, h> = <c, A> for random weights c and
+    sample gradients h (A is the exact transpose: a wrong conjugate, phase sign or cell index fails), and its R2 and
+    phase derivatives against central differences of L = Re sum conj(h) s in one spin's R2 and phase."""
+    seq = W.first_trs(W.gre3d_sequence(32), 2)
+    plan = nufft.make_plan(seq, 6, 3)
+    rng = np.random.default_rng(5)
+    N = 1500
+    pos = rng.uniform(-0.11, 0.11, (N, 3))
+    db0 = rng.uniform(-100, 100, N)
+    order = nufft.sort_order(plan, pos, db0, seq)
+    pos, db0 = pos[order], db0[order]
+    n_pad = -(-N // 256) * 256
+    assert nufft.windows(plan, pos, db0, seq, N, n_pad)
+    q = nufft.sample_cycles(pos, db0, seq, plan.kinds[0])
+    r2 = 1 / rng.uniform(0.03, 2.0, N)
+    c = rng.normal(size=N) + 1j * rng.normal(size=N)
+    h = rng.normal(size=plan.n) + 1j * rng.normal(size=plan.n)
+    s = emulate(plan, q, r2, c, n_pad)
+    A, dR, dF = emulate_adjoint(plan, q, r2, h)
+    lhs, rhs = (np.conj(s) * h).sum(), (np.conj(c) * A).sum()
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+    L = lambda qq, rr: float(np.real((np.conj(h) * emulate(plan, qq, rr, c, n_pad)).sum()))
+    for j in (3, 700, 1499):
+        e = np.zeros(N)
+        e[j] = 1e-4
+        fd_r = (L(q, r2 + e) - L(q, r2 - e)) / 2e-4
+        an_r = float(np.real(c[j] * np.conj(dR[j])))
+        assert abs(fd_r - an_r) < 1e-6 * max(abs(an_r), 1e-3), (j, fd_r, an_r)
+        e[j] = 1e-7
+        fd_q = (L(q + e, r2) - L(q - e, r2)) / 2e-7
+        an_q = float(np.real(c[j] * np.conj(dF[j])))
+        assert abs(fd_q - an_q) < 1e-5 * max(abs(an_q), 1.0), (j, fd_q, an_q)
