@@ -200,6 +200,28 @@ def test_reused_pulse_maps_in_the_adjoint(gpu):
     assert max(errors.values()) < 1e-4, errors
 
 
+def test_reused_pulse_maps_in_the_adjoint_two_pulses(gpu):
+    """A TSE with sinc pulses (a 90 excitation and an equal 180 refocusing pulse, each reused: two maps, two seed
+    sets): every spin input's gradient and the flip scale's (one amplitude for both pulses) against the oracle."""
+    maps = bp.shepp_logan_maps(12)
+    ph = bp.phantom_from_maps(maps["pd"], maps["t1"], maps["t2"], 0.22)
+    g = torch.Generator().manual_seed(5)
+    pos = ph.pos.clone()
+    pos[:, 2] = (torch.rand(len(ph), generator=g, dtype=f64) - 0.5) * 4e-3
+    ph = replace(ph, pos=pos, b1=torch.linspace(0.95, 1.05, len(ph), dtype=f64))
+    seq = bp.tse_2d(12, etl=4, tr=0.3, slice_thickness=5e-3, rf_duration=1e-3, system=bp.System(rf_raster=20e-6))
+    from blochparty_amd.engines import pulses
+
+    assert len(pulses.reused_pulses(seq)) == 2
+    coils = bp.birdcage_sensitivities(ph.pos, 4)
+    ref = map_gradients(oracle_sim, ph, seq, coils, None)
+    out = map_gradients(gpu_sim, ph, seq, coils, None, pulses="reuse", checkpoint_every=128)
+    err = {k: rel(out[k], ref[k]) for k in ref if ref[k].abs().max() > 0}
+    print("two pulses", {k: f"{v:.1e}" for k, v in err.items()}, flush=True)
+    assert {"pos", "pd", "t1", "t2", "b1", "flip", "coils"} <= set(err)
+    assert max(err.values()) < 1e-4, err
+
+
 def test_reused_pulse_gradients_bit_identical_for_any_checkpoint_interval(gpu):
     """Occurrences straddle the intervals differently for each interval length (the map is applied at an
     occurrence's first segment, the rest is the identity)."""
