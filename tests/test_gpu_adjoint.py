@@ -276,6 +276,17 @@ def test_nufft_readouts_in_the_adjoint(gpu):
     assert max(err.values()) < 1e-3, err
 
 
+def test_nufft_readouts_in_the_adjoint_16_coils(gpu):
+    """16 coils (the adjoint's widest register set) with NUFFT runs: every gradient against the oracle."""
+    ph, seq, _, m0 = nufft_case(side=48)
+    coils = bp.birdcage_sensitivities(ph.pos, 16)
+    ref = map_gradients(oracle_sim, ph, seq, coils, m0)
+    out = map_gradients(gpu_sim, ph, seq, coils, m0, readouts="nufft", checkpoint_every=128)
+    err = {k: rel(out[k], ref[k]) for k in ref if ref[k].abs().max() > 0}
+    print("16 coils nufft", {k: f"{v:.1e}" for k, v in err.items()}, flush=True)
+    assert max(err.values()) < 1e-4, err
+
+
 def test_nufft_gradients_bit_identical_for_any_checkpoint_interval(gpu):
     """Runs straddle the intervals differently for each interval length (a run is applied at its first segment, the
     rest is the identity)."""
