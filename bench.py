@@ -194,6 +194,9 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=64, help="grid size of the 3D GRE")
+    ap.add_argument("--coils", type=int, default=8, help="receive coils")
+    ap.add_argument("--iso", type=int, choices=(1, 4), default=4,
+                    help="isochromats per voxel (4: 1 x 2 x 2 sub-voxel, the benchmark's; 1: one per voxel)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity sample")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint leg")
     ap.add_argument("--no-exact", action="store_true", help="skip the exact-RF forward leg")
@@ -226,7 +229,7 @@ def main() -> None:
     hip.load()
     lib = hip.HIP.lib()
     t_build = time.perf_counter()
-    ph, seq, coils = W.gre3d(args.n)
+    ph, seq, coils = W.gre3d(args.n, n_coils=args.coils, spins_per_voxel=(1, 2, 2) if args.iso == 4 else (1, 1, 1))
     S = len(seq)
     log(f"workload built in {time.perf_counter() - t_build:.1f} s: N={len(ph)} S={S} n_adc={seq.n_adc} "
         f"C={coils.shape[1]}")
@@ -337,8 +340,8 @@ def main() -> None:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"3D GRE {args.n}^3 x 4 sub-voxel isochromats, three-class random phantom, +-100 Hz dB0, "
-                        f"2 ms sinc slab (TBW 4, 200 substeps), 8 coils, TR 8 ms; forward",
+            "workload": f"3D GRE {args.n}^3 x {args.iso} isochromat(s) per voxel, three-class random phantom, "
+                        f"+-100 Hz dB0, 2 ms sinc slab (TBW 4, 200 substeps), {args.coils} coils, TR 8 ms; forward",
             "spins": len(ph), "segments": S, "n_adc": seq.n_adc, "coils": int(coils.shape[1]),
             "rf": rf_desc, "pass": "forward",
         },
