@@ -228,3 +228,14 @@ def test_default_checkpoint_interval():
     for n in (100, 5000, 123457):
         e = adjoint.default_checkpoint_every(n)
         assert e % 64 == 0 and e >= n**0.5 and e - 64 < max(n**0.5, 64)
+
+
+def test_bench_runs_on_one_gpu_only():
+    """bench.py --gpus N > 1 exits with a message (this build measures one GPU's share)."""
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    bench = Path(__file__).resolve().parent.parent / "bench.py"
+    r = subprocess.run([sys.executable, str(bench), "--gpus", "2"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2 and "one GPU" in r.stderr, (r.returncode, r.stderr[-300:])
