@@ -1,34 +1,41 @@
 """Benchmark of the GPU path on one MI355X: spin-samples/s of simulate(..., engine="gpu") on the 3D GRE 64^3 workload.
 
-    python bench.py [--steps K] [--warmup W] [--n 64] [--no-cpu] [--no-adjoint] [--no-exact] [--copy-probe MIB]
+    python bench.py [--steps K] [--warmup W] [--n 64] [--no-cpu] [--no-adjoint] [--no-exact] [--no-e2e]
 
 Workload: 3D GRE n^3 x 4 sub-voxel isochromats (1,048,576 spins at n = 64), a three-class smooth random phantom,
 +-100 Hz dB0, 8 coils, a 2 ms sinc slab pulse (TBW 4, 200 substeps) every TR of 8 ms, n^2 lines of n samples.
-Every input is resident in HBM before a timed region starts (host packing and upload are outside it); each timed
-region is bracketed by torch.cuda.synchronize(). Throughput = spins x ADC samples x steps / seconds (the reference's
-metric, benchmarks/bench.py:65).
+Every input of a device leg is resident in HBM before its timed region starts (host packing and upload are outside
+it); each timed region is bracketed by torch.cuda.synchronize(). Throughput = spins x ADC samples x steps / seconds
+(the reference's metric, benchmarks/bench.py:65).
 
-Legs, K timed steps each after W warmup steps:
-- forward with the learned RF surrogate (pulses="surrogate") and NUFFT readouts (readouts="nufft", the default of
-  --readouts): every spin's pulse map from the network, the exact maps of every 257th spin composed and compared
-  (the judge), then the event loop with every readout as a type-1 NUFFT -- the JSON line's ``value``. The network is
-  trained once before the timed steps (its training time is reported, not timed);
-- forward with reused exact maps (pulses="reuse": the sinc pulse composed once per spin in float64), same readouts;
-- forward stepping every RF substep (the exact-RF baseline);
+The JSON line is printed as soon as the value's leg, its roofline and the CPU baseline exist, and printed again,
+extended, after every later leg: the last line printed is the complete one.
+
+Legs (the value's leg runs K timed steps after W warmup steps; every other leg runs min(K, its cap) timed steps after
+min(W, 1) warmup steps, the caps being --leg-steps and --adjoint-steps, and each leg states its own counts):
+- forward with the learned RF surrogate (pulses="surrogate") and NUFFT readouts: every spin's pulse map from the
+  network, the exact maps of every 257th spin composed and compared (the judge), then the event loop with every
+  readout as a type-1 NUFFT -- the JSON line's ``value``. The network is trained once before the timed steps (its
+  training time is reported, not timed);
+- forward with reused exact maps (pulses="reuse": the sinc pulse composed once per spin in float64), NUFFT readouts;
+- forward stepping every RF substep, NUFFT readouts (exact RF with the same readouts: the RF path's own ratio);
+- forward stepping every RF substep, per-sample readouts (the exact-RF baseline);
 - forward + adjoint of L = sum |signal|^2 w.r.t. the flip angle (through the RF gradients), the T1 map and the coil
   maps (the reference's --grad case, benchmarks/bench.py:69-95): the forward with checkpoints, dL/dsignal on the
-  device, and the reverse sweep; with exact RF (every substep stepped and reversed), then with the learned
-  surrogate's maps (each occurrence one map in both sweeps, the maps' parameter gradients from float64 central
-  differences of the exact composition once per step), and with the surrogate's maps and NUFFT readouts (each
-  readout run reversed as the transpose of its NUFFT: the headline forward + adjoint, whose ratio to the exact-RF leg
-  is ``fwd_adjoint_speedup_vs_exact_rf``). Each reports dL/dflip, which should agree.
+  device, and the reverse sweep; with exact RF, then with the learned surrogate's maps (per-sample readouts), then
+  with the surrogate's maps and NUFFT readouts (each readout run reversed as the transpose of its NUFFT; its ratio
+  to the exact-RF leg is ``fwd_adjoint_speedup_vs_exact_rf``). Each reports dL/dflip, which should agree;
+- ``end_to_end``: simulate() timed from the caller's Phantom / Sequence / coils to the CPU result, the way the
+  reference's harness times it (benchmarks/bench.py:47-95; sequence and phantom built outside the timed region):
+  the value's configuration forward, and forward + .backward() with the flip angle, the T1 map and the coils
+  requiring grad.
 Every forward-kernel launch of the value's leg is bracketed by HIP events on the stream it runs on; the roofline's
 ``achieved`` is the algorithmic flops per launch over their mean duration. ``traffic`` is the HBM bytes per launch
 of the same kernel from the committed rocprofv3 PMC summary (profiles/), when present.
 
 The CPU baseline is the FP64 oracle timed on this host's cores (the container quota) on a bounded sample of the same
 workload (a seeded subset of spins over the first TRs); the GPU runs the same sample on every RF path and reports
-their relative errors against the oracle. One JSON line goes to stdout; progress goes to stderr.
+their relative errors against the oracle. Progress goes to stderr.
 """
 
 from __future__ import annotations
@@ -137,11 +144,12 @@ def forward_leg(case, lib, driver, steps: int, warmup: int, work: int, label: st
     kernel_ms = [events[2 * i].elapsed_time(events[2 * i + 1]) for i in range(len(events) // 2)]
     flops = dispatch_flops(case, driver)
     mean_ms, mean_flops = float(np.mean(kernel_ms)), float(np.mean(flops))
-    out = {"value": work * steps / t, "ms_per_step": 1e3 * t / steps, "kernel_mean_ms": mean_ms,
-           "launches": len(kernel_ms), "flops_per_launch": mean_flops,
+    out = {"value": work * steps / t, "ms_per_step": 1e3 * t / steps, "steps": steps, "warmup": warmup,
+           "kernel_mean_ms": mean_ms, "launches": len(kernel_ms), "flops_per_launch": mean_flops,
            "kernel_tflops": mean_flops / (mean_ms * 1e-3) / 1e12}
-    log(f"forward ({label}): {out['ms_per_step']:.1f} ms/step, {out['value']:.3e} spin-samples/s; forward kernel "
-        f"mean {mean_ms:.3f} ms over {len(kernel_ms)} launches, {out['kernel_tflops']:.1f} TFLOP/s algorithmic")
+    log(f"forward ({label}): {out['ms_per_step']:.1f} ms/step over {steps} steps, {out['value']:.3e} spin-samples/s; "
+        f"forward kernel mean {mean_ms:.3f} ms over {len(kernel_ms)} launches, {out['kernel_tflops']:.1f} TFLOP/s "
+        f"algorithmic")
     return out
 
 
@@ -180,28 +188,69 @@ def fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, steps: int,
 
     t_a = timed(step, steps, warmup)
     assert torch.isfinite(out["flip"]) and torch.isfinite(out["t1"]).all() and torch.isfinite(out["coils"]).all()
-    leg = {"value": work * steps / t_a, "ms_per_step": 1e3 * t_a / steps, "rf": label, "checkpoint_every": every,
-           "loss": "sum |signal|^2", "wrt": ["flip angle", "T1 map", "coil maps"],
-           "dL_dflip_per_deg": float(out["flip"])}
-    log(f"forward + adjoint ({label}): {leg['ms_per_step']:.1f} ms/step, {leg['value']:.3e} spin-samples/s, "
-        f"dL/dflip {leg['dL_dflip_per_deg']:.6e}")
+    leg = {"value": work * steps / t_a, "ms_per_step": 1e3 * t_a / steps, "steps": steps, "warmup": warmup,
+           "rf": label, "checkpoint_every": every, "loss": "sum |signal|^2",
+           "wrt": ["flip angle", "T1 map", "coil maps"], "dL_dflip_per_deg": float(out["flip"])}
+    log(f"forward + adjoint ({label}): {leg['ms_per_step']:.1f} ms/step over {steps} steps, "
+        f"{leg['value']:.3e} spin-samples/s, dL/dflip {leg['dL_dflip_per_deg']:.6e}")
     return leg
+
+
+def end_to_end(bp, W, args, ph, seq, coils, work: int, steps: int, warmup: int) -> dict:
+    """simulate() timed from the caller's objects to the CPU result (host packing, upload, the GPU work, readback),
+    as the reference's harness times it; the grad case adds .backward() of sum |signal|^2 with the flip angle, the
+    T1 map and the coils requiring grad. Inputs are built before the timed region, as the reference's are."""
+    opts = dict(engine="gpu", pulses="surrogate", readouts="nufft")
+    out = {"options": "pulses='surrogate', readouts='nufft'", "steps": steps, "warmup": warmup}
+
+    def fwd():
+        sig = bp.simulate(ph, seq, coils, **opts).signal
+        assert torch.isfinite(sig).all()
+
+    t = timed(fwd, steps, warmup)
+    out["forward"] = {"value": work * steps / t, "s_per_call": t / steps}
+    log(f"end to end: simulate() forward {t / steps:.2f} s/call, {work * steps / t:.3e} spin-samples/s")
+    # the grad case: the flip as a tensor through the sequence builder, the T1 map and the coils as leaves
+    flip = torch.tensor(FLIP_DEG, dtype=torch.float64, requires_grad=True)
+    seq_g = W.gre3d_sequence(args.n, flip_deg=flip)
+    t1 = ph.t1.detach().clone().requires_grad_()
+    ph_g = bp.Phantom(ph.pos, ph.pd, t1, ph.t2, ph.db0, b1=ph.b1)
+    coils_g = coils.detach().clone().requires_grad_()
+
+    def fwd_bwd():
+        for x in (flip, t1, coils_g):
+            x.grad = None
+        sig = bp.simulate(ph_g, seq_g, coils_g, **opts).signal
+        (sig.abs() ** 2).sum().backward()
+        assert torch.isfinite(flip.grad) and torch.isfinite(t1.grad).all() and torch.isfinite(coils_g.grad).all()
+
+    t = timed(fwd_bwd, steps, warmup)
+    out["forward_backward"] = {"value": work * steps / t, "s_per_call": t / steps,
+                               "dL_dflip_per_deg": float(flip.grad)}
+    log(f"end to end: simulate() + backward {t / steps:.2f} s/call, {work * steps / t:.3e} spin-samples/s, "
+        f"dL/dflip {float(flip.grad):.6e}")
+    return out
 
 
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--leg-steps", type=int, default=3,
+                    help="timed steps of each forward leg other than the value's, at most (and at most --steps)")
+    ap.add_argument("--adjoint-steps", type=int, default=1,
+                    help="timed steps of each forward + adjoint leg and each end-to-end case, at most")
     ap.add_argument("--n", type=int, default=64, help="grid size of the 3D GRE")
     ap.add_argument("--coils", type=int, default=8, help="receive coils")
     ap.add_argument("--iso", type=int, choices=(1, 4), default=4,
                     help="isochromats per voxel (4: 1 x 2 x 2 sub-voxel, the benchmark's; 1: one per voxel)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity sample")
-    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint leg")
-    ap.add_argument("--no-exact", action="store_true", help="skip the exact-RF forward leg")
-    ap.add_argument("--no-surrogate", action="store_true",
-                    help="skip the surrogate leg (the value is then the reused exact maps' forward)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint legs")
+    ap.add_argument("--no-exact", action="store_true", help="skip the exact-RF legs")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end simulate() timing")
+    ap.add_argument("--only-value", action="store_true",
+                    help="the value's leg alone (no other leg, no CPU baseline): for profiler runs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the legs, three elementwise copies of MIB MiB (read + write) that calibrate the HBM "
                          "byte counters of a profiler run")
@@ -218,6 +267,10 @@ def main() -> None:
         print(f"bench.py runs on one GPU: the path shards over spins, but this build measures one GPU's share "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
+    if args.only_value:
+        args.no_cpu = args.no_adjoint = args.no_exact = args.no_e2e = True
+    if args.checkpoint_every % 64:
+        raise SystemExit("--checkpoint-every must be a multiple of 64")
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import blochparty_amd as bp
@@ -234,69 +287,50 @@ def main() -> None:
     log(f"workload built in {time.perf_counter() - t_build:.1f} s: N={len(ph)} S={S} n_adc={seq.n_adc} "
         f"C={coils.shape[1]}")
     work = len(ph) * seq.n_adc
+    nufft = (6, 3) if args.readouts == "nufft" else None
+    K, Wm = args.steps, args.warmup
+    leg_k, leg_w = max(1, min(K, args.leg_steps)), min(Wm, 1)
+    adj_k, adj_w = max(1, min(K, args.adjoint_steps)), min(Wm, 1)
 
-    # -- the surrogate leg (the value) and the reused exact maps' leg
-    surrogate = None
-    if not args.no_surrogate:
-        t0 = time.perf_counter()
-        nufft = (6, 3) if args.readouts == "nufft" else None
-        case_s = driver.prepare_case(ph, seq, coils, surrogate=True, nufft=nufft)
-        torch.cuda.synchronize()
-        nets = case_s.extra["surrogate"]
-        log(f"surrogate: case prepared and {len(nets)} network(s) trained in {time.perf_counter() - t0:.1f} s")
-        leg_s = forward_leg(case_s, lib, driver, args.steps, args.warmup, work, "learned RF surrogate")
-        surrogate = {"forward": leg_s, "judge": sur.report(case_s)}
-        del case_s
-        torch.cuda.empty_cache()
-    case_r = driver.prepare_case(ph, seq, coils, reuse=True, nufft=(6, 3) if args.readouts == "nufft" else None)
-    leg_r = forward_leg(case_r, lib, driver, args.steps, args.warmup, work, "reused exact maps")
-    del case_r
+    rf_desc = (f"learned surrogate maps (network per spin; exact maps of every {sur.JUDGE_STRIDE}th spin as the "
+               f"judge); readouts {args.readouts}" + (" (type-1 NUFFT, width 6, 3 decay terms)" if nufft else ""))
+    line = {
+        "metric": METRIC, "value": None, "unit": "spin·samples/s", "n_gpus": 1, "steps": K, "warmup": Wm,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded phantom, coils and sequence of the configuration's shape)",
+        "config": {
+            "workload": f"3D GRE {args.n}^3 x {args.iso} isochromat(s) per voxel, three-class random phantom, "
+                        f"+-100 Hz dB0, 2 ms sinc slab (TBW 4, 200 substeps), {args.coils} coils, TR 8 ms; forward",
+            "spins": len(ph), "segments": S, "n_adc": seq.n_adc, "coils": int(coils.shape[1]),
+            "rf": rf_desc, "pass": "forward",
+            "value_is": "device-resident kernel throughput (inputs uploaded before the timed region); simulate() "
+                        "timed end to end is under end_to_end",
+        },
+        "roofline": None, "cpu_baseline": None,
+    }
+
+    def emit():
+        print(json.dumps(line), flush=True)
+
+    # -- the value: the surrogate leg
+    t0 = time.perf_counter()
+    case_s = driver.prepare_case(ph, seq, coils, surrogate=True, nufft=nufft)
+    torch.cuda.synchronize()
+    nets = case_s.extra["surrogate"]
+    log(f"surrogate: case prepared and {len(nets)} network(s) trained in {time.perf_counter() - t0:.1f} s")
+    lead = forward_leg(case_s, lib, driver, K, Wm, work, "learned RF surrogate")
+    line["value"], line["ms_per_step"] = lead["value"], lead["ms_per_step"]
+    line["roofline"] = {
+        "bound": "valu", "kernel": "forward_kernel", "achieved": lead["kernel_tflops"], "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s", "frac": lead["kernel_tflops"] / FP32_PEAK_TFLOPS, "traffic": profiled_traffic(),
+        "mean_launch_ms": lead["kernel_mean_ms"], "launches": lead["launches"],
+        "flops_per_launch": lead["flops_per_launch"],
+    }
+    line["surrogate"] = {"forward": lead, "judge": sur.report(case_s)}
+    del case_s
     torch.cuda.empty_cache()
-    lead = surrogate["forward"] if surrogate else leg_r
 
-    # -- forward, exact RF
-    exact = None
-    if not args.no_exact:
-        case_e = driver.prepare_case(ph, seq, coils)
-        exact = forward_leg(case_e, lib, driver, args.steps, args.warmup, work, "exact RF")
-        del case_e
-        torch.cuda.empty_cache()
-
-    # -- forward + adjoint: exact RF, then the learned surrogate's maps (per-sample readouts, then NUFFT readouts)
-    fwd_adj = fwd_adj_sur = fwd_adj_nu = None
-    if not args.no_adjoint:
-        every = args.checkpoint_every
-        if every % 64:
-            raise SystemExit("--checkpoint-every must be a multiple of 64")
-        case_a = driver.prepare_case(ph, seq, coils, chunk=every, aligned=True)
-        fwd_adj = fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, args.steps, args.warmup, work,
-                                  "exact RF")
-        del case_a
-        torch.cuda.empty_cache()
-        if not args.no_surrogate:
-            case_a = driver.prepare_case(ph, seq, coils, chunk=every, aligned=True, surrogate=True)
-            fwd_adj_sur = fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, args.steps, args.warmup,
-                                          work, "learned RF surrogate")
-            del case_a
-            torch.cuda.empty_cache()
-            if args.readouts == "nufft":
-                case_a = driver.prepare_case(ph, seq, coils, chunk=every, aligned=True, surrogate=True, nufft=(6, 3))
-                fwd_adj_nu = fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, args.steps,
-                                             args.warmup, work, "learned RF surrogate, NUFFT readouts")
-                del case_a
-                torch.cuda.empty_cache()
-
-    if args.copy_probe:  # a plain copy of known size: x * 1 into y reads and writes copy_probe MiB each
-        x = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(3):
-            torch.mul(x, 1.0, out=y)
-        torch.cuda.synchronize()
-        log(f"copy probe: 3 x {args.copy_probe} MiB read + {args.copy_probe} MiB written")
-        del x, y
-
-    # -- CPU baseline and parity sample
-    cpu = parity = None
+    # -- the CPU baseline (bounded sample); the GPU's parity on the same sample comes after the device legs
     if not args.no_cpu:
         from oracle import core  # the checker / CPU baseline only
 
@@ -307,69 +341,84 @@ def main() -> None:
         t1 = time.perf_counter()
         ref_sig, ref_m = core.simulate(ph_s, seq_s, c_s)
         t_cpu = time.perf_counter() - t1
-        cpu = {"value": len(ph_s) * seq_s.n_adc / t_cpu, "unit": "spin·samples/s", "cores": cores, "kind": "port",
-               "sample": f"FP64 oracle, {len(ph_s)} of the {len(ph)} spins (seeded subset) over the first "
-                         f"{args.cpu_trs} TRs ({seq_s.n_adc} samples, {len(seq_s)} segments) of the same sequence; "
-                         f"{t_cpu:.1f} s"}
+        line["cpu_baseline"] = {
+            "value": len(ph_s) * seq_s.n_adc / t_cpu, "unit": "spin·samples/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle, {len(ph_s)} of the {len(ph)} spins (seeded subset) over the first "
+                      f"{args.cpu_trs} TRs ({seq_s.n_adc} samples, {len(seq_s)} segments) of the same sequence; "
+                      f"{t_cpu:.1f} s"}
+        log(f"cpu oracle {line['cpu_baseline']['value']:.3e} spin-samples/s on {cores} cores ({t_cpu:.1f} s)")
+    emit()
+
+    # -- reused exact maps, same readouts
+    case_r = driver.prepare_case(ph, seq, coils, reuse=True, nufft=nufft)
+    line["reuse_exact_maps_forward"] = forward_leg(case_r, lib, driver, leg_k, leg_w, work, "reused exact maps")
+    del case_r
+    torch.cuda.empty_cache()
+    emit()
+
+    # -- exact RF: with the value's readouts, then per-sample readouts (the baseline)
+    if not args.no_exact:
+        if nufft is not None:
+            case_e = driver.prepare_case(ph, seq, coils, nufft=nufft)
+            line["exact_rf_nufft_forward"] = forward_leg(case_e, lib, driver, leg_k, leg_w, work,
+                                                         "exact RF, NUFFT readouts")
+            line["speedup_vs_exact_rf_same_readouts"] = lead["value"] / line["exact_rf_nufft_forward"]["value"]
+            del case_e
+            torch.cuda.empty_cache()
+            emit()
+        case_e = driver.prepare_case(ph, seq, coils)
+        line["exact_rf_forward"] = forward_leg(case_e, lib, driver, leg_k, leg_w, work, "exact RF")
+        line["speedup_vs_exact_rf"] = lead["value"] / line["exact_rf_forward"]["value"]
+        del case_e
+        torch.cuda.empty_cache()
+        emit()
+
+    # -- forward + adjoint: the surrogate's maps with the value's readouts first, then exact RF
+    if not args.no_adjoint:
+        every = args.checkpoint_every
+        legs = [("fwd_adjoint_surrogate_nufft", dict(surrogate=True, nufft=nufft),
+                 "learned RF surrogate, NUFFT readouts")] if nufft else []
+        legs += [("fwd_adjoint_surrogate", dict(surrogate=True), "learned RF surrogate"),
+                 ("fwd_adjoint", {}, "exact RF")]
+        for key, kw, label in legs:
+            if key == "fwd_adjoint" and args.no_exact:
+                continue
+            case_a = driver.prepare_case(ph, seq, coils, chunk=every, aligned=True, **kw)
+            line[key] = fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, adj_k, adj_w, work, label)
+            del case_a
+            torch.cuda.empty_cache()
+            best = line.get("fwd_adjoint_surrogate_nufft") or line.get("fwd_adjoint_surrogate")
+            if line.get("fwd_adjoint") and best:
+                line["fwd_adjoint_speedup_vs_exact_rf"] = best["value"] / line["fwd_adjoint"]["value"]
+            emit()
+
+    # -- simulate() end to end
+    if not args.no_e2e:
+        line["end_to_end"] = end_to_end(bp, W, args, ph, seq, coils, work, adj_k, adj_w)
+        emit()
+
+    # -- the GPU on the CPU baseline's sample, every RF path, against the oracle
+    if not args.no_cpu:
         parity = {"sample": "the CPU-baseline sample"}
-        paths = (("surrogate", "surrogate"), ("reuse", "reuse"), ("exact", "step"))
-        for name, opt in paths[0 if surrogate else 1:]:
+        for name, opt in (("surrogate", "surrogate"), ("reuse", "reuse"), ("exact", "step")):
             ro = args.readouts if opt != "step" else "direct"
             o = bp.simulate(ph_s, seq_s, c_s, engine="gpu", pulses=opt, readouts=ro)
             parity[f"{name}_signal_rel_err"] = float((o.signal.to(torch.complex128) - ref_sig).abs().max()
                                                      / ref_sig.abs().max())
             parity[f"{name}_m_rel_err"] = float((o.m.double() - ref_m).abs().max() / ref_m.abs().max())
-        log(f"cpu oracle {cpu['value']:.3e} spin-samples/s on {cores} cores; GPU vs oracle on the sample: "
-            + ", ".join(f"{k[:-15]} {v:.2e}" for k, v in parity.items() if k.endswith("signal_rel_err")))
+        line["parity"] = parity
+        log("GPU vs oracle on the sample: " + ", ".join(f"{k[:-15]} {v:.2e}" for k, v in parity.items()
+                                                        if k.endswith("signal_rel_err")))
+        emit()
 
-    rf_desc = ("learned surrogate maps (network per spin; exact maps of every 257th spin as the judge)" if surrogate
-               else "reused per-spin pulse maps (float64, exact)")
-    rf_desc += f"; readouts {args.readouts}" + (" (type-1 NUFFT, width 6, 3 decay terms)" if args.readouts == "nufft"
-                                                else "")
-    line = {
-        "metric": METRIC,
-        "value": lead["value"],
-        "unit": "spin·samples/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": lead["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"3D GRE {args.n}^3 x {args.iso} isochromat(s) per voxel, three-class random phantom, "
-                        f"+-100 Hz dB0, 2 ms sinc slab (TBW 4, 200 substeps), {args.coils} coils, TR 8 ms; forward",
-            "spins": len(ph), "segments": S, "n_adc": seq.n_adc, "coils": int(coils.shape[1]),
-            "rf": rf_desc, "pass": "forward",
-        },
-        "roofline": {
-            "bound": "valu",
-            "kernel": "forward_kernel",
-            "achieved": lead["kernel_tflops"],
-            "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": lead["kernel_tflops"] / FP32_PEAK_TFLOPS,
-            "traffic": profiled_traffic(),
-            "mean_launch_ms": lead["kernel_mean_ms"],
-            "launches": lead["launches"],
-            "flops_per_launch": lead["flops_per_launch"],
-        },
-        "surrogate": surrogate,
-        "reuse_exact_maps_forward": leg_r,
-        "exact_rf_forward": exact,
-        "speedup_vs_exact_rf": None if exact is None else lead["value"] / exact["value"],
-        "fwd_adjoint": fwd_adj,
-        "fwd_adjoint_surrogate": fwd_adj_sur,
-        "fwd_adjoint_surrogate_nufft": fwd_adj_nu,
-        "fwd_adjoint_speedup_vs_exact_rf": (None if fwd_adj is None or (fwd_adj_nu or fwd_adj_sur) is None
-                                            else (fwd_adj_nu or fwd_adj_sur)["value"] / fwd_adj["value"]),
-        "cpu_baseline": cpu,
-        "parity": parity,
-    }
-    print(json.dumps(line), flush=True)
+    if args.copy_probe:  # a plain copy of known size: x * 1 into y reads and writes copy_probe MiB each
+        x = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        log(f"copy probe: 3 x {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+        del x, y
 
 
 if __name__ == "__main__":

@@ -31,8 +31,9 @@ def gre3d_phantom(n: int = 64, spins_per_voxel=(1, 2, 2)) -> Phantom:
     return phantom_from_maps(maps["pd"], maps["t1"], maps["t2"], FOV, maps["db0"], spins_per_voxel=spins_per_voxel)
 
 
-def gre3d_sequence(n: int = 64, n_dummy: int = 0):
-    return gre_cartesian((n, n, n), FOV, flip_deg=10.0, tr=8e-3, n_dummy=n_dummy, slice_thickness=FOV,
+def gre3d_sequence(n: int = 64, n_dummy: int = 0, flip_deg=10.0):
+    """``flip_deg`` may be a tensor that requires grad (the RF waveform is then flip_deg x the shape per degree)."""
+    return gre_cartesian((n, n, n), FOV, flip_deg=flip_deg, tr=8e-3, n_dummy=n_dummy, slice_thickness=FOV,
                          rf_duration=2e-3, time_bw_product=4.0, system=System(rf_raster=10e-6))
 
 
