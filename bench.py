@@ -201,7 +201,8 @@ def end_to_end(bp, W, args, ph, seq, coils, work: int, steps: int, warmup: int) 
     as the reference's harness times it; the grad case adds .backward() of sum |signal|^2 with the flip angle, the
     T1 map and the coils requiring grad. Inputs are built before the timed region, as the reference's are."""
     opts = dict(engine="gpu", pulses="surrogate", readouts="nufft")
-    out = {"options": "pulses='surrogate', readouts='nufft'", "steps": steps, "warmup": warmup}
+    out = {"options": "pulses='surrogate', readouts='nufft' (rf_grad='scale' for the gradient case: the flip angle "
+                      "is each occurrence's amplitude)", "steps": steps, "warmup": warmup}
 
     def fwd():
         sig = bp.simulate(ph, seq, coils, **opts).signal
@@ -220,7 +221,7 @@ def end_to_end(bp, W, args, ph, seq, coils, work: int, steps: int, warmup: int) 
     def fwd_bwd():
         for x in (flip, t1, coils_g):
             x.grad = None
-        sig = bp.simulate(ph_g, seq_g, coils_g, **opts).signal
+        sig = bp.simulate(ph_g, seq_g, coils_g, rf_grad="scale", **opts).signal
         (sig.abs() ** 2).sum().backward()
         assert torch.isfinite(flip.grad) and torch.isfinite(t1.grad).all() and torch.isfinite(coils_g.grad).all()
 

@@ -16,6 +16,7 @@ import dataclasses
 import torch
 from torch import Tensor
 
+from ..interop import from_reference
 from ..phantom import Phantom
 from ..result import SimResult
 from ..sequence import Sequence
@@ -23,7 +24,8 @@ from . import hip
 from .capabilities import check, features
 
 ENGINES = ("gpu", "hip")
-GPU_OPTIONS = ("chunk", "group", "mma", "readouts", "nufft_width", "nufft_terms", "pulses")
+# the reference's GPU options (engines/__init__.py:23), then rf_grad: how rf is differentiated through reused pulses
+GPU_OPTIONS = ("chunk", "group", "mma", "readouts", "nufft_width", "nufft_terms", "pulses", "rf_grad")
 _OTHER_ENGINES = ("core", "chunked", "metal", "cuda")
 
 
@@ -86,15 +88,21 @@ def simulate(
     options: ``chunk`` (segments per dispatch, default 32768), ``group`` (256), ``mma`` (the matrix cores always
     contract the coils here; any value is accepted), ``readouts`` ("direct", or "nufft": readouts of >= 16 equal
     samples as a type-1 NUFFT with ``nufft_width`` kernel cells (6) and ``nufft_terms`` decay terms (3); under
-    autograd the reverse sweep takes each run as the transpose of its NUFFT), ``pulses`` ("step"; "reuse": a pulse that recurs up to a global RF phase is composed
-    once per spin into its affine map, in float64, and applied at every occurrence; "surrogate": those maps from
+    autograd the reverse sweep takes each run as the transpose of its NUFFT), ``pulses`` ("step"; "reuse": a pulse
+    that recurs up to a global RF phase is composed once per spin into its affine map, in float64, and applied at
+    every occurrence; "surrogate": those maps from
     the learned RF surrogate, trained on the GPU on first use, with the exact composition as the fallback outside its
     trained box and as the judge of its error on every run, reported in ``SimResult.surrogate``; under autograd both
     run as maps in the adjoint too, with gradients for the spin inputs, coils, m_init and each occurrence's RF
-    amplitude and phase, and ValueError when dt, grad or b0_offset require a gradient).
-    Other keywords raise TypeError.
+    amplitude and phase, and ValueError when dt, grad or b0_offset require a gradient), ``rf_grad`` (None, or
+    "scale": with reused pulses under autograd, rf's gradient is each occurrence's complex-amplitude component, exact
+    for flip angles and RF phases and not for a pulse's shape; with rf requiring grad, pulses="reuse" / "surrogate"
+    raise ValueError unless rf_grad="scale" accepts that).
+    Other keywords raise TypeError. ``phantom`` and ``seq`` may be the reference package's objects (or any dataclass
+    with their fields): they are taken over field by field, tensors shared (``interop.from_reference``).
     """
     name = resolve(engine)
+    phantom, seq = from_reference(phantom), from_reference(seq)
     if unknown := sorted(set(options) - set(GPU_OPTIONS)):
         raise TypeError(f"simulate(engine={engine!r}) got unexpected keyword argument(s) {', '.join(unknown)}")
     if not merge_free_runs:
@@ -114,12 +122,15 @@ def simulate(
         raise ValueError(f"pulses must be 'step', 'reuse' or 'surrogate', not {pulses!r}")
     if chunk < 1:
         raise ValueError("chunk must be positive")
+    rf_grad = options.get("rf_grad")
+    if rf_grad not in (None, "scale"):
+        raise ValueError(f"rf_grad must be None or 'scale', not {rf_grad!r}")
     grad = torch.is_grad_enabled() and requires_grad(phantom, seq, coils, m_init)
     check(name, features(phantom, seq, m_init, scanner, maps), grad=grad)
     if grad:
         from .adjoint import simulate_grad
 
-        return simulate_grad(phantom, seq, coils, m_init, checkpoint_every, pulses, nufft)
+        return simulate_grad(phantom, seq, coils, m_init, checkpoint_every, pulses, nufft, rf_grad)
     from .driver import forward
 
     with torch.no_grad():

@@ -260,11 +260,13 @@ def adjoint_sens(coils: torch.Tensor, n_pad: int, nc: int) -> torch.Tensor:
 
 
 def simulate_grad(phantom: Phantom, seq: Sequence, coils, m_init, checkpoint_every, pulses: str = "step",
-                  nufft=None) -> SimResult:
+                  nufft=None, rf_grad: str | None = None) -> SimResult:
     """simulate(engine="gpu") with gradients: the inputs as tensors of ``GPUSimulate`` (adjoint.py:264-323).
     ``pulses``: "step", or "reuse" / "surrogate" (reused pulses as maps in both sweeps); ``nufft``: None or (width,
     terms), readout runs as NUFFTs in both sweeps (the reference's reused pulses and NUFFT readouts are forward only,
-    driver.py:221-225)."""
+    driver.py:221-225). ``rf_grad``: "scale" accepts that, with maps, rf's gradient is each occurrence's
+    complex-amplitude component (``project_pulse_rf``); without it a reused pulse with rf requiring grad raises
+    ValueError, since a loss through the pulse's shape would get a wrong gradient."""
     n, pd = len(phantom), phantom.pd
     if coils is None:
         coils = torch.ones(n, 1, dtype=torch.complex64 if pd.dtype == torch.float32 else torch.complex128)
@@ -282,6 +284,10 @@ def simulate_grad(phantom: Phantom, seq: Sequence, coils, m_init, checkpoint_eve
             what = f"pulses={pulses!r}" if pulses != "step" else "readouts='nufft'"
             raise ValueError(f"{what} differentiates spin parameters, coils, m_init and RF amplitudes and phases, not "
                              f"{', '.join(fixed)}: use pulses='step' and readouts='direct'")
+    if pulses != "step" and seq.rf.requires_grad and rf_grad != "scale":
+        raise ValueError(f"pulses={pulses!r} gives rf only each reused pulse occurrence's complex-amplitude gradient "
+                         f"(exact for flip angles and RF phases, not for a pulse's shape): pass rf_grad='scale' to "
+                         f"accept that, or use pulses='step' for every rf sample's gradient")
     opts = Opts(list(seq.is_rf), list(seq.adc), list(seq.spoil), every, pulses, nufft)
     sig, m = GPUSimulate.apply(opts, phantom.pos, pd, phantom.t1, phantom.t2, phantom.db0, torch.view_as_real(coils),
                                seq.dt, seq.rf, seq.grad, seq.adc_phase, m0, phantom.b1, seq.b0_offset)

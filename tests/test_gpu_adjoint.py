@@ -190,7 +190,7 @@ def pulse_case(n=8, seed=3, n_phantom=None):
 def test_reused_pulse_maps_in_the_adjoint(gpu):
     ph, seq, coils, m0 = pulse_case()
     ref = map_gradients(oracle_sim, ph, seq, coils, m0)
-    out = map_gradients(gpu_sim, ph, seq, coils, m0, pulses="reuse", checkpoint_every=64)
+    out = map_gradients(gpu_sim, ph, seq, coils, m0, pulses="reuse", rf_grad="scale", checkpoint_every=64)
     step = map_gradients(gpu_sim, ph, seq, coils, m0, checkpoint_every=64)
     errors = {k: rel(out[k], ref[k]) for k in ref if ref[k].abs().max() > 0}
     errors_step = {k: rel(step[k], ref[k]) for k in ref if ref[k].abs().max() > 0}
@@ -215,7 +215,7 @@ def test_reused_pulse_maps_in_the_adjoint_two_pulses(gpu):
     assert len(pulses.reused_pulses(seq)) == 2
     coils = bp.birdcage_sensitivities(ph.pos, 4)
     ref = map_gradients(oracle_sim, ph, seq, coils, None)
-    out = map_gradients(gpu_sim, ph, seq, coils, None, pulses="reuse", checkpoint_every=128)
+    out = map_gradients(gpu_sim, ph, seq, coils, None, pulses="reuse", rf_grad="scale", checkpoint_every=128)
     err = {k: rel(out[k], ref[k]) for k in ref if ref[k].abs().max() > 0}
     print("two pulses", {k: f"{v:.1e}" for k, v in err.items()}, flush=True)
     assert {"pos", "pd", "t1", "t2", "b1", "flip", "coils"} <= set(err)
@@ -226,7 +226,8 @@ def test_reused_pulse_gradients_bit_identical_for_any_checkpoint_interval(gpu):
     """Occurrences straddle the intervals differently for each interval length (the map is applied at an
     occurrence's first segment, the rest is the identity)."""
     ph, seq, coils, m0 = pulse_case(6, seed=4)
-    runs = [map_gradients(gpu_sim, ph, seq, coils, m0, pulses="reuse", checkpoint_every=k) for k in (64, 192)]
+    runs = [map_gradients(gpu_sim, ph, seq, coils, m0, pulses="reuse", rf_grad="scale", checkpoint_every=k)
+            for k in (64, 192)]
     for k in runs[0]:
         assert torch.equal(runs[0][k], runs[1][k]), k
 
@@ -237,6 +238,20 @@ def test_reused_pulse_gradients_refuse_timing_and_gradient_inputs(gpu):
         s = replace(seq, **{name: getattr(seq, name).clone().requires_grad_()})
         with pytest.raises(ValueError, match="pulses='step'"):
             bp.simulate(ph, s, coils, engine="gpu", pulses="reuse")
+
+
+def test_reused_pulse_rf_gradient_needs_the_scale_opt_in(gpu):
+    """With maps, rf's gradient is each occurrence's complex-amplitude component only: a waveform leaf that requires
+    grad is refused unless rf_grad='scale' accepts that; the spin inputs alone need no opt-in."""
+    ph, seq, coils, m0 = pulse_case(4)
+    s = replace(seq, rf=seq.rf.clone().requires_grad_())
+    for pulses in ("reuse", "surrogate"):
+        with pytest.raises(ValueError, match="rf_grad='scale'"):
+            bp.simulate(ph, s, coils, engine="gpu", pulses=pulses)
+    t1 = ph.t1.clone().requires_grad_()
+    out = bp.simulate(replace(ph, t1=t1), seq, coils, engine="gpu", pulses="reuse")
+    (out.signal.abs() ** 2).sum().backward()
+    assert torch.isfinite(t1.grad).all() and t1.grad.abs().max() > 0
 
 
 # ---- NUFFT readouts in the adjoint (readouts="nufft"; the reference's are forward only)

@@ -152,8 +152,8 @@ def test_gradients_through_the_surrogate_maps(gpu, trained, slab):
         return r.signal, r.m
 
     ref = map_gradients(lambda p, s, c, m0: core.simulate(p, s, c, m_init=m0), ph_s, seq4, c_s, None)
-    sur_g = map_gradients(gpu, ph_s, seq4, c_s, None, pulses="surrogate", checkpoint_every=128)
-    exact = map_gradients(gpu, ph_s, seq4, c_s, None, pulses="reuse", checkpoint_every=128)
+    sur_g = map_gradients(gpu, ph_s, seq4, c_s, None, pulses="surrogate", rf_grad="scale", checkpoint_every=128)
+    exact = map_gradients(gpu, ph_s, seq4, c_s, None, pulses="reuse", rf_grad="scale", checkpoint_every=128)
     err = {k: rel(sur_g[k], ref[k]) for k in ref if ref[k].abs().max() > 0}
     err_x = {k: rel(exact[k], ref[k]) for k in ref if ref[k].abs().max() > 0}
     print("surrogate", {k: f"{v:.1e}" for k, v in err.items()}, flush=True)
@@ -177,8 +177,9 @@ def test_gradients_through_the_surrogate_maps_and_nufft_readouts(gpu):
         return r.signal, r.m
 
     ref = map_gradients(lambda p, s, c, m0: core.simulate(p, s, c, m_init=m0), ph_s, seq4, c_s, None)
-    sur_nu = map_gradients(gpu, ph_s, seq4, c_s, None, pulses="surrogate", readouts="nufft", checkpoint_every=256)
-    exact_nu = map_gradients(gpu, ph_s, seq4, c_s, None, pulses="reuse", readouts="nufft", checkpoint_every=256)
+    kw = dict(rf_grad="scale", readouts="nufft", checkpoint_every=256)
+    sur_nu = map_gradients(gpu, ph_s, seq4, c_s, None, pulses="surrogate", **kw)
+    exact_nu = map_gradients(gpu, ph_s, seq4, c_s, None, pulses="reuse", **kw)
     err = {k: rel(sur_nu[k], ref[k]) for k in ref if ref[k].abs().max() > 0}
     err_x = {k: rel(exact_nu[k], ref[k]) for k in ref if ref[k].abs().max() > 0}
     print("surrogate + nufft", {k: f"{v:.1e}" for k, v in err.items()}, flush=True)
