@@ -63,8 +63,15 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
     adc = np.asarray(seq.adc, dtype=bool)
     spoil = np.asarray(seq.spoil, dtype=bool)
     rec = np.zeros(S, dtype=SEG_DTYPE)
-    rec["f"] = np.stack([dt, rf[:, 0], rf[:, 1], g[:, 0], g[:, 1], g[:, 2], b0, psi], 1).astype(np.float32)
-    rec["d"] = np.stack([dt, g[:, 0] * dt, g[:, 1] * dt, g[:, 2] * dt, b0 * dt], 1)
+    packed = np.empty((S, 8), dtype=np.float32)
+    for c, col in enumerate((dt, rf[:, 0], rf[:, 1], g[:, 0], g[:, 1], g[:, 2], b0, psi)):
+        packed[:, c] = col
+    rec["f"] = packed
+    moments = np.empty((S, 5))
+    moments[:, 0] = dt
+    moments[:, 1:4] = g * dt[:, None]
+    moments[:, 4] = b0 * dt
+    rec["d"] = moments
     flags = is_rf * RF | adc * ADC | spoil * SPOIL
     # cache hints: a free segment equal to the previous free one, an RF segment with the previous RF one's dt,
     # both within one dispatch
@@ -124,10 +131,10 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
         # and keep the factor) has the same float32 sample dt and length
         heads = np.flatnonzero(flags & (RUN | NUFFT))
         flags[heads] &= ~SAME_RUN
-        for prev, cur in zip(heads[:-1], heads[1:]):
-            if (chunk_id[cur] == chunk_id[prev] and rec["pad"][cur, 3] == rec["pad"][prev, 3]
-                    and rec["f"][cur, 0] == rec["f"][prev, 0]):
-                flags[cur] |= SAME_RUN
+        prev, cur = heads[:-1], heads[1:]
+        same = ((chunk_id[cur] == chunk_id[prev]) & (rec["pad"][cur, 3] == rec["pad"][prev, 3])
+                & (rec["f"][cur, 0] == rec["f"][prev, 0]))
+        flags[cur[same]] |= SAME_RUN
     mark_trains(flags, rec, is_rf, adc, spoil, chunk_id, stepped)
     # an occurrence's or a NUFFT run's other segments: the identity (the record is applied at its first segment)
     spans = [(a, k) for _, k, occ in pulses for a, _ in occ] + (list(nufft.runs) if nufft is not None else [])
@@ -139,8 +146,10 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
             rec["pad"][a, 4] = r  # the run's index (the adjoint's grid)
     rec["flags"] = flags
     w = -2 * math.pi * dt  # the rotation-vector factors, formed in float64
-    rec["w"] = np.stack([w, w * rf[:, 0], w * rf[:, 1], w * g[:, 0], w * g[:, 1], w * g[:, 2], w * b0,
-                         (w * rf[:, 0]) ** 2 + (w * rf[:, 1]) ** 2], 1).astype(np.float32)
+    for c, col in enumerate((w, w * rf[:, 0], w * rf[:, 1], w * g[:, 0], w * g[:, 1], w * g[:, 2], w * b0,
+                             (w * rf[:, 0]) ** 2 + (w * rf[:, 1]) ** 2)):
+        packed[:, c] = col
+    rec["w"] = packed
     for _, _, occ in pulses:  # a reused pulse's head is never stepped: its w_rf holds cos, sin of the RF phase
         for a, ph in occ:
             rec["w"][a, 1], rec["w"][a, 2] = math.cos(ph), math.sin(ph)
@@ -150,35 +159,38 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
     return rec
 
 
+def _stretches(ok: np.ndarray, link: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """(first, end) of every maximal stretch of ``ok`` segments, where ``link[s]`` says segment s continues the
+    stretch of segment s - 1 (it implies ok[s] and ok[s - 1])."""
+    head = ok & ~link
+    tail = ok.copy()
+    tail[:-1] &= ~link[1:]
+    return np.flatnonzero(head), np.flatnonzero(tail) + 1
+
+
+def _chained(ok: np.ndarray, also: np.ndarray) -> np.ndarray:
+    """link[s] = ok[s] and ok[s - 1] and ``also[s]`` (the condition on the pair (s - 1, s))."""
+    link = np.zeros(ok.size, dtype=bool)
+    link[1:] = ok[1:] & ok[:-1] & also[1:]
+    return link
+
+
 def mark_runs(flags, rec, key, is_rf, adc, spoil, chunk_id, stepped) -> None:
     """HB_RUN on the first of every maximal run (>= MIN_RUN) of consecutive free ADC segments with one key (dt, g,
     b0_offset), none spoiled, in one dispatch; its length in pad[3]. The run's first segment keeps its SAME_FREE
-    hint (the kernel's factor cache), the rest of the run is skipped; SAME_RUN when the previous run in the dispatch
-    has the same dt and length."""
-    S = len(flags)
+    hint (the kernel's factor cache); the rest of the run is skipped, so hints on it are cleared."""
     ok = ~is_rf & adc & ~spoil & stepped
-    same_next = np.zeros(S, dtype=bool)
-    same_next[:-1] = ok[:-1] & ok[1:] & _close(key[1:], key[:-1]).all(1) & (chunk_id[1:] == chunk_id[:-1])
-    prev_key = None
-    s = 0
-    while s < S:
-        if not ok[s]:
-            s += 1
-            continue
-        e = s + 1
-        while e < S and same_next[e - 1]:
-            e += 1
-        if e - s >= MIN_RUN:
-            flags[s] |= RUN
-            rec["pad"][s, 3] = e - s
-            this = (chunk_id[s], key[s, 0], e - s)
-            if prev_key is not None and prev_key[0] == this[0] and prev_key[2] == this[2] and _close(
-                    np.array([prev_key[1]]), np.array([this[1]])).all():
-                flags[s] |= SAME_RUN
-            prev_key = this
-            # the run's inner segments are not stepped: hints that name them would refer to nothing
-            flags[s + 1 : e] &= ~(SAME_FREE | NEXT_FREE)
-        s = e
+    pair = np.zeros(ok.size, dtype=bool)
+    pair[1:] = _close(key[1:], key[:-1]).all(1) & (chunk_id[1:] == chunk_id[:-1])
+    first, end = _stretches(ok, _chained(ok, pair))
+    keep = end - first >= MIN_RUN
+    first, end = first[keep], end[keep]
+    flags[first] |= RUN
+    rec["pad"][first, 3] = end - first
+    inner = np.zeros(ok.size + 1, dtype=np.int64)
+    np.add.at(inner, first + 1, 1)
+    np.add.at(inner, end, -1)
+    flags[np.cumsum(inner[:-1]) > 0] &= ~(SAME_FREE | NEXT_FREE)
 
 
 def merge_free(flags, rec, is_rf, adc, spoil, chunk_id, stepped) -> None:
@@ -188,54 +200,72 @@ def merge_free(flags, rec, is_rf, adc, spoil, chunk_id, stepped) -> None:
     one free segment of the summed duration, gradient moment and b0_offset area. Only for forwards without
     checkpoints (the adjoint steps and differentiates every segment)."""
     ok = stepped & ~is_rf & ~adc
-    S = len(ok)
-    s = 0
-    while s < S:
-        if not ok[s]:
-            s += 1
-            continue
-        e = s + 1
-        while e < S and ok[e] and chunk_id[e] == chunk_id[s] and not spoil[e - 1]:
-            e += 1
-        if e < S and stepped[e] and not is_rf[e] and adc[e] and chunk_id[e] == chunk_id[s] and not spoil[e - 1]:
-            e += 1  # a sampled free segment ends the run
-        if e - s >= 2:
-            d = rec["d"][s:e].sum(0)
-            rec["d"][s] = d
-            rec["f"][s, 0] = np.float32(d[0])
-            flags[s] |= MERGED | (flags[e - 1] & (SPOIL | ADC))
-            rec["pad"][s, 3] = e - s
-            stepped[s + 1 : e] = False
-        s = e
+    S = ok.size
+    same_chunk = np.zeros(S, dtype=bool)
+    same_chunk[1:] = chunk_id[1:] == chunk_id[:-1]
+    unspoiled_before = np.zeros(S, dtype=bool)
+    unspoiled_before[1:] = ~spoil[:-1]
+    first, end = _stretches(ok, _chained(ok, same_chunk & unspoiled_before))
+    # a sampled free segment right after a stretch (same dispatch, no spoil between) is its last member
+    e = np.minimum(end, S - 1)
+    takes = (end < S) & stepped[e] & ~is_rf[e] & adc[e] & same_chunk[e] & unspoiled_before[e]
+    end = end + takes
+    keep = end - first >= 2
+    first, end = first[keep], end[keep]
+    if first.size == 0:
+        return
+    bounds = np.stack([first, end], 1).reshape(-1)
+    sums = np.add.reduceat(rec["d"], bounds[bounds < S], axis=0)[::2]
+    rec["d"][first] = sums
+    rec["f"][first, 0] = sums[:, 0].astype(np.float32)
+    flags[first] |= MERGED | (flags[end - 1] & (SPOIL | ADC))
+    rec["pad"][first, 3] = end - first
+    covered = np.zeros(S + 1, dtype=np.int64)
+    np.add.at(covered, first + 1, 1)
+    np.add.at(covered, end, -1)
+    stepped[np.cumsum(covered[:-1]) > 0] = False
 
 
 def zap_spoiled(flags, rec, is_rf, spoil, chunk_id, stepped) -> np.ndarray:
     """SPOILED_FREE on the stepped free records without ADC (single segments or merged runs) whose transverse
-    magnetization never reaches a sample: spoiled at their end, or starting at Mxy = 0 (right after a spoil, or right
-    after another such record). Only Mz matters there: the kernel relaxes it over the record and zeroes Mxy (no
-    precession factors, so such records neither fill nor use the cache). A record that starts at Mxy = 0 right after
-    another one in the same dispatch is folded into it (MERGED: relaxation over the summed duration). Returns the
-    marked records."""
-    S = len(flags)
+    magnetization never reaches a sample: spoiled at their end, or starting at Mxy = 0 (right after a spoil). Only
+    Mz matters there: the kernel relaxes it over the record and zeroes Mxy (no precession factors, so such records
+    neither fill nor use the cache). The free records that follow such a record back to back in the same dispatch
+    also start at Mxy = 0 and are folded into it (MERGED: relaxation over the summed duration). Returns the marked
+    records."""
+    S = flags.size
+    r = np.flatnonzero(stepped)  # records, in order
     zapped = np.zeros(S, dtype=bool)
-    length = lambda s: int(rec["pad"][s, 3]) if flags[s] & MERGED else 1
-    open_, open_end = -1, -1
-    for s in np.flatnonzero(stepped):
-        s = int(s)
-        free = not is_rf[s] and not flags[s] & ADC
-        if free and s == open_end and chunk_id[s] == chunk_id[open_]:
-            L = length(s)
-            stepped[s] = False
-            rec["d"][open_] += rec["d"][s]  # the sums of the folded record's segments (only the duration is used)
-            rec["f"][open_, 0] = np.float32(rec["d"][open_, 0])
-            open_end = s + L
-            flags[open_] |= MERGED
-            rec["pad"][open_, 3] = open_end - open_
-        elif free and (flags[s] & SPOIL or (s > 0 and spoil[s - 1])):
-            zapped[s] = True
-            open_, open_end = s, s + length(s)
-        else:
-            open_, open_end = -1, -1
+    if r.size == 0:
+        return zapped
+    length = np.where(flags[r] & MERGED, rec["pad"][r, 3], 1)
+    end = r + length
+    free = ~is_rf[r] & (flags[r] & ADC == 0)
+    after_spoil = np.concatenate([[False], spoil[:-1]])[r]
+    can_open = free & ((flags[r] & SPOIL != 0) | after_spoil)
+    # records linked to the previous one: free, starting where it ends, in its dispatch
+    link = np.zeros(r.size, dtype=bool)
+    link[1:] = free[1:] & (r[1:] == end[:-1]) & (chunk_id[r[1:]] == chunk_id[r[:-1]])
+    # in a group of linked records, a chain opens at the first record that can open one and runs to the group's end
+    group = np.cumsum(~link) - 1
+    group_first = np.flatnonzero(~link)
+    candidate = np.where(can_open, np.arange(r.size), r.size)
+    opens_at = np.minimum.reduceat(candidate, group_first)[group]
+    head = np.arange(r.size) == opens_at
+    folded = np.arange(r.size) > opens_at
+    if folded.any():
+        h = r[opens_at[folded]]  # each folded record's chain head (a segment index)
+        d_add = np.zeros((S, rec["d"].shape[1]))
+        np.add.at(d_add, h, rec["d"][r[folded]])
+        heads_with = np.unique(h)
+        rec["d"][heads_with] += d_add[heads_with]
+        rec["f"][heads_with, 0] = rec["d"][heads_with, 0].astype(np.float32)
+        chain_end = np.zeros(S, dtype=np.int64)
+        np.maximum.at(chain_end, h, end[folded])
+        flags[heads_with] |= MERGED
+        rec["pad"][heads_with, 3] = chain_end[heads_with] - heads_with
+        stepped[r[folded]] = False
+    zapped[r[head]] = True
     flags[zapped] |= SPOILED_FREE
     return zapped
 
@@ -245,19 +275,12 @@ def mark_trains(flags, rec, is_rf, adc, spoil, chunk_id, stepped) -> None:
     spoil in one dispatch, its length in pad[3]. The forward kernel steps a train in a loop of its own, with the same
     arithmetic as segment by segment (so checkpointed forwards may use trains too); the segments keep their hints."""
     ok = is_rf & ~adc & ~spoil & stepped
-    S = len(ok)
-    s = 0
-    while s < S:
-        if not ok[s]:
-            s += 1
-            continue
-        e = s + 1
-        while e < S and ok[e] and chunk_id[e] == chunk_id[s]:
-            e += 1
-        if e - s >= MIN_TRAIN:
-            flags[s] |= TRAIN
-            rec["pad"][s, 3] = e - s
-        s = e
+    same_chunk = np.zeros(ok.size, dtype=bool)
+    same_chunk[1:] = chunk_id[1:] == chunk_id[:-1]
+    first, end = _stretches(ok, _chained(ok, same_chunk))
+    keep = end - first >= MIN_TRAIN
+    flags[first[keep]] |= TRAIN
+    rec["pad"][first[keep], 3] = (end - first)[keep]
 
 
 def dispatch_starts(seq: Sequence, chunk: int, rows_cap: int, spans=(), rows_w=None, heads=(),

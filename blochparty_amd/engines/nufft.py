@@ -50,37 +50,38 @@ TOL = 1e-9
 
 
 def readout_runs(seq: Sequence) -> list[tuple[int, int]]:
-    """(first segment, samples) of every run of >= MIN_SAMPLES consecutive ADC segments without RF, with one dt,
-    gradient and b0_offset (to 1e-9), a spoil only on the last, of the most common (samples, dt). A builder
-    readout's first sample sits half a dwell in, so its run starts at the second sample."""
+    """(first segment, samples) of the readout runs a NUFFT takes: stretches of >= MIN_SAMPLES ADC segments without
+    RF in which every segment has the previous one's dt, gradient and b0_offset (to TOL) and no segment but the last
+    is spoiled; of those, only the kind (samples, dt) that occurs most often (ties: the earliest). A readout whose
+    first sample ends a half-dwell segment starts its run at the second sample, where the dt becomes constant."""
     adc, rf, sp = (np.asarray(f, dtype=bool) for f in (seq.adc, seq.is_rf, seq.spoil))
+    S = len(adc)
+    if S == 0:
+        return []
     dt = seq.dt.detach().double().cpu().numpy()
     g = seq.grad.detach().double().cpu().numpy()
     b0 = np.zeros_like(dt) if seq.b0_offset is None else seq.b0_offset.detach().double().cpu().numpy()
-    ok = adc & ~rf
-    same = np.zeros(len(dt), dtype=bool)
-    tol_g = TOL * max(float(np.abs(g).max()) if len(g) else 0.0, 1.0)
-    same[1:] = (ok[1:] & ok[:-1] & ~sp[:-1] & (np.abs(dt[1:] - dt[:-1]) <= TOL * dt[1:])
-                & (np.abs(g[1:] - g[:-1]) <= tol_g).all(1) & (np.abs(b0[1:] - b0[:-1]) <= TOL))
-    runs = []
-    s = 0
-    S = len(dt)
-    while s < S:
-        if not ok[s]:
-            s += 1
-            continue
-        e = s + 1
-        while e < S and same[e]:
-            e += 1
-        if e - s >= MIN_SAMPLES:
-            runs.append((s, e - s))
-        s = e
-    if not runs:
+    sample = adc & ~rf
+    g_scale = TOL * max(float(np.abs(g).max()), 1.0)
+    # link[s]: segment s continues the stretch that segment s - 1 is in
+    link = np.zeros(S, dtype=bool)
+    link[1:] = (sample[1:] & sample[:-1] & ~sp[:-1] & (np.abs(np.diff(dt)) <= TOL * dt[1:])
+                & (np.abs(np.diff(g, axis=0)) <= g_scale).all(1) & (np.abs(np.diff(b0)) <= TOL))
+    first = np.flatnonzero(sample & ~link)
+    stop = np.flatnonzero(~np.append(link[1:], False))  # the last segment of every stretch (and of non-samples)
+    last = stop[np.searchsorted(stop, first)]
+    length = last + 1 - first
+    long_ = length >= MIN_SAMPLES
+    first, length = first[long_], length[long_]
+    if first.size == 0:
         return []
-    kinds: dict[tuple, list] = {}
-    for a, k in runs:
-        kinds.setdefault((k, round(float(dt[a]) * 1e12)), []).append((a, k))
-    return max(kinds.values(), key=len)
+    kind = np.stack([length, np.round(dt[first] * 1e12)], axis=1)
+    kinds, inverse, counts = np.unique(kind, axis=0, return_inverse=True, return_counts=True)
+    inverse = inverse.reshape(-1)
+    best = counts.max()
+    chosen = inverse[np.flatnonzero(counts[inverse] == best)[0]]  # the earliest run of a most common kind
+    keep = inverse == chosen
+    return list(zip(first[keep].tolist(), length[keep].tolist()))
 
 
 def es_kernel(z: np.ndarray, beta: float) -> np.ndarray:

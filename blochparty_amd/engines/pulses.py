@@ -3,10 +3,19 @@ phase, is composed once per spin into its affine map m -> m + D m + v, in float6
 occurrence conjugated by the occurrence's phase: an RF phase phi is a rotation of the frame about z, so the
 occurrence maps m to m + Rz(phi) (D Rz(-phi) m + v).
 
-Reference: ``src/blochparty/engines/gpu/driver.py:109-164`` (grouping: runs of >= 2 consecutive RF segments without
-ADC, a spoil only on the last, with an earlier run's dt, gradient, b0_offset and flags and its RF times e^{i phase}
-to ``tol`` of the peak, phase by least squares) and ``cuda/pulse.cu.in`` + ``kernels.cu.in:141-165`` (maps and their
-application). The reference's are forward only; here the adjoint applies them too (``engines/adjoint.py``).
+Reference behaviour: ``src/blochparty/engines/gpu/driver.py:109-164`` (which runs count as one pulse) and
+``cuda/pulse.cu.in`` + ``kernels.cu.in:141-165`` (maps and their application). The reference's are forward only;
+here the adjoint applies them too (``engines/adjoint.py``).
+
+What counts as one pulse: maximal runs of >= 2 consecutive RF segments without ADC (a spoiled segment ends its run)
+that have the same length, and whose dt, gradient, b0_offset and spoil flags equal a first run's, and whose RF equals
+the first run's times e^{i phi} for one phase phi, each to ``tol`` of the larger peak; phi is the least-squares phase.
+
+How (``reused_pulses``): runs are found with array operations, then, per run length, grouped by prototype
+elimination: the earliest ungrouped run is a prototype, all remaining runs of its length are compared with it at
+once (one matrix-vector product gives every candidate's least-squares phase, one reduction each comparison), the
+matches join it, and the next ungrouped run is the next prototype. A pulse's occurrences are in time order, the
+first (the prototype) with phase 0; pulses come in the order of their first occurrence.
 """
 
 from __future__ import annotations
@@ -17,63 +26,65 @@ from ..sequence import Sequence
 
 
 def rf_runs(is_rf: np.ndarray, adc: np.ndarray, spoil: np.ndarray) -> list[tuple[int, int]]:
-    """(first, length) of every maximal run of RF segments without ADC, cut after a spoiled segment."""
-    runs, start = [], None
-    live = is_rf & ~adc
-    for s in range(len(is_rf)):
-        if live[s]:
-            if start is None:
-                start = s
-            if spoil[s]:
-                runs.append((start, s + 1 - start))
-                start = None
-        elif start is not None:
-            runs.append((start, s - start))
-            start = None
-    if start is not None:
-        runs.append((start, len(is_rf) - start))
-    return runs
+    """(first, length) of every maximal run of RF segments without ADC; a spoiled segment is the last of its run."""
+    live = np.asarray(is_rf, dtype=bool) & ~np.asarray(adc, dtype=bool)
+    if not live.any():
+        return []
+    after_spoil = np.zeros_like(live)
+    after_spoil[1:] = np.asarray(spoil, dtype=bool)[:-1]
+    prev = np.concatenate([[False], live[:-1]])
+    starts = np.flatnonzero(live & (~prev | after_spoil))
+    # a run ends before the next start or at the first segment that is not live
+    dead = np.append(np.flatnonzero(~live), live.size)  # with a sentinel past the end
+    nxt_dead = dead[np.searchsorted(dead, starts)]
+    nxt_start = np.append(starts[1:], live.size)
+    ends = np.minimum(nxt_dead, nxt_start)
+    return list(zip(starts.tolist(), (ends - starts).tolist()))
+
+
+def _within(diff_peak: np.ndarray, peak_a: np.ndarray, peak_b: np.ndarray, tol: float) -> np.ndarray:
+    return diff_peak <= tol * np.maximum(np.maximum(peak_a, peak_b), 1e-300)
 
 
 def reused_pulses(seq: Sequence, tol: float = 1e-6) -> list[tuple[int, int, list[tuple[int, float]]]]:
-    """[(first segment, length, [(occurrence start, phase [rad]), ...])] of every pulse that occurs at least twice,
-    most segments saved first."""
-    is_rf = np.asarray(seq.is_rf, dtype=bool)
-    adc = np.asarray(seq.adc, dtype=bool)
-    spoil = np.asarray(seq.spoil, dtype=bool)
-    S = len(is_rf)
+    """[(first segment, length, [(occurrence start, phase [rad]), ...])] of every pulse that occurs at least twice
+    (the module's rules), in the order of first occurrence."""
+    is_rf, adc, spoil = (np.asarray(f, dtype=bool) for f in (seq.is_rf, seq.adc, seq.spoil))
     dt = seq.dt.detach().double().cpu().numpy()
     rf = seq.rf.detach().double().cpu().numpy()
-    z = rf[:, 0] + 1j * rf[:, 1]
-    g = seq.grad.detach().double().cpu().numpy()
-    b0 = np.zeros(S) if seq.b0_offset is None else seq.b0_offset.detach().double().cpu().numpy()
-
-    def close(x, y):
-        return np.abs(x - y).max() <= tol * max(np.abs(x).max(), np.abs(y).max(), 1e-300)
-
-    groups: list[tuple[int, int, list]] = []
-    by_shape: dict[tuple, list[int]] = {}
-    for a, k in rf_runs(is_rf, adc, spoil):
-        if k < 2:
-            continue
-        cur = slice(a, a + k)
-        key = (k, round(float(dt[cur].sum()) * 1e9))
-        for gi in by_shape.get(key, ()):
-            first = groups[gi][0]
-            ref = slice(first, first + k)
-            if not (np.array_equal(spoil[cur], spoil[ref]) and close(dt[cur], dt[ref]) and close(g[cur], g[ref])
-                    and close(b0[cur], b0[ref])):
+    z_all = rf[:, 0] + 1j * rf[:, 1]
+    g_all = seq.grad.detach().double().cpu().numpy()
+    b0_all = np.zeros(len(dt)) if seq.b0_offset is None else seq.b0_offset.detach().double().cpu().numpy()
+    runs = [(a, k) for a, k in rf_runs(is_rf, adc, spoil) if k >= 2]
+    by_len: dict[int, list[int]] = {}
+    for a, k in runs:
+        by_len.setdefault(k, []).append(a)
+    found = []
+    for k, starts in by_len.items():
+        idx = np.asarray(starts)[:, None] + np.arange(k)[None, :]  # (R, k) segment indices
+        Z, D, G, B, P = z_all[idx], dt[idx], g_all[idx], b0_all[idx], spoil[idx]
+        peak = {name: np.abs(x).reshape(len(starts), -1).max(1) for name, x in (("z", Z), ("d", D), ("g", G),
+                                                                                  ("b", B))}
+        open_ = np.ones(len(starts), dtype=bool)
+        for p in range(len(starts)):
+            if not open_[p]:
                 continue
-            c = np.vdot(z[ref], z[cur])  # least-squares phase of this run against the first
-            ph = float(np.angle(c)) if c != 0 else 0.0
-            if close(z[cur], z[ref] * np.exp(1j * ph)):
-                groups[gi][2].append((a, ph))
-                break
-        else:
-            by_shape.setdefault(key, []).append(len(groups))
-            groups.append((a, k, [(a, 0.0)]))
-    groups = [p for p in groups if len(p[2]) > 1]
-    return sorted(groups, key=lambda p: -(len(p[2]) - 1) * p[1])
+            cand = np.flatnonzero(open_)
+            c = Z[cand] @ np.conj(Z[p])  # least-squares phase of each candidate against the prototype
+            phi = np.where(c != 0, np.angle(c), 0.0)
+            fit = Z[p][None, :] * np.exp(1j * phi)[:, None]
+            ok = (P[cand] == P[p]).all(1)
+            for name, X in (("d", D), ("g", G), ("b", B)):
+                gap = np.abs(X[cand] - X[p][None]).reshape(cand.size, -1).max(1)
+                ok &= _within(gap, peak[name][cand], peak[name][p], tol)
+            ok &= _within(np.abs(Z[cand] - fit).max(1), peak["z"][cand], peak["z"][p], tol)
+            ok[cand == p] = True
+            members = cand[ok]
+            open_[members] = False
+            if members.size > 1:
+                occ = [(int(starts[m]), 0.0 if m == p else float(phi[np.searchsorted(cand, m)])) for m in members]
+                found.append((int(starts[p]), k, occ))
+    return sorted(found, key=lambda pulse: pulse[0])
 
 
 def pulse_table(seq: Sequence, pulses) -> tuple[np.ndarray, np.ndarray]:

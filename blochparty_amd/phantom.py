@@ -64,17 +64,45 @@ TISSUES = {
 LABELS = ["background", *TISSUES]
 
 
-def _per_axis_counts(spins_per_voxel, ndim: int) -> tuple[int, ...]:
-    if isinstance(spins_per_voxel, int):
-        side = round(spins_per_voxel ** (1 / ndim))
-        if side**ndim != spins_per_voxel:
-            raise ValueError(f"spins_per_voxel={spins_per_voxel} is not a perfect {ndim}-th power; "
-                             f"give spins per side along each axis as a tuple instead")
-        return (side,) * ndim
-    counts = tuple(int(k) for k in spins_per_voxel)
-    if len(counts) != ndim or min(counts) < 1:
-        raise ValueError(f"spins_per_voxel needs {ndim} positive counts (map axis order), got {spins_per_voxel}")
-    return counts
+def _axis_counts(spins_per_voxel, ndim: int) -> tuple[int, ...]:
+    """Isochromats per voxel along each map axis: a tuple as given, or an int that is a perfect ndim-th power."""
+    if not isinstance(spins_per_voxel, int):
+        counts = tuple(int(k) for k in spins_per_voxel)
+        if len(counts) != ndim or min(counts) < 1:
+            raise ValueError(f"spins_per_voxel needs {ndim} positive counts (map axis order), got {spins_per_voxel}")
+        return counts
+    side = round(spins_per_voxel ** (1 / ndim))
+    if side**ndim != spins_per_voxel:
+        raise ValueError(f"spins_per_voxel={spins_per_voxel} is not a perfect {ndim}-th power; "
+                         f"give spins per side along each axis as a tuple instead")
+    return (side,) * ndim
+
+
+def _subcell_offsets(counts: tuple[int, ...], n_vox: int, stratified: bool, gen, dtype) -> Tensor:
+    """(n_vox, k, 3) offsets from the voxel centre in voxel units, (x, y, z) order: sub-cell c of count m along a map
+    axis sits at (c + u) / m - 1/2, u = 1/2 (the sub-cell's centre) or, stratified, a seeded uniform draw per spin
+    and axis. Sub-cells are numbered with the last map axis fastest."""
+    ndim, k = len(counts), math.prod(counts)
+    index = torch.stack(torch.meshgrid(*(torch.arange(m, dtype=dtype) for m in counts), indexing="ij"), -1)
+    index = index.reshape(k, ndim)
+    u = (torch.rand(n_vox, k, ndim, generator=gen, dtype=dtype) if stratified
+         else torch.full((n_vox, k, ndim), 0.5, dtype=dtype))
+    per_axis = (index + u) / torch.tensor(counts, dtype=dtype) - 0.5  # map axis order (z, y, x) / (y, x)
+    out = torch.zeros(n_vox, k, 3, dtype=dtype)
+    out[..., :ndim] = per_axis.flip(-1)
+    return out
+
+
+def _lorentzian_offsets(hz_half_width: Tensor, k: int, quantiles: bool, gen, dtype) -> Tensor:
+    """(n_vox * k,) off-resonance offsets of k isochromats per voxel drawn from a Cauchy (Lorentzian) line of the
+    given half width: at the k equal-probability quantile midpoints in a seeded random order per voxel, or at
+    independent uniform probabilities. F^-1(p) = w tan(pi (p - 1/2))."""
+    n_vox = hz_half_width.shape[0]
+    draw = torch.rand(n_vox, k, generator=gen, dtype=dtype)
+    if quantiles:
+        rank = draw.argsort(dim=1).argsort(dim=1)
+        draw = (rank.to(dtype) + 0.5) / k
+    return (hz_half_width[:, None] * torch.tan(math.pi * (draw - 0.5))).reshape(-1)
 
 
 def phantom_from_maps(pd: Tensor, t1: Tensor, t2: Tensor, fov: float, db0: Tensor | None = None, *,
@@ -82,63 +110,64 @@ def phantom_from_maps(pd: Tensor, t1: Tensor, t2: Tensor, fov: float, db0: Tenso
                       t2prime: Tensor | None = None, t2prime_sampling: str = "quantile", b1: Tensor | None = None,
                       fat_fraction: Tensor | None = None, field_strength: float = 3.0, fat_t1=None, fat_t2=None,
                       fat_peaks=FAT_PEAKS) -> Phantom:
-    """Spins from (z, y, x)-ordered maps; PD == 0 voxels are dropped. ``spins_per_voxel`` sub-voxel isochromats
-    carry PD / K each, on a regular sub-grid or (``jitter="stratified"``) at a seeded uniform point per sub-cell.
-    ``t2prime`` adds Lorentzian (Cauchy-quantile) offsets per voxel; ``fat_fraction`` adds multi-peak fat spins."""
-    mask = pd > 0
-    pos = grid_positions(pd.shape, fov, dtype=pd.dtype)[mask]
-    db0 = torch.zeros_like(pd) if db0 is None else db0
-    pick = lambda m: None if m is None else m[mask]
-    pd_v, t1_v, t2_v, db0_v, b1_v = pd[mask], t1[mask], t2[mask], db0[mask], pick(b1)
-    counts = _per_axis_counts(spins_per_voxel, pd.ndim)
-    k = math.prod(counts)
+    """Spins from maps ordered (z, y, x) or (y, x) on ``fov``; voxels with PD 0 hold none.
+
+    - ``spins_per_voxel`` isochromats per voxel (int or per-axis counts), each with PD / k, spin (voxel v, sub-cell
+      c) at index v k + c; on the sub-cells' centres, or (``jitter="stratified"``) at a seeded uniform point in
+      each sub-cell;
+    - ``t2prime``: each voxel's isochromats get Lorentzian off-resonance offsets of half width 1 / (2 pi T2'),
+      ``t2prime_sampling`` "quantile" (stratified, shuffled) or "random";
+    - ``fat_fraction``: the water spins keep PD (1 - ff); after them come the fat spins of every peak of
+      ``fat_peaks`` (ppm, relative amplitude) in turn, PD ff x amplitude, shifted by ppm x gamma_bar x B0, with fat
+      T1 / T2 (``fat_t1``, ``fat_t2``, default the ``TISSUES`` fat values).
+    """
     if jitter not in (None, "stratified"):
         raise ValueError(f"jitter must be None or 'stratified', got {jitter!r}")
     if t2prime_sampling not in ("quantile", "random"):
         raise ValueError(f"t2prime_sampling must be 'quantile' or 'random', got {t2prime_sampling!r}")
-    if k == 1 and jitter is None and t2prime is None and fat_fraction is None:
-        return Phantom(pos=pos, pd=pd_v, t1=t1_v, t2=t2_v, db0=db0_v, b1=b1_v)
+    dtype = pd.dtype
+    occupied = pd > 0
+    centres = grid_positions(pd.shape, fov, dtype=dtype)[occupied]
+    per_voxel = lambda m: None if m is None else m[occupied]
+    water = {"pd": pd[occupied], "t1": t1[occupied], "t2": t2[occupied],
+             "db0": torch.zeros_like(pd)[occupied] if db0 is None else db0[occupied], "b1": per_voxel(b1)}
+    counts = _axis_counts(spins_per_voxel, pd.ndim)
+    k = math.prod(counts)
     gen = torch.Generator().manual_seed(seed)
-    n_vox = pos.shape[0]
     if k > 1 or jitter is not None:
-        frac = torch.zeros(n_vox, k, 3, dtype=pd.dtype)
-        cells = torch.cartesian_prod(*(torch.arange(c) for c in counts)).reshape(k, len(counts))
-        u = (torch.rand(n_vox, k, len(counts), generator=gen, dtype=pd.dtype) if jitter == "stratified"
-             else torch.full((n_vox, k, len(counts)), 0.5, dtype=pd.dtype))
-        sides = torch.tensor(counts, dtype=pd.dtype)
-        frac[..., : len(counts)] = ((cells + u) / sides - 0.5).flip(-1)  # (z, y, x) map axes -> (x, y, z)
-        voxel = torch.tensor([fov / n for n in reversed(pd.shape)] + [0.0] * (3 - pd.ndim), dtype=pd.dtype)
-        pos = (pos[:, None, :] + frac * voxel).reshape(-1, 3)
-    rep = lambda v: None if v is None else v.repeat_interleave(k)
-    pd_s, t1_s, t2_s, db0_s, b1_s = rep(pd_v / k if k > 1 else pd_v), rep(t1_v), rep(t2_v), rep(db0_v), rep(b1_v)
+        voxel = torch.tensor([fov / m for m in reversed(pd.shape)] + [0.0] * (3 - pd.ndim), dtype=dtype)
+        offsets = _subcell_offsets(counts, centres.shape[0], jitter == "stratified", gen, dtype)
+        pos = (centres[:, None, :] + offsets * voxel).reshape(-1, 3)
+    else:
+        pos = centres
+    spins = {name: None if v is None else v.repeat_interleave(k) for name, v in water.items()}
+    if k > 1:
+        spins["pd"] = spins["pd"] / k
     if t2prime is not None:
-        if t2prime_sampling == "quantile":
-            q = (torch.arange(k, dtype=pd.dtype) + 0.5) / k
-            q = q[torch.argsort(torch.rand(n_vox, k, generator=gen), dim=1)]
-        else:
-            q = torch.rand(n_vox, k, generator=gen, dtype=pd.dtype)
-        half_width = 1.0 / (2 * math.pi * t2prime[mask])
-        db0_s = db0_s + (half_width[:, None] * torch.tan(math.pi * (q - 0.5))).reshape(-1)
+        half_width = 1.0 / (2 * math.pi * t2prime[occupied])
+        spins["db0"] = spins["db0"] + _lorentzian_offsets(half_width, k, t2prime_sampling == "quantile", gen, dtype)
     if fat_fraction is None:
-        return Phantom(pos=pos, pd=pd_s, t1=t1_s, t2=t2_s, db0=db0_s, b1=b1_s)
-    ff = rep(fat_fraction[mask])
-    fat = ff > 0
-    peaks = torch.tensor(fat_peaks, dtype=pd.dtype).reshape(-1, 2)
-    shift_hz, amp = peaks[:, 0] * (1e-6 * GAMMA_BAR * field_strength), peaks[:, 1] / peaks[:, 1].sum()
-    n_peaks, n_fat = peaks.shape[0], int(fat.sum())
+        return Phantom(pos=pos, **spins)
+    ff = fat_fraction[occupied].repeat_interleave(k)
+    fatty = ff > 0
+    table = torch.tensor(fat_peaks, dtype=dtype).reshape(-1, 2)
+    shift = table[:, 0] * (1e-6 * GAMMA_BAR * field_strength)
+    weight = table[:, 1] / table[:, 1].sum()
+    n_peaks = table.shape[0]
 
-    def fat_param(v, default):
-        v = torch.as_tensor(default if v is None else v, dtype=pd.dtype)
-        return v.expand(n_fat) if v.ndim == 0 else rep(v[mask])[fat]
+    def fat_value(given, tissue_default):
+        v = torch.as_tensor(tissue_default if given is None else given, dtype=dtype)
+        v = v.expand(int(fatty.sum())) if v.ndim == 0 else v[occupied].repeat_interleave(k)[fatty]
+        return v.repeat(n_peaks)
 
-    tile = lambda v: v.repeat(n_peaks)
+    fat_pd = (spins["pd"] * ff)[fatty]
     return Phantom(
-        pos=torch.cat([pos, pos[fat].repeat(n_peaks, 1)]),
-        pd=torch.cat([pd_s * (1 - ff), ((pd_s * ff)[fat][None, :] * amp[:, None]).reshape(-1)]),
-        t1=torch.cat([t1_s, tile(fat_param(fat_t1, TISSUES["fat"][1]))]),
-        t2=torch.cat([t2_s, tile(fat_param(fat_t2, TISSUES["fat"][2]))]),
-        db0=torch.cat([db0_s, (db0_s[fat][None, :] + shift_hz[:, None]).reshape(-1)]),
-        b1=None if b1_s is None else torch.cat([b1_s, tile(b1_s[fat])]),
+        pos=torch.cat([pos, pos[fatty].repeat(n_peaks, 1)]),
+        pd=torch.cat([spins["pd"] * (1 - ff), (weight[:, None] * fat_pd[None, :]).reshape(-1)]),
+        t1=torch.cat([spins["t1"], fat_value(fat_t1, TISSUES["fat"][1])]),
+        t2=torch.cat([spins["t2"], fat_value(fat_t2, TISSUES["fat"][2])]),
+        db0=torch.cat([spins["db0"], (spins["db0"][fatty][None, :] + shift[:, None]).reshape(-1)]),
+        b1=None if spins["b1"] is None else torch.cat([spins["b1"], spins["b1"][fatty].repeat(n_peaks)]),
     )
 
 

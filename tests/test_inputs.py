@@ -77,3 +77,58 @@ def test_workload_sizes():
     assert (len(ph), len(seq), seq.n_adc, coils.shape) == (2039, 4752, 4096, (2039, 8))
     seq3 = W.gre3d_sequence(8)
     assert sum(seq3.is_rf) == 200 * 64 and seq3.n_adc == 8**3
+
+
+def test_phantom_subvoxel_options_by_their_invariants():
+    """Sub-voxel isochromats, T2' offsets and fat peaks, each checked against what it must preserve."""
+    m = bp.shepp_logan_maps(6, ndim=2)
+    base = bp.phantom_from_maps(m["pd"], m["t1"], m["t2"], 0.22)
+    k = 4
+    jit = bp.phantom_from_maps(m["pd"], m["t1"], m["t2"], 0.22, spins_per_voxel=(2, 2), jitter="stratified", seed=3)
+    # each stratified spin lies in its own quarter of its voxel; the voxel's PD is shared out exactly
+    off = (jit.pos[:, :2].reshape(-1, k, 2) - base.pos[:, None, :2]) / (0.22 / 6)
+    assert off.abs().max() < 0.5
+    cell = torch.tensor([[0, 0], [1, 0], [0, 1], [1, 1]], dtype=torch.float64)  # (x, y) halves, x fastest
+    assert torch.all((off >= 0.5 * cell - 0.5) & (off <= 0.5 * cell))
+    torch.testing.assert_close(jit.pd.reshape(-1, k).sum(1), base.pd)
+    # T2': per voxel the k offsets are the Cauchy quantile midpoints of half width 1 / (2 pi T2')
+    t2p = torch.full_like(m["pd"], 0.05)
+    lor = bp.phantom_from_maps(m["pd"], m["t1"], m["t2"], 0.22, spins_per_voxel=(2, 2), t2prime=t2p, seed=1)
+    w = 1 / (2 * torch.pi * 0.05)
+    q = (torch.arange(k, dtype=torch.float64) + 0.5) / k
+    want = w * torch.tan(torch.pi * (q - 0.5))
+    torch.testing.assert_close(lor.db0.reshape(-1, k).sort(1).values, want.expand(len(base), k))
+    # fat: water keeps PD (1 - ff); the peaks carry ff x PD by their relative amplitudes, shifted by ppm x B0
+    ff = torch.where(m["label"] == bp.LABELS.index("fat"), 0.3, 0.0).to(torch.float64)
+    fat = bp.phantom_from_maps(m["pd"], m["t1"], m["t2"], 0.22, fat_fraction=ff)
+    n, n_fat = len(base), int((ff[m["pd"] > 0] > 0).sum())
+    assert len(fat) == n + 6 * n_fat
+    torch.testing.assert_close(fat.pd.sum(), base.pd.sum())
+    shifts = fat.db0[n:].reshape(6, n_fat)[:, 0]
+    ppm = torch.tensor([p for p, _ in bp.phantom.FAT_PEAKS], dtype=torch.float64)
+    torch.testing.assert_close(shifts, ppm * 1e-6 * bp.phantom.GAMMA_BAR * 3.0)
+    assert torch.all(fat.t1[n:] == bp.TISSUES["fat"][1])
+
+
+def test_block_split_by_hand():
+    """One block worked out by hand: an x lobe rising over 30 us (cut at the 10 us raster), flat for 40 us, falling
+    over 20 us, with 4 ADC samples of 10 us dwell starting at 30 us and an RF pulse of 3 x 20 us samples from 100 us
+    (after the dead time), then the ringdown. Every segment's gradient is the lobe's level at its midpoint."""
+    from blochparty_amd.builder import ADC, RFPulse, Trapezoid
+
+    sysm = bp.System()
+    amp = 1000.0
+    lobe = Trapezoid(torch.tensor([amp, 0.0, 0.0], dtype=torch.float64), 30e-6, 40e-6, 20e-6)
+    rf = RFPulse(torch.tensor([[1.0, 0.0], [2.0, 0.0], [3.0, 0.0]], dtype=torch.float64), 20e-6, 100e-6, 30e-6)
+    seq = bp.SequenceBuilder(system=sysm).block(lobe, rf=rf, adc=ADC(4, 10e-6, 30e-6, phase=0.25)).build()
+    us = 1e-6
+    knots = np.array([0, 10, 20, 30, 35, 45, 55, 65, 70, 80, 90, 100, 120, 140, 160, 190]) * us
+    np.testing.assert_allclose(seq.dt.numpy(), np.diff(knots), rtol=1e-12, atol=1e-18)
+    mid = 0.5 * (knots[1:] + knots[:-1])
+    level = np.clip(np.minimum(mid / (30 * us), (90 * us - mid) / (20 * us)), 0, 1)
+    np.testing.assert_allclose(seq.grad[:, 0].numpy(), amp * level, rtol=1e-12, atol=1e-9)
+    assert seq.adc == [k in (35, 45, 55, 65) for k in knots[1:] / us]
+    np.testing.assert_array_equal(seq.adc_phase.numpy(), np.where(seq.adc, 0.25, 0.0))
+    expect_rf = [0] * 11 + [1, 2, 3, 0]
+    np.testing.assert_array_equal(seq.rf[:, 0].numpy(), expect_rf)
+    assert seq.is_rf == [v > 0 for v in expect_rf]

@@ -56,3 +56,29 @@ def test_dispatches_never_split_an_occurrence_and_records_skip_them():
         inner[a : a + k] = True
     # cache hints name stepped segments only: no hint on a segment inside an occurrence
     assert not np.any(fl[inner] & (driver.SAME_FREE | driver.SAME_RF_DT | driver.NEXT_FREE | driver.NEXT_RF_DT))
+
+
+def test_rf_runs_against_a_plain_scan():
+    """rf_runs by array operations against a segment-by-segment scan of the definition, on random flag patterns."""
+
+    def scan(is_rf, adc, spoil):
+        runs, start = [], None
+        for s in range(len(is_rf)):
+            live = is_rf[s] and not adc[s]
+            if start is not None and not live:
+                runs.append((start, s - start))
+                start = None
+            if live:
+                start = s if start is None else start
+                if spoil[s]:
+                    runs.append((start, s + 1 - start))
+                    start = None
+        if start is not None:
+            runs.append((start, len(is_rf) - start))
+        return runs
+
+    rng = np.random.default_rng(0)
+    for trial in range(200):
+        n = int(rng.integers(1, 60))
+        is_rf, adc, spoil = rng.random(n) < 0.7, rng.random(n) < 0.1, rng.random(n) < 0.15
+        assert pulses.rf_runs(is_rf, adc, spoil) == scan(is_rf, adc, spoil), trial
