@@ -15,11 +15,18 @@ fallback and the judge:
   the network's: ``SimResult.surrogate`` reports the largest difference of a map entry (D or v, magnetization
   units), and if it exceeds ``JUDGE_TOL`` every spin falls back to the exact maps.
 
-Training: ``N_TRAIN`` inputs drawn uniformly from the box (the phantom's range of df, u and b1 plus a margin), their
-exact block quaternions from ``hb_spinor_blocks`` (float64), then ``STEPS`` Adam steps on batches of ``BATCH``, the
-training set reshuffled every epoch (``hb_mlp_train_steps``: a fused forward + backward, the weight gradients in
-fixed-order chunk sums, Adam; the step loop runs in C), the step size on a cosine schedule. Trained networks are kept for the life of the process and reused for any later case of the same
-pulse shape whose spins lie in their box.
+Amplitude: a network learns a pulse's shape, its RF divided by its peak |RF| (``PulseShape.scale``), and takes as
+its third input the spin's B1 scale times that peak, the spin's peak RF in Hz. Pulses that differ only in amplitude
+(a flip-angle change, a train of refocusing flips) share one network: each pulse uses it with the B1 column of the
+input normalisation divided by its own peak, which is exact since RF enters the Bloch equations only as B1 x RF.
+
+Training: ``N_TRAIN`` inputs drawn uniformly from the box (the range of df, u and B1 x peak over the phantom's spins
+and every pulse of the shape, plus a margin; B1 x peak at least +-``MIN_REL_B1`` of its centre, so a later flip
+within that margin finds the network), their exact block quaternions from ``hb_spinor_blocks`` (float64), then
+``STEPS`` Adam steps on batches of ``BATCH``, the training set reshuffled every epoch (``hb_mlp_train_steps``: a fused
+forward + backward, the weight gradients in fixed-order chunk sums, Adam; the step loop runs in C), the step size on
+a cosine schedule. Trained networks are kept for the life of the process and reused for any later case of the same
+shape whose spins' inputs (not the padded box) lie in their box.
 """
 
 from __future__ import annotations
@@ -44,7 +51,8 @@ STEPS = 20000
 BATCH = 8192
 LR = (3e-3, 1e-5)  # cosine from the first to the second
 MARGIN = 0.05  # the box extends the phantom's range by this fraction of its width on each side
-MIN_HALF = (1.0, 1.0, 0.01)  # smallest half-widths of the box: df [Hz], u [Hz], b1
+MIN_HALF = (1.0, 1.0, 1e-6)  # smallest half-widths of the box: df [Hz], u [Hz], b1 x peak [Hz]
+MIN_REL_B1 = 0.05  # the box's B1 x peak half-width is at least this fraction of its centre
 JUDGE_STRIDE = 257
 JUDGE_TOL = 1e-3
 
@@ -84,13 +92,19 @@ class Net:
 class PulseShape:
     """One reused pulse as the surrogate sees it."""
 
-    table: np.ndarray  # (K, 8) float64 rows of its first occurrence (pulses.pulse_table's columns)
+    table: np.ndarray  # (K, 8) float64 rows of its first occurrence (pulses.pulse_table's columns), RF / scale
     bounds: np.ndarray  # (BLOCKS + 1,) int32 block boundaries (substep indices)
     blocks: np.ndarray  # (BLOCKS, 2) float64 tau_j, b0_offset area B_j
     g: np.ndarray  # (3,) float64 the constant gradient [Hz/m]
+    scale: float = 1.0  # the pulse's peak |RF| [Hz]
 
     def key(self) -> str:
-        return hashlib.sha1(self.table.tobytes()).hexdigest()
+        """The shape's identity: every column quantised to 2^-32 of its largest value (pulses equal up to rounding
+        of the division by their peak share a key)."""
+        peak = np.abs(self.table).max(0)
+        q = np.round(self.table / np.where(peak > 0, peak, 1.0) * 2.0**32).astype(np.int64)
+        head = np.array([float(f"{v:.9e}") for v in peak])
+        return hashlib.sha1(q.tobytes() + head.tobytes()).hexdigest()
 
 
 def block_bounds(k: int) -> np.ndarray:
@@ -103,24 +117,44 @@ def shape_of(table: np.ndarray) -> PulseShape | None:
     during the pulse, or fewer substeps than blocks)."""
     k = len(table)
     g = table[:, 3:6]
-    if k < BLOCKS or np.abs(g - g[0]).max() > 1e-9 * max(np.abs(g).max(), 1e-300):
+    scale = float(np.hypot(table[:, 1], table[:, 2]).max()) if k else 0.0
+    if k < BLOCKS or scale == 0.0 or np.abs(g - g[0]).max() > 1e-9 * max(np.abs(g).max(), 1e-300):
         return None
+    shape_rows = np.array(table, dtype=np.float64)
+    shape_rows[:, 1:3] /= scale
     bounds = block_bounds(k)
     tau = np.add.reduceat(table[:, 0], bounds[:-1])
     area = np.add.reduceat(table[:, 0] * table[:, 6], bounds[:-1])
-    return PulseShape(np.ascontiguousarray(table), bounds, np.ascontiguousarray(np.stack([tau, area], 1)),
-                      g[0].copy())
+    return PulseShape(np.ascontiguousarray(shape_rows), bounds, np.ascontiguousarray(np.stack([tau, area], 1)),
+                      g[0].copy(), scale)
 
 
-def input_box(spins: torch.Tensor, pos_lo: torch.Tensor, n: int, g: np.ndarray) -> tuple:
-    """(centre, half-widths) of the box around the spins' (df, u, b1) with the margin."""
+def input_range(spins: torch.Tensor, pos_lo: torch.Tensor, n: int, g: np.ndarray, scale: float) -> np.ndarray:
+    """(2, 3) low and high of the spins' inputs (df, u, b1 x the pulse's peak)."""
     pos = spins[:3, :n].double() + pos_lo[:, :n].double()
     u = (torch.as_tensor(g, dtype=torch.float64, device=pos.device)[:, None] * pos).sum(0)
-    cols = (spins[6, :n].double(), u, spins[7, :n].double())
-    lo = np.array([float(c.min()) for c in cols])
-    hi = np.array([float(c.max()) for c in cols])
-    half = np.maximum(0.5 * (hi - lo) * (1 + 2 * MARGIN), MIN_HALF)
-    return tuple(0.5 * (lo + hi)), tuple(half)
+    cols = (spins[6, :n].double(), u, spins[7, :n].double() * scale)
+    return np.array([[float(c.min()) for c in cols], [float(c.max()) for c in cols]])
+
+
+def padded_box(rng: np.ndarray) -> tuple:
+    """(centre, half-widths) of the training box around an input range: the margin on each side, and the minimum
+    half-widths (B1 x peak: ``MIN_REL_B1`` of its centre)."""
+    lo, hi = rng
+    centre = 0.5 * (lo + hi)
+    floor = np.array(MIN_HALF, dtype=np.float64)
+    floor[2] = max(floor[2], MIN_REL_B1 * abs(centre[2]))
+    half = np.maximum(0.5 * (hi - lo) * (1 + 2 * MARGIN), floor)
+    return tuple(centre), tuple(half)
+
+
+def pulse_norm(net: "Net", shape: PulseShape) -> torch.Tensor:
+    """``net``'s normalisation for a pulse of peak ``shape.scale``: the B1 column's centre and half-width divided by
+    the peak, so the kernels' per-spin B1 scale enters as B1 x peak."""
+    norm = net.norm.clone()
+    norm[2] /= shape.scale
+    norm[5] /= shape.scale
+    return norm
 
 
 def _check(rc: int, lib, name: str) -> None:
@@ -215,29 +249,36 @@ def gradient(net: Net, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
 
 
 def prepare(case, seq) -> None:
-    """Train (or fetch) a network for every reused pulse of ``case`` the surrogate covers; sets
-    ``case.extra["surrogate"]`` = [(pid, shape, net)]."""
-    nets = []
+    """Train (or fetch) a network for every reused pulse of ``case`` the surrogate covers, one per pulse shape (the
+    pulses of one shape, whatever their amplitudes, share it); sets ``case.extra["surrogate"]`` = [(pid, shape,
+    net)]."""
     table = case.pulse_table.cpu().numpy()
     index = case.pulse_index.cpu().numpy()
+    by_key: dict[str, list] = {}
     for pid, (off, k) in enumerate(index[: case.n_pulses]):
         shape = shape_of(table[off : off + k])
-        if shape is None:
-            continue
-        box = input_box(case.spins, case.pos_lo, case.n, shape.g)
-        net = cached(shape.key(), box)
+        if shape is not None:
+            by_key.setdefault(shape.key(), []).append((pid, shape))
+    nets = []
+    for key, members in by_key.items():
+        rngs = [input_range(case.spins, case.pos_lo, case.n, shape.g, shape.scale) for _, shape in members]
+        rng = np.stack([np.min([r[0] for r in rngs], 0), np.max([r[1] for r in rngs], 0)])
+        net = cached(key, rng)
         if net is None:
-            net = train(shape, box, dev=case.spins.device)
-            _CACHE[(shape.key(), box)] = net
-        nets.append((pid, shape, net))
-    case.extra["surrogate"] = nets
+            box = padded_box(rng)
+            net = train(members[0][1], box, dev=case.spins.device)
+            _CACHE[(key, box)] = net
+        nets += [(pid, shape, net) for pid, shape in members]
+    case.extra["surrogate"] = sorted(nets, key=lambda t: t[0])
 
 
-def cached(key: str, box: tuple) -> Net | None:
-    """A network trained on this pulse shape whose box holds ``box`` (the smallest such), or None."""
+def cached(key: str, rng: np.ndarray) -> Net | None:
+    """A network trained on this pulse shape whose box holds the input range ``rng`` (the smallest such), or
+    None."""
     best = None
     for (k, b), net in _CACHE.items():
-        inside = all(abs(c - cb) + h <= hb * (1 + 1e-9) for c, h, cb, hb in zip(box[0], box[1], b[0], b[1]))
+        lo, hi = np.array(b[0]) - np.array(b[1]), np.array(b[0]) + np.array(b[1])
+        inside = bool(np.all(rng[0] >= lo) and np.all(rng[1] <= hi))
         if k == key and inside and (best is None or np.prod(b[1]) < np.prod(best.box[1])):
             best = net
     return best
@@ -268,8 +309,11 @@ def compose_maps(case, lib) -> None:
     for pid, shape, net in nets:
         blocks = torch.from_numpy(shape.blocks).to(dev)
         sp = HbSurPulse(blocks.data_ptr(), (ctypes.c_double * 3)(*shape.g), pid, 0)
-        _check(lib.hb_surrogate_maps(ctypes.byref(st), ctypes.byref(net.struct()), ctypes.byref(sp),
-                                     case.maps.data_ptr(), outside.data_ptr(), stream()), lib, "hb_surrogate_maps")
+        norm = pulse_norm(net, shape)
+        mlp = HbMlp(net.params.data_ptr(), net.adam_m.data_ptr(), net.adam_v.data_ptr(), norm.data_ptr(), HIDDEN,
+                    N_OUT)
+        _check(lib.hb_surrogate_maps(ctypes.byref(st), ctypes.byref(mlp), ctypes.byref(sp), case.maps.data_ptr(),
+                                     outside.data_ptr(), stream()), lib, "hb_surrogate_maps")
         mine = case.maps.view(case.n_pulses, 12, case.n_pad)[pid][:, sel]
         exact = jm.view(case.n_pulses, 12, case.n_pad)[pid][:, sel]
         err = (mine - exact).abs().max()

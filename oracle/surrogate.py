@@ -24,8 +24,10 @@ How each function is pinned (tests/test_surrogate_oracle.py):
 - ``block_map``: with one substep per block (M = K) the split is the exact substep sequence, so it equals
   core.rf_segment applied K times to 1e-12 (with relaxation); its error at M = 10 against that is under 1e-5
   (the Strang split error, second order: halving short blocks' length divides it by about four);
-- ``mlp_grad``: central finite differences of ``mlp_forward``'s loss to 1e-7 relative, with a planted mistake (the
-  tanh derivative dropped) that this check catches;
+- ``mlp_forward`` / ``mlp_grad`` / ``mlp_loss``: torch.nn's Linear / Tanh stack (the library routine) loaded with
+  the same flat parameters, outputs and autograd's flat gradient of the mean squared error to 1e-12; central finite
+  differences of the loss to 1e-7 relative, with a planted mistake (the tanh derivative dropped) that this check
+  catches;
 - ``adam``: torch.optim.Adam (the library routine) to 1e-12.
 """
 

@@ -101,7 +101,7 @@ def trained(slab):
         pytest.skip("needs a ROCm GPU and the built library")
     ph, seq, coils, shape = slab
     case = driver.prepare_case(ph, seq, coils, reuse=True)
-    box = sur.input_box(case.spins, case.pos_lo, case.n, shape.g)
+    box = sur.padded_box(sur.input_range(case.spins, case.pos_lo, case.n, shape.g, shape.scale))
     return sur.train(shape, box, dev="cuda"), box, shape
 
 
@@ -248,8 +248,37 @@ def test_ablation_training_and_cost(gpu, trained, slab):
 
     t_exact = timed(lambda: lib.hb_pulse_maps(ctypes.byref(st), big.pulse_table.data_ptr(), big.pulse_index.data_ptr(),
                                               big.n_pulses, None, 0, big.maps.data_ptr(), sur.stream()))
-    t_net = timed(lambda: lib.hb_surrogate_maps(ctypes.byref(st), ctypes.byref(net.struct()), ctypes.byref(sp),
+    norm = sur.pulse_norm(net, shape)
+    mlp = sur.HbMlp(net.params.data_ptr(), net.adam_m.data_ptr(), net.adam_v.data_ptr(), norm.data_ptr(), sur.HIDDEN,
+                    sur.N_OUT)
+    t_net = timed(lambda: lib.hb_surrogate_maps(ctypes.byref(st), ctypes.byref(mlp), ctypes.byref(sp),
                                                 big.maps.data_ptr(), outside.data_ptr(), sur.stream()))
     print(f"maps of {big.n} spins: exact composition {t_exact:.3f} ms, network {t_net:.3f} ms "
           f"({t_exact / t_net:.1f}x)")
     assert t_net < t_exact
+
+
+def test_a_flip_change_reuses_the_network(gpu, slab):
+    """The network learns the pulse's shape (RF over its peak) and takes B1 x peak: a different flip angle within the
+    box's margin uses the same network, trained once, and stays within the judge's bound; so do the 16 refocusing
+    flips of a TSE train (one network for all of them)."""
+    ph, seq, coils, _ = slab
+    seq2 = W.first_trs(seq, 2)
+    first = driver.prepare_case(ph, seq2, coils, surrogate=True)
+    net = first.extra["surrogate"][0][2]
+    bumped = W.first_trs(W.gre3d_sequence(16, flip_deg=10.3), 2)
+    again = driver.prepare_case(ph, bumped, coils, surrogate=True)
+    assert again.extra["surrogate"][0][2] is net  # no retraining for a 3 % flip change
+    driver.run_forward(again)
+    out = driver.read_result(again)
+    assert out.surrogate["pulses"][0]["judge_max_map_err"] < 1e-3
+    exact = bp.simulate(ph, bumped, coils, engine="gpu", pulses="reuse")
+    assert rel(out.signal, exact.signal) < 1e-3
+    flips = torch.linspace(150.0, 180.0, 4, dtype=torch.float64)
+    tse = bp.tse_2d(16, etl=4, tr=0.3, refocus_flip_deg=flips, slice_thickness=5e-3, rf_duration=2e-3,
+                    system=bp.System(rf_raster=20e-6))
+    tph = bp.shepp_logan(16)
+    tcase = driver.prepare_case(tph, tse, bp.birdcage_sensitivities(tph.pos, 4), surrogate=True)
+    nets = {id(n) for _, _, n in tcase.extra["surrogate"]}
+    assert tcase.n_pulses == 5 and len(tcase.extra["surrogate"]) == 5
+    assert len(nets) == 2  # the excitation, and one network for the four refocusing flips

@@ -154,3 +154,60 @@ def test_adam_matches_torch():
         tp.grad = torch.tensor(g)
         opt.step()
     assert np.abs(p - tp.detach().numpy()).max() < 1e-12
+
+
+def test_pulse_shape_key_and_norm_fold_the_amplitude():
+    """Pulses that differ only in amplitude share a shape key; the per-pulse normalisation turns a spin's B1 into
+    B1 x peak (CPU)."""
+    import torch
+
+    from blochparty_amd import workloads as W
+    from blochparty_amd.engines import pulses
+    from blochparty_amd.engines import surrogate as sur
+
+    shapes = []
+    for flip in (10.0, 13.7):
+        seq = W.first_trs(W.gre3d_sequence(4, flip_deg=flip), 2)
+        table, index = pulses.pulse_table(seq, pulses.reused_pulses(seq))
+        off, k = index[0]
+        shapes.append(sur.shape_of(table[off : off + k]))
+    a, b = shapes
+    assert abs(b.scale / a.scale - 13.7 / 10.0) < 1e-12
+    assert a.key() == b.key()
+    np.testing.assert_allclose(a.table, b.table, rtol=0, atol=1e-15)
+    other = sur.shape_of(np.concatenate([a.table[:, :1], 2 * a.table[:, 1:3], a.table[:, 3:]], 1) * 1.0)
+    assert other.key() == a.key()  # twice the amplitude, same shape
+    bent = a.table.copy()
+    bent[5, 1] *= 1.01
+    assert sur.shape_of(bent).key() != a.key()
+    net = sur.Net(torch.zeros(1), torch.zeros(1), torch.zeros(1), torch.tensor([1.0, 2.0, 30.0, 4.0, 5.0, 3.0]),
+                  ((1.0, 2.0, 30.0), (4.0, 5.0, 3.0)))
+    norm = sur.pulse_norm(net, b)
+    b1 = 0.93
+    assert abs((b1 - float(norm[2])) / float(norm[5]) - (b1 * b.scale - 30.0) / 3.0) < 1e-5  # float32 norm
+
+
+def test_mlp_forward_and_grad_against_torch_nn():
+    """The oracle's network against torch.nn's Linear / Tanh stack loaded with the same flat parameters (the weight
+    layout W1, b1, W2, b2, W3, b3, each Linear's (out, in) weight): outputs to 1e-12, and the flat gradient of the
+    mean squared error against autograd's, in the same order (CPU)."""
+    import torch
+
+    h, n_out = 16, 10
+    rng = np.random.default_rng(7)
+    p = rng.normal(size=S.mlp_size(h, n_out)) * 0.4
+    x = rng.uniform(-1, 1, (50, 3))
+    y = rng.normal(size=(50, n_out))
+    net = torch.nn.Sequential(torch.nn.Linear(3, h), torch.nn.Tanh(), torch.nn.Linear(h, h), torch.nn.Tanh(),
+                              torch.nn.Linear(h, n_out)).double()
+    with torch.no_grad():
+        for t, v in zip(net.parameters(), S.mlp_unpack(p, h, n_out)):
+            t.copy_(torch.from_numpy(v))
+    out, _ = S.mlp_forward(p, x, h, n_out)
+    ref = net(torch.from_numpy(x))
+    np.testing.assert_allclose(out, ref.detach().numpy(), rtol=0, atol=1e-12)
+    loss = torch.nn.functional.mse_loss(ref, torch.from_numpy(y))
+    loss.backward()
+    auto = np.concatenate([t.grad.numpy().ravel() for t in net.parameters()])
+    np.testing.assert_allclose(S.mlp_grad(p, x, y, h, n_out), auto, rtol=0, atol=1e-12)
+    assert abs(S.mlp_loss(p, x, y, h, n_out) - float(loss)) < 1e-12
