@@ -25,6 +25,8 @@ min(W, 1) warmup steps, the caps being --leg-steps and --adjoint-steps, and each
   device, and the reverse sweep; with exact RF, then with the learned surrogate's maps (per-sample readouts), then
   with the surrogate's maps and NUFFT readouts (each readout run reversed as the transpose of its NUFFT; its ratio
   to the exact-RF leg is ``fwd_adjoint_speedup_vs_exact_rf``). Each reports dL/dflip, which should agree;
+- the multi-slice workload of the learned surrogate (5 slices x 256^2 x 8 isochromats, 3 ms sinc at 30 deg, B1
+  0.7-1.3): forward with the surrogate's maps and forward stepping every RF substep, same readouts (``multislice``);
 - ``end_to_end``: simulate() timed from the caller's Phantom / Sequence / coils to the CPU result, the way the
   reference's harness times it (benchmarks/bench.py:47-95; sequence and phantom built outside the timed region):
   the value's configuration forward, and forward + .backward() with the flip angle, the T1 map and the coils
@@ -196,6 +198,39 @@ def fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, steps: int,
     return leg
 
 
+def multislice_leg(W, driver, sur, lib, steps: int, warmup: int, nufft) -> dict:
+    """The learned surrogate's own workload (multi-slice 2D GRE, 5 slices x 256^2 x 8 isochromats, 3 ms sinc at
+    30 deg, B1 0.7-1.3, +-200 Hz): forward with the surrogate's maps against stepping every RF substep, both with
+    the same readouts; the surrogate's signal error against the exact path's, and the judge's map errors."""
+    t0 = time.perf_counter()
+    ph, seq, coils = W.multislice()
+    work = len(ph) * seq.n_adc
+    log(f"multislice workload built in {time.perf_counter() - t0:.1f} s: N={len(ph)} S={len(seq)} "
+        f"n_adc={seq.n_adc}")
+    t0 = time.perf_counter()
+    case = driver.prepare_case(ph, seq, coils, surrogate=True, nufft=nufft)
+    torch.cuda.synchronize()
+    log(f"multislice surrogate: {len(case.extra['surrogate'])} network(s) ready in {time.perf_counter() - t0:.1f} s")
+    out = {"workload": "multi-slice 2D GRE, 5 slices x 256^2 x 8 isochromats through 3 mm, 3 ms sinc (TBW 4, "
+                       "300 substeps) at 30 deg per slice, B1 U(0.7, 1.3), +-200 Hz, 8 coils, TR 10 ms",
+           "spins": len(ph), "n_adc": seq.n_adc, "readouts": "nufft" if nufft else "direct"}
+    out["surrogate"] = forward_leg(case, lib, driver, steps, warmup, work, "multislice, learned RF surrogate")
+    out["judge"] = sur.report(case)
+    sig_s = driver.read_result(case).signal.to(torch.complex128)
+    del case
+    torch.cuda.empty_cache()
+    case = driver.prepare_case(ph, seq, coils, nufft=nufft)
+    out["exact_rf"] = forward_leg(case, lib, driver, steps, warmup, work, "multislice, exact RF")
+    sig_e = driver.read_result(case).signal.to(torch.complex128)
+    del case
+    torch.cuda.empty_cache()
+    out["speedup_vs_exact_rf"] = out["surrogate"]["value"] / out["exact_rf"]["value"]
+    out["surrogate_signal_rel_err_vs_exact"] = float((sig_s - sig_e).abs().max() / sig_e.abs().max())
+    log(f"multislice: surrogate {out['speedup_vs_exact_rf']:.1f}x exact RF, signal error vs exact "
+        f"{out['surrogate_signal_rel_err_vs_exact']:.2e}")
+    return out
+
+
 def end_to_end(bp, W, args, ph, seq, coils, work: int, steps: int, warmup: int) -> dict:
     """simulate() timed from the caller's objects to the CPU result (host packing, upload, the GPU work, readback),
     as the reference's harness times it; the grad case adds .backward() of sum |signal|^2 with the flip angle, the
@@ -250,6 +285,8 @@ def main() -> None:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint legs")
     ap.add_argument("--no-exact", action="store_true", help="skip the exact-RF legs")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end simulate() timing")
+    ap.add_argument("--no-multislice", action="store_true",
+                    help="skip the multi-slice surrogate workload (surrogate vs exact RF forward)")
     ap.add_argument("--only-value", action="store_true",
                     help="the value's leg alone (no other leg, no CPU baseline): for profiler runs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -269,7 +306,7 @@ def main() -> None:
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
     if args.only_value:
-        args.no_cpu = args.no_adjoint = args.no_exact = args.no_e2e = True
+        args.no_cpu = args.no_adjoint = args.no_exact = args.no_e2e = args.no_multislice = True
     if args.checkpoint_every % 64:
         raise SystemExit("--checkpoint-every must be a multiple of 64")
 
@@ -372,6 +409,11 @@ def main() -> None:
         line["speedup_vs_exact_rf"] = lead["value"] / line["exact_rf_forward"]["value"]
         del case_e
         torch.cuda.empty_cache()
+        emit()
+
+    # -- the surrogate's own workload: multi-slice GRE, surrogate vs exact RF
+    if not args.no_multislice:
+        line["multislice"] = multislice_leg(W, driver, sur, lib, leg_k, leg_w, nufft)
         emit()
 
     # -- forward + adjoint: the surrogate's maps with the value's readouts first, then exact RF

@@ -6,6 +6,11 @@
   smooth random phantom (T1 0.8 / 1.3 / 4.0 s, T2 80 / 110 / 2000 ms, PD 0.7 / 0.8 / 1.0) with a smooth +-100 Hz
   off-resonance, 8 coils, a 2 ms windowed-sinc slab pulse (TBW 4, 200 substeps on a 10 us RF raster) of 10 deg over
   the field of view, TR 8 ms, n x n phase encodes x n readout samples.
+- ``multislice``: multi-slice 2D spoiled GRE, 5 slices of 3 mm (centres 6 mm apart) acquired one after another,
+  256 x 256 in-plane, 8 isochromats through each slice's thickness (2,621,440 spins), three tissue classes per
+  slice, B1 drawn uniformly from 0.7-1.3 per voxel, a smooth +-200 Hz off-resonance, 8 coils, a 3 ms windowed-sinc
+  slice pulse (TBW 4, 300 substeps on a 10 us raster) of 30 deg shifted to each slice, TR 10 ms: the learned RF
+  surrogate's workload (five distinct reused pulses, one per slice offset).
 """
 
 from __future__ import annotations
@@ -15,7 +20,9 @@ import torch
 from .builder import System
 from .coils import birdcage_sensitivities
 from .library import gre_2d, gre_cartesian
-from .phantom import Phantom, phantom_from_maps, shepp_logan, three_class_maps
+from .phantom import THREE_CLASSES, Phantom, grid_positions, phantom_from_maps, shepp_logan, smooth_field, \
+    three_class_maps
+from .sequence import Sequence
 
 FOV = 0.22
 
@@ -60,3 +67,58 @@ def first_trs(seq, n_tr: int):
     e = int(ends[n_tr - 1]) + 1 if len(ends) >= n_tr else len(seq)
     return Sequence(seq.dt[:e], seq.rf[:e], seq.grad[:e], seq.adc_phase[:e], seq.is_rf[:e], seq.adc[:e],
                     seq.spoil[:e], dict(seq.meta), None if seq.b0_offset is None else seq.b0_offset[:e])
+
+
+def concat(seqs: list[Sequence]) -> Sequence:
+    """The sequences played one after another (their meta from the first, with each part's segment count)."""
+    cat = lambda name: torch.cat([getattr(q, name) for q in seqs])
+    flags = lambda name: [v for q in seqs for v in getattr(q, name)]
+    meta = dict(seqs[0].meta)
+    meta["parts"] = [len(q) for q in seqs]
+    return Sequence(cat("dt"), cat("rf"), cat("grad"), cat("adc_phase"), flags("is_rf"), flags("adc"),
+                    flags("spoil"), meta)
+
+
+SLICES, SLICE_THICKNESS, SLICE_SPACING, SLICE_ISO = 5, 3e-3, 6e-3, 8
+
+
+def slice_centres(slices: int = SLICES, spacing: float = SLICE_SPACING) -> list[float]:
+    return [(s - (slices - 1) / 2) * spacing for s in range(slices)]
+
+
+def multislice_sequence(n: int = 256, lines: int | None = None, slices: int = SLICES, flip_deg=30.0,
+                        tr: float = 10e-3) -> Sequence:
+    """Each slice in turn: a spoiled GRE of ``lines`` (default n) phase encodes x n samples with the 3 ms sinc
+    (TBW 4) shifted to the slice's centre."""
+    parts = [gre_cartesian((lines or n, n), FOV, flip_deg=flip_deg, tr=tr, slice_thickness=SLICE_THICKNESS,
+                           slice_offset=z, rf_duration=3e-3, time_bw_product=4.0, system=System(rf_raster=10e-6))
+             for z in slice_centres(slices)]
+    return concat(parts)
+
+
+def multislice_phantom(n: int = 256, slices: int = SLICES, iso: int = SLICE_ISO, seed: int = 1) -> Phantom:
+    """Per slice an n x n grid of three-class voxels (seeded smooth field thresholded at its terciles), each voxel
+    ``iso`` isochromats at equal steps through the slice's thickness, PD / iso each; B1 ~ U(0.7, 1.3) per voxel
+    (seed 4), off-resonance a smooth field scaled to +-200 Hz (seed 2). Spins slice by slice, voxel by voxel."""
+    shape = (slices, n, n)
+    field = smooth_field(shape, 4.0, seed)
+    lo, hi = torch.quantile(field.flatten()[:: max(1, field.numel() // 65536)],
+                            torch.tensor([1 / 3, 2 / 3], dtype=torch.float64))
+    label = ((field > lo).long() + (field > hi).long()).reshape(-1)
+    props = torch.tensor(THREE_CLASSES, dtype=torch.float64)[label]
+    db0 = smooth_field(shape, 8.0, 2)
+    db0 = (db0 / db0.abs().max() * 200.0).reshape(-1)
+    b1 = 0.7 + 0.6 * torch.rand(slices * n * n, generator=torch.Generator().manual_seed(4), dtype=torch.float64)
+    plane = grid_positions(n, FOV).reshape(-1, 3)  # z = 0
+    steps = ((torch.arange(iso, dtype=torch.float64) + 0.5) / iso - 0.5) * SLICE_THICKNESS
+    z = torch.tensor(slice_centres(slices), dtype=torch.float64)[:, None, None] + steps[None, None, :]
+    pos = plane[None, :, None, :].expand(slices, n * n, iso, 3).clone()
+    pos[..., 2] = z.expand(slices, n * n, iso)
+    rep = lambda v: v.repeat_interleave(iso)
+    return Phantom(pos=pos.reshape(-1, 3), pd=rep(props[:, 0]) / iso, t1=rep(props[:, 1]), t2=rep(props[:, 2]),
+                   db0=rep(db0), b1=rep(b1))
+
+
+def multislice(n: int = 256, lines: int | None = None, n_coils: int = 8, slices: int = SLICES):
+    ph = multislice_phantom(n, slices)
+    return ph, multislice_sequence(n, lines, slices), birdcage_sensitivities(ph.pos, n_coils)
