@@ -9,6 +9,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
@@ -35,16 +36,30 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm is needed to build blochparty_amd)")
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile the library (always; a few seconds per source) and return its path."""
-    sources = sorted(str(p) for p in CSRC.glob("*.hip"))
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *FLAGS, "-I", str(HERE.parent / "include"), *sources, "-o", str(tmp)]
+OBJ = HERE.parent / "_build"
+
+
+def _run(cmd: list[str], verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
+        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile the library (always: every source to an object, in parallel, then one link) and return its path."""
+    sources = sorted(CSRC.glob("*.hip"))
+    OBJ.mkdir(exist_ok=True)
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    objects = [OBJ / (src.stem + ".o") for src in sources]
+    jobs = max(1, min(len(sources), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as pool:
+        for f in [pool.submit(_run, [hipcc(), *compile_flags, "-c", "-I", str(HERE.parent / "include"), str(src),
+                                     "-o", str(obj)], verbose) for src, obj in zip(sources, objects)]:
+            f.result()
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([hipcc(), *FLAGS, *map(str, objects), "-o", str(tmp)], verbose)
     os.replace(tmp, LIB)
     return LIB
 
