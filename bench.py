@@ -161,7 +161,7 @@ def fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, steps: int,
     rf = flip x shape: dL/dflip = sum_s rf_s . dL/drf_s / flip for stepped RF; a reused pulse's occurrence carries
     dL/d(ln amplitude) in its first segment's rf_x slot."""
     S, n_adc = len(seq), seq.n_adc
-    nc = next(c for c in adjoint.ADJOINT_COILS if c >= case_a.n_coils)
+    nc = adjoint.adjoint_width(case_a.n_coils)
     dev = case_a.spins.device
     if case_a.perm is not None:  # NUFFT readouts: the spins are sorted on the device
         coils = coils[torch.from_numpy(case_a.perm)]

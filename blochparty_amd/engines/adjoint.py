@@ -93,11 +93,7 @@ class GPUSimulate(torch.autograd.Function):
         dev = case.spins.device
         n, n_pad, C, S, every = case.n, case.n_pad, case.n_coils, len(opts.is_rf), opts.every
         adc_idx = torch.tensor(opts.adc).nonzero()[:, 0]
-        nc = 0
-        if g_sig is not None and case.n_adc:
-            nc = next((c for c in ADJOINT_COILS if c >= C), None)
-            if nc is None:
-                raise NotImplementedError(f"hip engine: gradients with more than {ADJOINT_COILS[-1]} coils")
+        nc = adjoint_width(C) if g_sig is not None and case.n_adc else 0
         h = torch.zeros(max(case.n_adc, 1), max(nc, 1), 2, dtype=torch.float64)
         if nc:
             psi = adc_phase.detach().double().cpu()[adc_idx][:, None]
@@ -157,17 +153,34 @@ class GPUSimulate(torch.autograd.Function):
                         for i, g in enumerate(out)])
 
 
+PASS_COILS = ADJOINT_COILS[-1]  # coils per sweep; more coils take one sweep per tile of this many
+
+
+def adjoint_width(n_coils: int) -> int:
+    """Coil columns of dL/dsignal and sensitivity rows for ``n_coils``: the register width that holds them, or for
+    more than PASS_COILS a whole number of PASS_COILS-wide tiles (one sweep each)."""
+    if n_coils <= PASS_COILS:
+        return next(c for c in ADJOINT_COILS if c >= n_coils)
+    return PASS_COILS * -(-n_coils // PASS_COILS)
+
+
 def run_backward(case: driver.DeviceCase, ckpts: list, every: int, h: torch.Tensor, lam: torch.Tensor,
                  sens: torch.Tensor, nc: int, need_seg: int, need_coil: bool, lib=None):
     """The adjoint kernels on the current stream over ``case`` (aligned dispatches, forward run with ``keep``):
     ``h`` (n_adc, nc, 2) float32 dL/dsignal turned by exp(+i psi), ``lam`` (3, n_pad) float32 dL/dm at the end (it
-    ends as dL/dm0), ``sens`` (2 nc, n_pad). ``need_seg``: 0 no segment gradients, 1 all, 2 only rf's (the other
-    slots come back 0). Returns the float64 accumulators (gs (8, n_pad), gc (2 nc, n_pad), seg (S, 8)) and lam."""
+    ends as dL/dm0), ``sens`` (2 nc, n_pad), nc from ``adjoint_width``. ``need_seg``: 0 no segment gradients, 1 all,
+    2 only rf's (the other slots come back 0). Returns the float64 accumulators (gs (8, n_pad), gc (2 nc, n_pad),
+    seg (S, 8)) and lam.
+
+    More than PASS_COILS coils: the sweep is linear in its sources (lambda and every gradient are linear in
+    dL/dsignal and in dL/dm at the end), so it runs once per tile of PASS_COILS coils with that tile's columns of h
+    and rows of the sensitivities, dL/dm at the end entering the first pass only; the float64 accumulators and the
+    per-spin seeds of the pulse maps add up over the passes, and the maps' parameter gradients are formed once from
+    the summed seeds."""
     need_seg = int(need_seg)
     lib = lib or hip.HIP.lib()
     dev = case.spins.device
     n_pad, S = case.n_pad, case.n_segments
-    n_blocks = n_pad // driver.BLOCK
     P = case.n_pulses
     cs = case.struct()
     seeds = torch.zeros(P * 12 * n_pad, dtype=torch.float64, device=dev) if P else None
@@ -179,8 +192,39 @@ def run_backward(case: driver.DeviceCase, ckpts: list, every: int, h: torch.Tens
                             float(steps[:, 8].min()), maps_d.data_ptr())
     gs = torch.zeros(8, n_pad, dtype=torch.float64, device=dev)
     gc = torch.zeros(2 * max(nc, 1), n_pad, dtype=torch.float64, device=dev)
-    gseg = torch.zeros(n_blocks * every * 8 if need_seg else 1, dtype=torch.float32, device=dev)
     seg = torch.zeros(S * 8 if need_seg else 1, dtype=torch.float64, device=dev)
+    width = min(nc, PASS_COILS) if nc else 0
+    passes = max(1, -(-nc // PASS_COILS)) if nc else 1
+    lam_sum = None
+    for t in range(passes):
+        cols = slice(t * width, (t + 1) * width)
+        h_t = h if passes == 1 else h[:, cols].contiguous()
+        sens_t = sens if passes == 1 else sens[2 * cols.start : 2 * cols.stop].contiguous()
+        gc_t = gc if passes == 1 else torch.zeros(2 * width, n_pad, dtype=torch.float64, device=dev)
+        seg_t = seg if t == 0 else torch.zeros_like(seg)
+        lam_t = lam if t == 0 else torch.zeros_like(lam)
+        _sweep(case, cs, ckpts, every, h_t, lam_t, sens_t, width, need_seg, need_coil, lib, seeds, maps_d, gs, gc_t,
+               seg_t)
+        if passes > 1:
+            gc[2 * cols.start : 2 * cols.stop] = gc_t
+            if t:
+                seg += seg_t
+            lam_sum = lam_t.clone() if lam_sum is None else lam_sum + lam_t
+    if passes > 1:
+        lam.copy_(lam_sum)
+    if P:  # the spin parameters' gradients through the maps, from the seeds of the sweep(s)
+        st = torch.from_numpy(np.ascontiguousarray(steps[:, :8])).to(dev)
+        lib.pulse_map_grads(ctypes.byref(cs), case.pulse_table.data_ptr(), case.pulse_index.data_ptr(), P,
+                            st.data_ptr(), seeds.data_ptr(), gs.data_ptr())
+    return gs, gc, seg, lam
+
+
+def _sweep(case, cs, ckpts, every, h, lam, sens, nc, need_seg, need_coil, lib, seeds, maps_d, gs, gc, seg) -> None:
+    """One reverse sweep over every checkpoint interval, last first, with ``nc`` (<= PASS_COILS) coils."""
+    dev = case.spins.device
+    n_pad = case.n_pad
+    n_blocks = n_pad // driver.BLOCK
+    gseg = torch.zeros(n_blocks * every * 8 if need_seg else 1, dtype=torch.float32, device=dev)
     r8 = torch.empty((every + 7) // 8 * 3 * n_pad, dtype=torch.float32, device=dev)
     red = torch.empty(max(1, -(-n_blocks // SEG_CHUNK) * every * 8 if need_seg else 1), dtype=torch.float64,
                       device=dev)
@@ -200,11 +244,6 @@ def run_backward(case: driver.DeviceCase, ckpts: list, every: int, h: torch.Tens
         lib.adjoint_interval(ctypes.byref(cs), ctypes.byref(ad), s0, s1)
         if need_seg:
             lib.seg_reduce(gseg.data_ptr(), n_blocks, every, s0, s1 - s0, red.data_ptr(), seg.data_ptr())
-    if P:  # the spin parameters' gradients through the maps, from the seeds of the sweep
-        st = torch.from_numpy(np.ascontiguousarray(steps[:, :8])).to(dev)
-        lib.pulse_map_grads(ctypes.byref(cs), case.pulse_table.data_ptr(), case.pulse_index.data_ptr(), P,
-                            st.data_ptr(), seeds.data_ptr(), gs.data_ptr())
-    return gs, gc, seg, lam
 
 
 FD_TURN = 1e-4  # rad: how far a central-difference step turns a reused pulse's rotation

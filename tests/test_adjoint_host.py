@@ -66,3 +66,11 @@ def test_a_wrong_frame_turn_is_caught():
     m, lam = rng.normal(size=3), rng.normal(size=3)
     _, sD, _, _ = pulse_reverse(D, v, psi, m, lam)
     assert np.abs(np.outer(lam, m) - sD).max() > 1e-2
+
+
+def test_adjoint_width_and_coil_tiles():
+    """Coils up to 16 are padded to a register width; more take whole tiles of 16 (one sweep per tile)."""
+    from blochparty_amd.engines import adjoint
+
+    assert [adjoint.adjoint_width(c) for c in (1, 2, 3, 5, 8, 9, 16)] == [1, 2, 4, 8, 8, 16, 16]
+    assert [adjoint.adjoint_width(c) for c in (17, 20, 32, 33)] == [32, 32, 32, 48]

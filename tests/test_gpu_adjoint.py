@@ -64,8 +64,10 @@ def test_random_cases_every_input(gpu, seed):
     compare(ph, seq, coils, m0)
 
 
-@pytest.mark.parametrize("n_coils", [1, 3, 8, 16])
+@pytest.mark.parametrize("n_coils", [1, 3, 8, 16, 20, 33])
 def test_gre_coils(gpu, n_coils):
+    """Up to 16 coils in one sweep; 20 and 33 coils in sweeps over tiles of 16, summed (the sweep is linear in
+    dL/dsignal; the reference tests 20 coils, tests/engines/test_gpu_grad.py:38-44)."""
     ph = bp.shepp_logan(12)
     seq = bp.gre_2d(12, n_dummy=5, te=4e-3, tr=8e-3)
     coils = bp.birdcage_sensitivities(ph.pos, n_coils)
@@ -300,6 +302,25 @@ def test_nufft_readouts_in_the_adjoint_16_coils(gpu):
     err = {k: rel(out[k], ref[k]) for k in ref if ref[k].abs().max() > 0}
     print("16 coils nufft", {k: f"{v:.1e}" for k, v in err.items()}, flush=True)
     assert max(err.values()) < 1e-4, err
+
+
+def test_nufft_readouts_in_the_adjoint_20_coils_and_reused_pulses(gpu):
+    """More than one coil tile with NUFFT runs and reused pulse maps in the sweep: each tile's runs reversed from its
+    own adjoint grid, the maps' seeds summed over the passes."""
+    ph, seq, coils, m0 = pulse_case(8, seed=6, n_phantom=20)
+    coils = bp.birdcage_sensitivities(ph.pos, 20)
+    ref = map_gradients(oracle_sim, ph, seq, coils, m0)
+    out = map_gradients(gpu_sim, ph, seq, coils, m0, pulses="reuse", rf_grad="scale", checkpoint_every=64)
+    err = {k: rel(out[k], ref[k]) for k in ref if ref[k].abs().max() > 0}
+    print("20 coils, reused pulses", {k: f"{v:.1e}" for k, v in err.items()}, flush=True)
+    assert max(err.values()) < 1e-4, err
+    ph2, seq2, _, m02 = nufft_case(side=48)
+    coils2 = bp.birdcage_sensitivities(ph2.pos, 20)
+    ref2 = map_gradients(oracle_sim, ph2, seq2, coils2, m02)
+    out2 = map_gradients(gpu_sim, ph2, seq2, coils2, m02, readouts="nufft", checkpoint_every=128)
+    err2 = {k: rel(out2[k], ref2[k]) for k in ref2 if ref2[k].abs().max() > 0}
+    print("20 coils, nufft", {k: f"{v:.1e}" for k, v in err2.items()}, flush=True)
+    assert max(err2.values()) < 1e-3, err2
 
 
 def test_nufft_gradients_bit_identical_for_any_checkpoint_interval(gpu):
