@@ -30,6 +30,7 @@ from .phantom import (
     shepp_logan_maps,
     three_class_maps,
 )
+from .pulseq import read_pulseq
 from .result import SimResult
 from .sequence import Sequence
 
@@ -54,6 +55,7 @@ __all__ = [
     "gre_3d",
     "gre_cartesian",
     "phantom_from_maps",
+    "read_pulseq",
     "shepp_logan",
     "shepp_logan_maps",
     "simulate",

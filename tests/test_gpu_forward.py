@@ -168,3 +168,42 @@ def test_reused_pulses_under_autograd_keep_the_forward(gpu):
     for every in (64, 320):
         res = bp.simulate(bp.Phantom(ph.pos, ph.pd, t1, ph.t2, ph.db0), seq, pulses="reuse", checkpoint_every=every)
         assert rel(res.signal.detach(), fwd.signal) < 1e-6, every
+
+
+def test_pulseq_file_through_the_gpu(gpu):
+    """A GRE written as a Pulseq file (sinc slice pulse on its trapezoid, prephasers, readout trapezoid with the ADC
+    on its flat top, RF spoiling phases, spoiler) read by read_pulseq and run on the GPU, against the oracle."""
+    import math
+
+    import numpy as np
+
+    from seqwriter import Writer
+
+    n, fov, dwell, dur, thk, raster = 12, 0.22, 10e-6, 1e-3, 5e-3, 10e-6
+    dk = 1 / fov
+    k = round(dur / raster)
+    t = (np.arange(k) + 0.5) * raster
+    env = (0.5 + 0.5 * np.cos(2 * np.pi * (t - dur / 2) / dur)) * np.sinc(4 / dur * (t - dur / 2))
+    amp = math.radians(20) / (2 * math.pi) / (env.sum() * raster)
+    g = 4 / dur / thk
+    w = Writer(rasters={"RadiofrequencyRasterTime": raster})
+    for i in range(n + 2):
+        ph = math.radians(117) * i * (i + 1) / 2
+        w.add(dur + 2e-4, rf={"samples": env / env.max(), "amp": amp * env.max(), "delay": 1e-4, "phase": ph},
+              gz=("trap", g, 1e-4, dur, 1e-4, 0.0))
+        line = min(i, n - 1) - n // 2
+        w.add(6e-4, gx=("trap", -(n / 2 + 0.5) * dk / 4e-4, 1e-4, 4e-4, 1e-4, 0.0),
+              gy=("trap", line * dk / 5e-4, 1e-4, 4e-4, 1e-4, 0.0),
+              gz=("trap", -g * (dur / 2 + 5e-5) / 5e-4, 1e-4, 4e-4, 1e-4, 0.0))
+        w.add(n * dwell + 2e-4, gx=("trap", dk / dwell, 1e-4, n * dwell, 1e-4, 0.0),
+              adc=(n, dwell, 1e-4 - 0.5 * dwell + 0.5 * dwell, 0.0, ph))
+        w.add(1e-3, gx=("trap", 3 * n * dk / 9e-4, 1e-4, 8e-4, 1e-4, 0.0))
+        w.add(2e-3)
+    seq = bp.read_pulseq(w.text())
+    ph_ = bp.shepp_logan(16)
+    coils = bp.birdcage_sensitivities(ph_.pos, 4)
+    ref, ref_m = core.simulate(ph_, seq, coils)
+    out = bp.simulate(ph_, seq, coils, engine="gpu")
+    err = float((out.signal.to(torch.complex128) - ref).abs().max() / ref.abs().max())
+    print(f"pulseq file GRE on the GPU vs oracle: {err:.2e}", flush=True)
+    assert err < 1e-5
