@@ -31,6 +31,7 @@ from .phantom import (
     three_class_maps,
 )
 from .pulseq import read_pulseq
+from .recon import coil_combine, reconstruct, rss
 from .result import SimResult
 from .sequence import Sequence
 
@@ -56,6 +57,9 @@ __all__ = [
     "gre_cartesian",
     "phantom_from_maps",
     "read_pulseq",
+    "reconstruct",
+    "rss",
+    "coil_combine",
     "shepp_logan",
     "shepp_logan_maps",
     "simulate",

@@ -207,3 +207,27 @@ def test_pulseq_file_through_the_gpu(gpu):
     err = float((out.signal.to(torch.complex128) - ref).abs().max() / ref.abs().max())
     print(f"pulseq file GRE on the GPU vs oracle: {err:.2e}", flush=True)
     assert err < 1e-5
+
+
+def test_gre_image_from_the_gpu_signal(gpu):
+    """The GPU's signal of a spoiled GRE in steady state, reconstructed and coil-combined, against the closed form
+    (the GPU's share of the recon check in tests/test_recon.py)."""
+    import math
+
+    from blochparty_amd import recon
+
+    n, fov, flip, tr = 32, 0.22, 20.0, 10e-3
+    g = torch.Generator().manual_seed(1)
+    u = lambda lo, hi: lo + (hi - lo) * torch.rand(n, n, generator=g, dtype=torch.float64)
+    pd, t1, t2 = u(0.5, 1.0), u(0.15, 0.4), u(0.05, 0.2)
+    ph = bp.phantom_from_maps(pd, t1, t2, fov)
+    seq = bp.gre_2d(n, fov, flip_deg=flip, tr=tr, te=2e-3, dwell=1e-6, n_dummy=300)
+    coils = bp.birdcage_sensitivities(ph.pos, 8)
+    out = bp.simulate(ph, seq, coils, engine="gpu")
+    img = recon.coil_combine(recon.reconstruct(out.signal.to(torch.complex128), seq), coils.T.reshape(8, n, n))
+    e1 = torch.exp(-tr / t1)
+    a = math.radians(flip)
+    expected = 1j * pd * math.sin(a) * (1 - e1) / (1 - math.cos(a) * e1) * torch.exp(-float(seq.meta["te"]) / t2)
+    err = float((img - expected).abs().max() / expected.abs().max())
+    print(f"GPU GRE image vs steady-state closed form: {err:.2e}", flush=True)
+    assert err < 5e-4
