@@ -1,0 +1,173 @@
+"""FP64 CPU oracle of the bound-pool (magnetization transfer, rate model) path: a restatement of the reference's core
+with ``Phantom.bound``.
+
+TEST INFRASTRUCTURE. Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU leg may import or run
+anything under ``oracle/``, and only as the checker: the package ``blochparty_amd`` never imports it.
+
+What it computes, following ``src/blochparty/engines/core.py`` and ``exchange.py`` (file:line of the reference):
+
+- ``sl_F``            exchange.py:108-127  the super-Lorentzian integral F(x), here by adaptive quadrature (SciPy's
+                                          quad_vec) on both sides of the magic angle, no table
+- ``lineshape_G``     exchange.py:141-160  G(2 pi delta) [s]: Lorentzian, or T2b F(x) held constant inside the cut
+- ``z_exchange``      exchange.py:209-247  exact 2 x 2 longitudinal exchange over t: z_ss + exp(A t) (z0 - z_ss),
+                                          exp(A t) = e^{mt} (cosh(qt) I + sinh(qt)/q (A - m I)) (Cayley-Hamilton)
+- ``rf_segment_bound`` core.py:124-150     Strang: half transverse relaxation and half exchange (saturated at W), the
+                                          water's rotation, the other half
+- ``simulate_bound``  core.py:487-608     the plain path of ``core.simulate`` with the bound pool: free runs relax Mz
+                                          and Mzb by the unsaturated exchange over the run; returns m (N, 4)
+
+How each function is pinned (tests/test_exchange.py):
+- ``z_exchange``: SciPy's expm of the augmented 3 x 3 generator [[A, c], [0, 0]] (a library routine) to 1e-12, and the
+  steady state -A^-1 c it tends to;
+- ``sl_F`` / ``lineshape_G``: the normalisation int G dw = pi of an absorption lineshape (G = pi g, g of unit area
+  over rad/s) for both shapes (the package's Gauss-Legendre form of F is checked against this one);
+- ``rf_segment_bound``: a brute-force SciPy DOP853 integration of the coupled four-state equations on tiny inputs,
+  the Strang error falling by ~4 per halving of the steps;
+- ``simulate_bound``: with f = 0 it equals ``core.simulate`` (the plain path) to 1e-12.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import core
+
+U0 = 1.0 / math.sqrt(3.0)
+
+
+def sl_F(x: np.ndarray) -> np.ndarray:
+    """F(x) = sqrt(2 pi) int_0^1 exp(-2 x^2 / (3u^2 - 1)^2) / |3u^2 - 1| du for x > 0, vectorised, by adaptive
+    quadrature on [0, u0] and [u0, 1] (the integrand vanishes at the magic angle u0)."""
+    from scipy.integrate import quad_vec
+
+    x = np.atleast_1d(np.asarray(x, dtype=np.float64))
+
+    def f(u):
+        s = abs(3 * u * u - 1)
+        if s == 0.0:
+            return np.zeros_like(x)
+        return np.exp(-2 * x * x / (s * s)) / s
+
+    a = quad_vec(f, 0.0, U0, epsabs=0.0, epsrel=1e-12, limit=400)[0]
+    b = quad_vec(f, U0, 1.0, epsabs=0.0, epsrel=1e-12, limit=400)[0]
+    return math.sqrt(2 * math.pi) * (a + b)
+
+
+def lineshape_G(kind: str, delta: torch.Tensor, t2b: torch.Tensor, cutoff: float | None) -> torch.Tensor:
+    """G(2 pi delta) [s]."""
+    if kind == "lorentzian":
+        x = 2 * math.pi * delta * t2b
+        return t2b / (1 + x * x)
+    x = (2 * math.pi * delta.abs().clamp_min(cutoff) * t2b).detach().cpu().numpy()
+    return t2b * torch.as_tensor(sl_F(x), dtype=delta.dtype)
+
+
+def z_exchange(mz, mb, t, pd, r1, f, k, r1b, w=None):
+    """The exact longitudinal exchange step over ``t`` (saturation rate ``w`` of the bound pool, None: none)."""
+    a11, a12 = -(r1 + k * f), k
+    a21, a22 = k * f, -(r1b + k) if w is None else -(r1b + k + w)
+    c1, c2 = r1 * pd, r1b * f * pd
+    det = a11 * a22 - a12 * a21
+    z1, z2 = -(a22 * c1 - a12 * c2) / det, -(a11 * c2 - a21 * c1) / det
+    m, h = 0.5 * (a11 + a22), 0.5 * (a11 - a22)
+    q = torch.sqrt(h * h + a12 * a21)
+    em = torch.exp(m * t)
+    ch, sh = em * torch.cosh(q * t), torch.where(q > 0, em * torch.sinh(q * t) / torch.where(q > 0, q, 1.0), em * t)
+    d1, d2 = mz - z1, mb - z2
+    return z1 + ch * d1 + sh * (h * d1 + a12 * d2), z2 + ch * d2 + sh * (a21 * d1 - h * d2)
+
+
+class Pool:
+    def __init__(self, bound, n, dtype):
+        conv = lambda v: torch.as_tensor(v, dtype=dtype).expand(n)
+        self.f, self.k, self.r1b = conv(bound.f), conv(bound.k), 1.0 / conv(bound.t1)
+        self.t2b, self.dw = conv(bound.t2), conv(bound.dw)
+        self.kind, self.cutoff = bound.lineshape, bound.cutoff
+
+
+def rf_segment_bound(mx, my, mz, mb, dt, rf, g, off, f_rf, sp: core.Spins, pool: Pool, bound_only=False):
+    """One RF segment with the bound pool (core.py:124-150): the bound pool saturated at W = w1^2 G(f_rf - bz - dw)
+    over the whole segment, Strang-split around the water's rotation (no rotation for bound-only RF)."""
+    bz = sp.pos @ g + sp.db0
+    if off is not None:
+        bz = bz + off
+    w1sq = (2 * math.pi) ** 2 * (rf[0] ** 2 + rf[1] ** 2)
+    if sp.b1 is not None:
+        w1sq = w1sq * sp.b1 * sp.b1
+    sat = w1sq * lineshape_G(pool.kind, f_rf - bz - pool.dw, pool.t2b, pool.cutoff)
+
+    def half(mx, my, mz, mb):
+        e2 = torch.exp(-0.5 * dt * sp.r2)
+        mz, mb = z_exchange(mz, mb, 0.5 * dt, sp.pd, sp.r1, pool.f, pool.k, pool.r1b, sat)
+        return mx * e2, my * e2, mz, mb
+
+    mx, my, mz, mb = half(mx, my, mz, mb)
+    w = -core.TWO_PI * dt
+    wx, wy = (0.0 * w, 0.0 * w) if bound_only else (w * rf[0], w * rf[1])
+    if sp.b1 is not None and not bound_only:
+        wx, wy = wx * sp.b1, wy * sp.b1
+    mx, my, mz = core.rotate(mx, my, mz, wx + 0 * mx, wy + 0 * mx, w * bz)
+    return half(mx, my, mz, mb)
+
+
+def simulate_bound(phantom, seq, coils=None, *, m_init=None, rf_freq=None, dtype=torch.float64):
+    """``core.simulate`` with ``phantom.bound`` (rate model): returns (signal (C, n_adc), m (N, 4): mx, my, mz, mzb).
+    ``rf_freq``: (S,) each segment's RF frequency (default ``seq.rf_freq``, else the package's ``rf_frequency``
+    rule restated: minus the mean phase step per time to the RF neighbours)."""
+    pd = phantom.pd.to(dtype)
+    n = pd.shape[0]
+    conv = lambda t: None if t is None else t.to(dtype)
+    if coils is None:
+        sens_re, sens_im = torch.ones_like(pd)[:, None], torch.zeros_like(pd)[:, None]
+    else:
+        c = coils.to(torch.complex128)
+        sens_re, sens_im = c.real, c.imag
+    sp = core.Spins(conv(phantom.pos), conv(phantom.db0), pd, conv(phantom.t1), conv(phantom.t2), sens_re, sens_im,
+                    conv(phantom.b1))
+    pool = Pool(phantom.bound, n, dtype)
+    dt, rf, g, psi = conv(seq.dt), conv(seq.rf), conv(seq.grad), conv(seq.adc_phase)
+    off_all = conv(seq.b0_offset)
+    f_rf = conv(rf_freq) if rf_freq is not None else (conv(seq.rf_freq) if seq.rf_freq is not None
+                                                       else _rf_frequency(seq.is_rf, rf, dt))
+    if m_init is None:
+        mx, my, mz, mb = torch.zeros_like(pd), torch.zeros_like(pd), pd.clone(), pool.f * pd
+    else:
+        mi = conv(m_init)
+        mx, my, mz = mi[:, 0], mi[:, 1], mi[:, 2]
+        mb = mi[:, 3] if mi.shape[1] == 4 else pool.f * pd
+    bound_only = seq.bound_only or [False] * len(seq)
+    samples = []
+    for a, b in core.steps(seq.is_rf, seq.spoil, True):
+        if seq.is_rf[a]:
+            off = None if off_all is None else off_all[a]
+            mx, my, mz, mb = rf_segment_bound(mx, my, mz, mb, dt[a], rf[a], g[a], off, f_rf[a], sp, pool,
+                                              bool(bound_only[a]))
+            if seq.adc[a]:
+                samples.append(core.sample(mx, my, psi[a], sp)[None])
+        else:
+            idx = [i for i in range(b - a) if seq.adc[a + i]]
+            off = None if off_all is None else off_all[a:b]
+            mz0 = mz
+            mx, my, _, sig = core.free_run(mx, my, mz, dt[a:b], g[a:b], off, psi[a:b], idx, sp)
+            mz, mb = z_exchange(mz0, mb, dt[a:b].sum(), sp.pd, sp.r1, pool.f, pool.k, pool.r1b)
+            if sig is not None:
+                samples.append(sig)
+        if seq.spoil[b - 1]:
+            mx, my = torch.zeros_like(mx), torch.zeros_like(my)
+    sig = torch.cat(samples) if samples else pd.new_zeros(0, 2, sens_re.shape[1])
+    return torch.complex(sig[:, 0], sig[:, 1]).T, torch.stack([mx, my, mz, mb], dim=-1)
+
+
+def _rf_frequency(is_rf, rf, dt):
+    """Minus the phase step per unit time to each RF neighbour, averaged over the neighbours that carry RF."""
+    z = (rf[:, 0] + 1j * rf[:, 1]).numpy()
+    d = dt.numpy()
+    live = np.asarray(is_rf, dtype=bool) & (np.abs(z) > 0)
+    both = live[1:] & live[:-1]
+    step = np.where(both, -np.angle(z[1:] * np.conj(z[:-1])) / (math.pi * (d[1:] + d[:-1])), 0.0)
+    total = np.concatenate([[0.0], step]) + np.concatenate([step, [0.0]])
+    count = np.concatenate([[0.0], both]) + np.concatenate([both, [0.0]])
+    return torch.as_tensor(total / np.maximum(count, 1), dtype=dt.dtype)
