@@ -18,6 +18,7 @@ from .builder import (
     trapezoid,
 )
 from .coils import birdcage_sensitivities
+from .exchange import BoundPool
 from .engines import available, simulate
 from .library import gre_2d, gre_3d, gre_cartesian, tse_2d
 from .phantom import (
@@ -37,6 +38,7 @@ from .sequence import Sequence
 
 __all__ = [
     "ADC",
+    "BoundPool",
     "LABELS",
     "MAX_RF_STEP",
     "TISSUES",

@@ -1,0 +1,177 @@
+// The bound-pool (magnetization transfer, rate model) forward kernel. Replaces the reference's V = 1 variant of
+// bloch_chunk_mma for a rate bound pool (src/blochparty/engines/gpu/shaders/common.metal:113-208: lineshape, the
+// exchange step, step_mt).
+//
+// One lane per isochromat, workgroups of 256 (4 waves of 64), every lane walking the same plain segment records. Per
+// spin the free pool's (Mx, My, Mz) and the bound pool's Mzb; Mz and Mzb carry two-sum low parts.
+// - RF segment: Strang split. Half a segment of transverse decay and of longitudinal exchange, with the bound pool
+//   saturated at W = (2 pi b1 |rf|)^2 G(f_rf - bz - dw); the water's rotation (only the precession about z for RF
+//   that only the bound pool feels); the other half.
+// - Free segment: exact precession (float64 phase, as the plain kernels') and transverse decay; the unsaturated
+//   exchange over the segment.
+// - The exchange over t is exact: z_ss + exp(A t) (z - z_ss) for the 2 x 2 generator A of (Mz, Mzb), in increment
+//   form (exp(A t) - I)(z - z_ss) = (e^{mt} cosh(qt) - 1)(z - z_ss) + e^{mt} sinh(qt)/q (A - m I)(z - z_ss), with
+//   m +- q the eigenvalues and both factors from expm1 of m +- q, so that short steps keep their low bits.
+// - ADC: every sample's coil sums over the workgroup (wave butterflies, then the 4 waves in order in LDS) go to the
+//   partial rows that hb_signal_reduce sums in a fixed order: no atomics.
+#include "bloch_device.h"
+
+#include "../../include/blochparty_amd_mt.h"
+
+namespace hb {
+namespace {
+
+constexpr int kBlockM = 256;
+constexpr int kWavesM = kBlockM / 64;
+constexpr int kMaxCoilsM = 128;
+
+// The exchange step over t: the steady state and the two increment factors.
+struct XStep {
+    float z1, z2;      // steady state -A^-1 c
+    float chm1, sh;    // e^{mt} cosh(qt) - 1, e^{mt} sinh(qt) / q
+    float h, a12, a21; // A - m I = [[h, a12], [a21, -h]]
+};
+
+__device__ __forceinline__ XStep xstep(float t, float pd, float r1, float f, float k, float r1b, float w) {
+    const float kf = k * f;
+    const float a11 = -(r1 + kf), a12 = k, a21 = kf, a22 = -(r1b + k + w);
+    const float c1 = r1 * pd, c2 = r1b * f * pd;
+    const float det = a11 * a22 - a12 * a21;
+    XStep x;
+    x.z1 = (a12 * c2 - a22 * c1) / det;
+    x.z2 = (a21 * c1 - a11 * c2) / det;
+    const float m = 0.5f * (a11 + a22), h = 0.5f * (a11 - a22);
+    const float q = sqrtf(fmaf(h, h, a12 * a21));
+    x.chm1 = 0.5f * (expm1x((m + q) * t) + expm1x((m - q) * t));
+    const float lo = __expf((m - q) * t);
+    if (2.0f * q * t < 1.0f)
+        x.sh = q > 0.0f ? lo * 0.5f * expm1x(2.0f * q * t) / q : t * __expf(m * t);
+    else
+        x.sh = 0.5f * (__expf((m + q) * t) - lo) / q;
+    x.h = h, x.a12 = a12, x.a21 = a21;
+    return x;
+}
+
+__device__ __forceinline__ void exchange(const XStep& x, State& m, float& mb, float& lb) {
+    const float d1 = m.mz - x.z1, d2 = mb - x.z2;
+    two_sum(m.mz, m.lz, x.chm1 * d1 + x.sh * (x.h * d1 + x.a12 * d2));
+    two_sum(mb, lb, x.chm1 * d2 + x.sh * (x.a21 * d1 - x.h * d2));
+}
+
+// G(2 pi delta) [s]: the Lorentzian, or T2b F(x) from the table of ln F (held at the cut, 0 past the table's end).
+__device__ __forceinline__ float lineshape_g(const hb_mt& mt, float delta, float t2b) {
+    if (mt.lineshape == HB_LINE_LORENTZIAN) {
+        const float x = HB_TWO_PI * delta * t2b;
+        return t2b / fmaf(x, x, 1.0f);
+    }
+    const float x = HB_TWO_PI * fmaxf(fabsf(delta), mt.cutoff) * t2b;
+    const float pos = (logf(x) - mt.lx0) / mt.dlx;
+    if (!(pos <= (float)(mt.n_table - 1))) return 0.0f;
+    const float pc = fmaxf(pos, 0.0f);
+    const int i = min((int)pc, mt.n_table - 2);
+    const float a = pc - (float)i;
+    return t2b * expf(mt.table[i] * (1.0f - a) + mt.table[i + 1] * a);
+}
+
+// Rotation of the state by the rotation vector w (right-handed, angle |w|), as rf_step's: m + a (w x m) + b w x (w x m).
+__device__ __forceinline__ void rotate_state(State& m, float wx, float wy, float wz) {
+    float a, b;
+    rot_ab(wx * wx + wy * wy + wz * wz, a, b);
+    const float cx = wy * m.mz - wz * m.my, cy = wz * m.mx - wx * m.mz, cz = wx * m.my - wy * m.mx;
+    const float ex = wy * cz - wz * cy, ey = wz * cx - wx * cz, ez = wx * cy - wy * cx;
+    m.mx += a * cx + b * ex;
+    m.my += a * cy + b * ey;
+    two_sum(m.mz, m.lz, a * cz + b * ez);
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(kBlockM) void forward_mt_kernel(hb_case cs, hb_mt mt, int s0, int s1,
+                                                             float* __restrict__ partial, int rows, int row0) {
+    __shared__ float red[kWavesM][2 * kMaxCoilsM];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = blockIdx.x * kBlockM + tid;
+    const int np = cs.n_pad;
+    const Spin p = load_spin(cs.spins, cs.pos_lo, np, j);
+    const Pos64 r64 = pos64(p);
+    const float f = mt.pool[j], k = mt.pool[np + j], r1b = mt.pool[2 * np + j], t2b = mt.pool[3 * np + j];
+    const float dw = mt.pool[4 * np + j];
+    const float b1sq = p.b1 * p.b1;
+    State m = load_state(cs.state, np, j);
+    float mb = mt.mb[j], lb = mt.mb[np + j];
+    const int C = cs.n_coils, vals = 16 * cs.n_tiles;
+    for (int s = s0; s < s1; ++s) {
+        const hb_segment q = cs.segs[s];
+        const int fl = q.flags;
+        if (fl & HB_RF) {
+            const float bz = offset(p, q);
+            const float w1sq = HB_TWO_PI * HB_TWO_PI * b1sq * (q.rf_x * q.rf_x + q.rf_y * q.rf_y);
+            const float sat = w1sq * lineshape_g(mt, mt.rf_freq[s] - bz - dw, t2b);
+            const XStep x = xstep(0.5f * q.dt, p.pd, p.r1, f, k, r1b, sat);
+            const float e2 = expm1x(-0.5f * q.dt * p.r2);
+            m.mx += m.mx * e2, m.my += m.my * e2;
+            exchange(x, m, mb, lb);
+            const float wz = p.px * q.w_g_x + p.py * q.w_g_y + p.pz * q.w_g_z + (p.db0 * q.w + q.w_b0);
+            if (fl & HB_BOUND_ONLY)
+                rotate_state(m, 0.0f, 0.0f, wz);
+            else
+                rotate_state(m, p.b1 * q.w_rf_x, p.b1 * q.w_rf_y, wz);
+            m.mx += m.mx * e2, m.my += m.my * e2;
+            exchange(x, m, mb, lb);
+        } else {
+            float sn, cs_;
+            sincospi(-2.0f * free_cycles(r64, p, q), &sn, &cs_);
+            const float e2 = expm1x(-q.dt * p.r2);
+            const float nx = cs_ * m.mx - sn * m.my, ny = sn * m.mx + cs_ * m.my;
+            m.mx = nx + nx * e2;
+            m.my = ny + ny * e2;
+            exchange(xstep(q.dt, p.pd, p.r1, f, k, r1b, 0.0f), m, mb, lb);
+        }
+        if (fl & HB_ADC) {  // the workgroup's coil sums of this sample (uniform branch: every lane takes it)
+            const int r = q.pad[0] - row0;
+            for (int c = 0; c < C; ++c) {
+                const float sr = mt.sens[(size_t)(2 * c) * np + j], si = mt.sens[(size_t)(2 * c + 1) * np + j];
+                const float re = wave_sum(sr * m.mx - si * m.my), im = wave_sum(sr * m.my + si * m.mx);
+                if (lane == 0) red[wave][2 * c] = re, red[wave][2 * c + 1] = im;
+            }
+            __syncthreads();
+            if (r >= 0 && r < rows && tid < vals) {
+                float v = 0.0f;
+                if (tid < 2 * C) {
+                    v = red[0][tid];
+#pragma unroll
+                    for (int w = 1; w < kWavesM; ++w) v += red[w][tid];
+                }
+                partial[((size_t)blockIdx.x * rows + r) * vals + tid] = v;
+            }
+            __syncthreads();
+        }
+        if (fl & HB_SPOIL) m.mx = m.my = 0.0f;
+    }
+    store_state(cs.state, np, j, m);
+    mt.mb[j] = mb, mt.mb[np + j] = lb;
+}
+
+}  // namespace
+}  // namespace hb
+
+extern "C" int hb_forward_mt(const hb_case* cs, const hb_mt* mt, int32_t s0, int32_t s1, float* partial,
+                             int32_t rows, int32_t row0, void* stream) {
+    using namespace hb;
+    if (!cs || !mt || !mt->pool || !mt->mb || !mt->sens || !mt->rf_freq) return hipErrorInvalidValue;
+    if (cs->n_pad <= 0 || cs->n_pad % kBlockM || cs->n < 0 || cs->n > cs->n_pad) return hipErrorInvalidValue;
+    if (cs->n_coils < 1 || cs->n_coils > kMaxCoilsM || cs->n_tiles < 1 || 8 * cs->n_tiles < cs->n_coils)
+        return hipErrorInvalidValue;
+    if (s0 < 0 || s1 < s0 || s1 > cs->n_segments || rows < 0 || row0 < 0) return hipErrorInvalidValue;
+    if (mt->lineshape == HB_LINE_SUPERLORENTZIAN && (!mt->table || mt->n_table < 2 || !(mt->dlx > 0.0f)))
+        return hipErrorInvalidValue;
+    if (mt->lineshape != HB_LINE_SUPERLORENTZIAN && mt->lineshape != HB_LINE_LORENTZIAN) return hipErrorInvalidValue;
+    if (s1 == s0) return hipSuccess;
+    hipLaunchKernelGGL(forward_mt_kernel, dim3(cs->n_pad / kBlockM), dim3(kBlockM), 0, (hipStream_t)stream, *cs,
+                       *mt, s0, s1, partial, rows, row0);
+    return hipGetLastError();
+}

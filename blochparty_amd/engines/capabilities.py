@@ -2,8 +2,8 @@
 
 Reference: ``src/blochparty/engines/capabilities.py:12-91``. The feature names and the message are the reference's;
 the ``hip`` row is this engine's: the plain Bloch path with B1+ and an initial magnetization, forward and with
-gradients. Everything else (bound / free pools, diffusion, motion, the scanner field model, field maps, bound-only
-RF) is refused with ``NotImplementedError``.
+gradients; a rate-model bound pool and bound-only RF segments forward. Everything else (the generalized bound pool,
+free pools, diffusion, motion, the scanner field model, field maps) is refused with ``NotImplementedError``.
 """
 
 from __future__ import annotations
@@ -21,7 +21,7 @@ FEATURES = {
     "m_init": "an initial magnetization (m_init)",
 }
 SUPPORTED = {
-    ("hip", "forward"): frozenset({"b1", "m_init"}),
+    ("hip", "forward"): frozenset({"b1", "m_init", "bound", "bound_only"}),
     ("hip", "gradients"): frozenset({"b1", "m_init"}),
 }
 

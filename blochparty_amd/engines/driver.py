@@ -385,8 +385,8 @@ class DeviceCase:
 
 
 def check_inputs(phantom: Phantom, coils) -> None:
-    for what, present in (("a bound pool", phantom.bound is not None), ("free pools", bool(phantom.pools)),
-                          ("diffusion", phantom.diffusion is not None), ("motion", phantom.motion is not None)):
+    for what, present in (("free pools", bool(phantom.pools)), ("diffusion", phantom.diffusion is not None),
+                          ("motion", phantom.motion is not None)):
         if present:
             raise NotImplementedError(f"hip engine: {what} is not implemented")
     if coils is not None and coils.shape[1] > hip.MAX_COILS:
@@ -405,6 +405,10 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
 
     reuse = reuse or surrogate
     check_inputs(phantom, coils)
+    bound = phantom.bound is not None  # the bound-pool kernel (engines/mt.py): plain records only
+    if bound and (reuse or nufft is not None or aligned):
+        raise NotImplementedError("hip engine: a bound pool runs forward only, with pulses='step' and "
+                                  "readouts='direct'")
     dev = device if device is not None else hip.HIP.device
     n = len(phantom)
     n_pad = max(BLOCK, math.ceil(n / BLOCK) * BLOCK)
@@ -467,7 +471,11 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
     adc_cum = np.concatenate([[0], np.cumsum(rows_w)])
     row_starts = [int(adc_cum[s]) for s in starts]
     rows = [int(adc_cum[e] - adc_cum[s]) for s, e in zip(starts, ends)]
-    rec = segment_table(seq, starts, reused, runs=not aligned, nufft=plan)
+    rec = segment_table(seq, starts, reused, runs=not aligned and not bound, nufft=plan)
+    if bound:
+        from . import mt
+
+        mt.mark_bound_only(rec, seq)
     covered = np.zeros(len(seq), dtype=bool)  # segments applied as reused-pulse maps
     for _, k, occ in reused:
         for a, _ in occ:
@@ -502,6 +510,8 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
     )
     if plan is not None:
         attach_nufft(case, plan, order, seq, adc_np, starts, ends, dev)
+    if bound:
+        mt.attach(case, phantom, seq, coils, m_init, n, n_pad, dev)
     if surrogate and case.n_pulses:
         from . import surrogate as sur
 
@@ -551,6 +561,11 @@ def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> l
     called as timer("begin") / timer("end") around every forward-kernel launch (the benchmark records events)."""
     lib = lib or hip.HIP.lib()
     case.state.copy_(case.state0)
+    mt_bufs = case.extra.get("mt")
+    if mt_bufs is not None:
+        from . import mt
+
+        mt_bufs["mb"].copy_(mt_bufs["mb0"])
     st = case.struct()
     n_blocks = case.n_pad // BLOCK
     ckpts = [] if keep else None
@@ -572,8 +587,11 @@ def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> l
         st.rf_free = int(bool(case.rf_free) and case.rf_free[d])
         if timer:
             timer("begin")
-        lib.forward(hip.ctypes.byref(st), s0, s1, case.partial.data_ptr(), nr,
-                    None if nu_arg is None else hip.ctypes.byref(nu_arg))
+        if mt_bufs is not None:
+            mt.forward_dispatch(case, st, s0, s1, nr, r0)
+        else:
+            lib.forward(hip.ctypes.byref(st), s0, s1, case.partial.data_ptr(), nr,
+                        None if nu_arg is None else hip.ctypes.byref(nu_arg))
         if timer:
             timer("end")
         if nr:
@@ -595,11 +613,15 @@ def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> l
 
 
 def read_result(case: DeviceCase) -> SimResult:
-    """(C, n_adc) complex64 signal and (N, 3) float32 magnetization on the CPU (the GPU engines' contract)."""
+    """(C, n_adc) complex64 signal and (N, 3) float32 magnetization on the CPU (the GPU engines' contract); (N, 4)
+    with a bound pool, its Mzb last."""
     sig = case.sig[: case.n_adc].cpu()
     signal = torch.complex(sig[..., 0], sig[..., 1]).T.to(torch.complex64).contiguous()
     st = case.state.cpu().double()
     m = (st[:3, : case.n] + st[3:, : case.n]).T.float().contiguous()
+    if case.extra.get("mt") is not None:
+        mb = case.extra["mt"]["mb"].cpu().double()
+        m = torch.cat([m, (mb[0, : case.n] + mb[1, : case.n]).float()[:, None]], 1).contiguous()
     if case.perm is not None:  # back to the caller's spin order
         inv = np.empty_like(case.perm)
         inv[case.perm] = np.arange(len(case.perm))

@@ -25,7 +25,7 @@ class Phantom:
     t2: Tensor  # (N,) [s]
     db0: Tensor  # (N,) off-resonance [Hz]
     b1: Tensor | None = None  # (N,) transmit scale of every RF pulse; None = 1
-    bound: object | None = None  # magnetization transfer pool (refused by this engine)
+    bound: object | None = None  # magnetization transfer: a bound pool per spin (exchange.BoundPool)
     pools: tuple | None = None  # free exchanging pools (refused)
     diffusion: Tensor | None = None  # (refused)
     restriction: object | None = None  # (refused)
