@@ -43,6 +43,7 @@ their relative errors against the oracle. Progress goes to stderr.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -248,7 +249,7 @@ def end_to_end(bp, W, args, ph, seq, coils, work: int, steps: int, warmup: int) 
     log(f"end to end: simulate() forward {t / steps:.2f} s/call, {work * steps / t:.3e} spin-samples/s")
     # the grad case: the flip as a tensor through the sequence builder, the T1 map and the coils as leaves
     flip = torch.tensor(FLIP_DEG, dtype=torch.float64, requires_grad=True)
-    seq_g = W.gre3d_sequence(args.n, flip_deg=flip)
+    shape = seq.rf / FLIP_DEG  # the pulse's samples per degree (the builder's rf is linear in the flip angle)
     t1 = ph.t1.detach().clone().requires_grad_()
     ph_g = bp.Phantom(ph.pos, ph.pd, t1, ph.t2, ph.db0, b1=ph.b1)
     coils_g = coils.detach().clone().requires_grad_()
@@ -256,6 +257,7 @@ def end_to_end(bp, W, args, ph, seq, coils, work: int, steps: int, warmup: int) 
     def fwd_bwd():
         for x in (flip, t1, coils_g):
             x.grad = None
+        seq_g = dataclasses.replace(seq, rf=shape * flip)  # a fresh graph from the flip angle per call
         sig = bp.simulate(ph_g, seq_g, coils_g, rf_grad="scale", **opts).signal
         (sig.abs() ** 2).sum().backward()
         assert torch.isfinite(flip.grad) and torch.isfinite(t1.grad).all() and torch.isfinite(coils_g.grad).all()

@@ -52,7 +52,7 @@ BATCH = 8192
 LR = (3e-3, 1e-5)  # cosine from the first to the second
 MARGIN = 0.05  # the box extends the phantom's range by this fraction of its width on each side
 MIN_HALF = (1.0, 1.0, 1e-6)  # smallest half-widths of the box: df [Hz], u [Hz], b1 x peak [Hz]
-MIN_REL_B1 = 0.05  # the box's B1 x peak half-width is at least this fraction of its centre
+MIN_REL_B1 = 0.01  # the box's B1 x peak half-width is at least this fraction of its centre
 JUDGE_STRIDE = 257
 JUDGE_TOL = 1e-3
 
