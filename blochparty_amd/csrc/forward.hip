@@ -22,6 +22,8 @@
 // below 1/4), so D is as close to the exact sum as a float32 product would be, at six 16-cycle instructions per tile
 // and flush in place of sixteen 32-cycle v_mfma_f32_16x16x4f32. More tiles use the float32 instruction, with the
 // sensitivities read from memory.
+#include <mutex>
+
 #include "bloch_device.h"
 
 namespace hb {
@@ -485,6 +487,55 @@ extern "C" int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t 
     return hipGetLastError();
 }
 
+// ---- hb_signal_reduce_noalloc: hb_signal_reduce with its float64 chunk sums in a fixed device array.
+//
+// `scratch` is a file-scope device array that each GPU the library runs on holds until the process ends. Host code
+// never passes its name to a kernel (in host code the name is a host-side stand-in, not a device address): the
+// first call on a device resolves the array's address on that device with hipGetSymbolAddress, and every launch
+// uses that address. Calls on one GPU share its array; the chunk sums are written in full before they are read,
+// so nothing needs clearing, but calls that overlap on one GPU (other streams, other threads) race.
+static __device__ __align__(256) double scratch[HB_NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+double* scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch` (resolved once per device), or nullptr with *err set.
+double* device_scratch(hipError_t* err) {
+    int dev = 0;
+    *err = hipGetDevice(&dev);
+    if (*err != hipSuccess) return nullptr;
+    if (dev < 0 || dev >= kMaxDevices) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        *err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (*err != hipSuccess) return nullptr;
+        scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" int hb_signal_reduce_noalloc(const float* partial, int32_t n_blocks, int32_t rows, int32_t n_tiles,
+                                        int32_t n_coils, const double* psi, int32_t row0, const int32_t* row_map,
+                                        double* sig, void* stream) {
+    using namespace hb;
+    if (rows < 0 || n_blocks < 1 || n_tiles < 1 || n_tiles > kMaxTiles || n_coils < 1 || 8 * n_tiles < n_coils ||
+        row0 < 0)
+        return hipErrorInvalidValue;
+    if (hb_signal_reduce_workspace(n_blocks, rows, n_tiles) > HB_NOALLOC_SCRATCH_DOUBLES)
+        return hipErrorInvalidValue;  // beyond the fixed array: use hb_signal_reduce with a workspace
+    hipError_t err = hipSuccess;
+    double* work = device_scratch(&err);
+    if (!work) return err;
+    return hb_signal_reduce(partial, n_blocks, rows, n_tiles, n_coils, psi, row0, row_map, work, sig, stream);
+}
+
 extern "C" const char* hb_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 
-extern "C" int hb_abi_version(void) { return 19; }
+extern "C" int hb_abi_version(void) { return 20; }

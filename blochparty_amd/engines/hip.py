@@ -15,7 +15,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent.parent / "libblochparty_amd.so"
-ABI_VERSION = 19
+ABI_VERSION = 20
 MAX_COILS = 128  # 16 tiles of 8 coils (the forward kernel's LDS budget)
 
 
@@ -87,6 +87,7 @@ _P, _I, _V = ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p
 SIGNATURES = {
     "hb_forward": [ctypes.POINTER(HbCase), _I, _I, _P, _I, _P, _V],
     "hb_signal_reduce": [_P, _I, _I, _I, _I, _P, _I, _P, _P, _P, _V],
+    "hb_signal_reduce_noalloc": [_P, _I, _I, _I, _I, _P, _I, _P, _P, _V],
     "hb_nufft_grid": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _P, _P, _P, _P, _V],
     "hb_nufft_signal": [_P, _I, _P, _P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _V],
     "hb_nufft_adjoint_grid": [_P, _I, _P, _I, _I, _I, _I, _P, _P, _P, _V],

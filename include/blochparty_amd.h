@@ -121,6 +121,19 @@ int hb_signal_reduce(const float* partial, int32_t n_blocks, int32_t rows, int32
                      const double* psi, int32_t row0, const int32_t* row_map, double* work, double* sig,
                      void* stream);
 
+/* hb_signal_reduce without a workspace argument, for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc); no counterpart in the reference. It calls no hipMalloc or
+ * hipFree: the chunk sums go to a fixed device array of HB_NOALLOC_SCRATCH_DOUBLES doubles (16 MiB) that each GPU
+ * the library runs on holds until the process ends, its address resolved once per device on the first call there.
+ * Same arguments and results as hb_signal_reduce (the same kernels, bit for bit). hipErrorInvalidValue when
+ * hb_signal_reduce_workspace(n_blocks, rows, n_tiles) exceeds HB_NOALLOC_SCRATCH_DOUBLES (e.g. more than 8,192
+ * rows of one tile with 4,096 blocks). Calls on one GPU share the array: calls that overlap there (other streams,
+ * other threads) race. */
+#define HB_NOALLOC_SCRATCH_DOUBLES (1 << 21)
+int hb_signal_reduce_noalloc(const float* partial, int32_t n_blocks, int32_t rows, int32_t n_tiles,
+                             int32_t n_coils, const double* psi, int32_t row0, const int32_t* row_map, double* sig,
+                             void* stream);
+
 /* The NUFFT grids of a dispatch's runs, float64 (replaces the float atomics of the reference's grid rows): for slot
  * r < n_slots (run run_of[r], kind kind_of[run]), term m, cell u < ng and value v < 16 n_tiles,
  * grid[((r * terms + m) * ng + u) * vals + v] = sum over the entries (block b, local cell lc) of cell u's list, in
