@@ -4,10 +4,33 @@
 // A reused pulse of K substeps is cut into M = HB_SUR_BLOCKS blocks. Each block's pure rotation is a unit quaternion
 // taken in the frame that turns with the off-resonance precession: Q~_j = Rz(-theta_j/2) Q_j Rz(-theta_j/2),
 // theta_j = -2 pi (f tau_j + B_j) the block's free-precession angle (f the spin's offset, tau_j the block's length,
-// B_j its b0_offset area), so that Q~_j varies slowly with f. A network 3 -> 64 -> 64 -> 4 M (tanh) maps a spin's
-// (df, u, b1) (df its off-resonance [Hz], u = g . r its gradient offset during the pulse [Hz], b1 its B1 scale) to
-// the M quaternions. A spin's pulse map m -> m + D m + v is then composed in float64 with the relaxation put back
-// exactly, half a block before and after each rotation: a Strang split of the pulse into M blocks.
+// B_j its b0_offset area), so that Q~_j varies slowly with f. A network 10 -> 64 -> 64 -> 4 M (tanh) maps features
+// of a spin's (f, b) (f = df + g . r its offset during the pulse [Hz], b its B1 scale times the pulse's peak [Hz]) to
+// the M quaternions. The features are f and b scaled into the trained box and sin, cos of 2 pi f q S, q = 1..4,
+// S = the mean block length / 8: to first order in the RF, block j's rotation is the sum over its substeps of
+// rf_k e^{i 2 pi f (t_k - t_j)}, t_k - t_j within half a block of the block's middle, and these frequencies span
+// that range. The network does not learn that first-order part itself: a block's quaternion is the first-order
+// rotation exp(b F_j(f) / 2), F_j tabulated per network over its box's f range (first_order_kernel) and
+// interpolated, plus the network's output (the higher orders, two orders of magnitude smaller in x and y). A spin's
+// pulse map m -> m + D m + v is then composed in float64 with the relaxation put back exactly, half a block before
+// and after each rotation: a Strang split of the pulse into M blocks.
+//
+// First order: with v_k = -2 pi dt_k (b rf_k, f + off_k) substep k's rotation vector, Phi_k the z angle up to the
+// middle of substep k and theta the block's, Q = Rz(theta) prod_k Rz(-Phi_k) exp(v_k,xy) Rz(Phi_k) (each substep
+// split symmetrically about its z part, exact to first order), so to first order in the RF
+// Q~ = Rz(-theta/2) Q Rz(-theta/2) = exp(F), F = sum_k Rz(theta/2 - Phi_k) v_k,xy.
+//
+// The split's correction. To first order in the relaxation rates (Duhamel's formula; generator Rg = diag(R2, R2, R1),
+// which commutes with Rz), the block's exact map differs from relax(tau/2) U relax(tau/2) by U times the error of the
+// trapezoidal rule on int_0^tau V(s)^T Rg V(s) ds (V(s) the rotation from the block's start to s in the frame turning
+// with the precession; its part along z commutes with Rg), and on the offset by U times that error on
+// int V(s)^T z ds R1 PD. With v(s) = V's rotation vector in the block's middle frame, to second order in v:
+// linear part Rz(theta/2) Q~ (R2 - R1) (S(H) - T(G)) Rz(theta/2), offset Rz(theta/2) Q~ R1 PD (H_y, -H_x, -tr(G) / 2),
+// where H = sum_k s_k u_k is the first time moment of F's terms u_k (s_k from the block's middle), G =
+// int v v^T ds - (tau/2) v(tau) v(tau)^T the 2 x 2 second moment of the path, S(H) the symmetric matrix with
+// (0, 2) = (2, 0) = H_y, (1, 2) = (2, 1) = -H_x, and T(G) = [[-G_yy, G_xy, 0], [G_xy, -G_xx, 0], [0, 0, tr G]]. It
+// matters where a block's precession angle is large (spins far from the pulse's band, whose magnetization nutates to
+// and fro within a block); H and G are tabulated beside F.
 //
 // Kernels: the exact spinor products that make the training data (float64); the network's training step (a fused
 // per-sample forward + backward, per-chunk weight gradients on the matrix cores summed in a fixed order, Adam); its
@@ -20,7 +43,9 @@ namespace hb {
 constexpr int kH = HB_MLP_HIDDEN;
 constexpr int kM = HB_SUR_BLOCKS;
 constexpr int kOut = 4 * kM;
-constexpr int kW1 = 0, kB1 = kW1 + kH * 3, kW2 = kB1 + kH, kB2 = kW2 + kH * kH, kW3 = kB2 + kH,
+constexpr int kIn = HB_MLP_INPUTS, kFourier = HB_MLP_FOURIER, kNormHead = HB_MLP_NORM_HEAD;
+constexpr int kRaw = 2;  // floats per raw input: f [Hz], b [Hz]
+constexpr int kW1 = 0, kB1 = kW1 + kH * kIn, kW2 = kB1 + kH, kB2 = kW2 + kH * kH, kW3 = kB2 + kH,
               kB3 = kW3 + kOut * kH, kParams = kB3 + kOut;
 static_assert(kParams == HB_MLP_PARAMS, "parameter layout");
 constexpr int kChunk = 512;  // samples per weight-gradient partial
@@ -56,14 +81,14 @@ __global__ __launch_bounds__(256) void spinor_blocks_kernel(const double* __rest
                                                             int n, float* __restrict__ y) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double df = x[3 * i], u = x[3 * i + 1], b1 = x[3 * i + 2];
+    const double f0 = x[kRaw * i], b1 = x[kRaw * i + 1];
     const double w0 = -2.0 * M_PI;
     for (int j = 0; j < kM; ++j) {
         Quat q = {1.0, 0.0, 0.0, 0.0};
         double theta = 0.0;
         for (int k = bounds[j]; k < bounds[j + 1]; ++k) {
             const double* r = table + (size_t)k * 8;
-            const double w = w0 * r[0], f = df + u + r[6];
+            const double w = w0 * r[0], f = f0 + r[6];
             q = qmul(rot_quat(w * b1 * r[1], w * b1 * r[2], w * f), q);
             theta += w * f;
         }
@@ -71,6 +96,91 @@ __global__ __launch_bounds__(256) void spinor_blocks_kernel(const double* __rest
         q = qmul(h, qmul(q, h));
         float* out = y + (size_t)i * kOut + 4 * j;
         out[0] = (float)q.w, out[1] = (float)q.x, out[2] = (float)q.y, out[3] = (float)q.z;
+    }
+}
+
+// ---- the first-order part ----
+
+// first[(j * n_grid + i) * 8 + {0..6}] = F_j (x, y), H_j (x, y) per Hz of b and G_j (xx, xy, yy) per Hz^2 of b at
+// f_lo + i df, float64 sums over the block's substeps (v(s) growing linearly within each substep); entry 7 is 0.
+__global__ __launch_bounds__(256) void first_order_kernel(const double* __restrict__ table,
+                                                          const int* __restrict__ bounds, double f_lo, double df,
+                                                          int n_grid, float* __restrict__ first) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= kM * n_grid) return;
+    const int j = t / n_grid, i = t - j * n_grid;
+    const double f = f_lo + i * df;
+    double z_total = 0.0;  // sum dt (f + off): theta = -2 pi z_total
+    for (int k = bounds[j]; k < bounds[j + 1]; ++k) z_total += table[(size_t)k * 8] * (f + table[(size_t)k * 8 + 6]);
+    double tau = 0.0;
+    for (int k = bounds[j]; k < bounds[j + 1]; ++k) tau += table[(size_t)k * 8];
+    double z = 0.0, time = 0.0, rx = 0.0, ry = 0.0, hx = 0.0, hy = 0.0, gxx = 0.0, gxy = 0.0, gyy = 0.0;
+    for (int k = bounds[j]; k < bounds[j + 1]; ++k) {
+        const double* r = table + (size_t)k * 8;
+        const double dt = r[0], zk = dt * (f + r[6]);
+        const double phi = -2.0 * M_PI * (z + 0.5 * zk - 0.5 * z_total);  // Phi_k - theta / 2
+        double sn, cs;
+        sincos(phi, &sn, &cs);
+        const double ax = -2.0 * M_PI * dt * r[1], ay = -2.0 * M_PI * dt * r[2];
+        const double ux = cs * ax + sn * ay, uy = cs * ay - sn * ax;  // Rz(-phi) (ax, ay)
+        const double sk = time + 0.5 * dt - 0.5 * tau;               // substep middle - block middle
+        hx += sk * ux, hy += sk * uy;
+        gxx += dt * (rx * rx + rx * ux + ux * ux / 3.0);  // int over the substep of v v^T, v = (r + a u), a 0 -> 1
+        gxy += dt * (rx * ry + 0.5 * (rx * uy + ux * ry) + ux * uy / 3.0);
+        gyy += dt * (ry * ry + ry * uy + uy * uy / 3.0);
+        rx += ux, ry += uy;
+        z += zk;
+        time += dt;
+    }
+    float* o = first + (size_t)t * 8;
+    o[0] = (float)rx, o[1] = (float)ry, o[2] = (float)hx, o[3] = (float)hy;
+    o[4] = (float)(gxx - 0.5 * tau * rx * rx), o[5] = (float)(gxy - 0.5 * tau * rx * ry);
+    o[6] = (float)(gyy - 0.5 * tau * ry * ry), o[7] = 0.0f;
+}
+
+// Block j's table entries at f (F_j, H_j per Hz of b, G_j per Hz^2 of b), interpolated linearly on the grid over
+// [c_f - h_f, c_f + h_f] (clamped at its ends).
+template <int NV>
+__device__ __forceinline__ void first_lookup(const float* __restrict__ first, int n_grid,
+                                             const float* __restrict__ norm, float f, int j, float (&v)[NV]) {
+    const float lo = norm[0] - norm[1], step = 2.0f * norm[1] / (float)(n_grid - 1);
+    const float pos = fminf(fmaxf((f - lo) / step, 0.0f), (float)(n_grid - 1));
+    const int i = min((int)pos, n_grid - 2);
+    const float a = pos - (float)i;
+    const float* g = first + ((size_t)j * n_grid + i) * 8;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) v[c] = fmaf(a, g[8 + c] - g[c], g[c]);
+}
+
+// Block j's first-order quaternion at (f, b [Hz]): the rotation by b F_j. Zero without a table (the network then
+// learns the whole quaternion).
+__device__ __forceinline__ void first_quat(const float* __restrict__ first, int n_grid, const float* __restrict__ norm,
+                                           float f, float b, int j, float (&q)[4]) {
+    if (!first) {
+        q[0] = q[1] = q[2] = q[3] = 0.0f;
+        return;
+    }
+    float v[2];
+    first_lookup(first, n_grid, norm, f, j, v);
+    const float rx = b * v[0], ry = b * v[1];
+    const float ang = sqrtf(rx * rx + ry * ry);
+    float sn, cs;
+    sincosf(0.5f * ang, &sn, &cs);
+    const float k = ang > 0.0f ? sn / ang : 0.5f;
+    q[0] = cs, q[1] = k * rx, q[2] = k * ry, q[3] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void first_quats_kernel(const float* __restrict__ first, int n_grid,
+                                                          const float* __restrict__ norm, const float* __restrict__ x,
+                                                          int n, float* __restrict__ q) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const float f = x[kRaw * s], b = x[kRaw * s + 1] * norm[5];
+    for (int j = 0; j < kM; ++j) {
+        float v[4];
+        first_quat(first, n_grid, norm, f, b, j, v);
+        float* o = q + (size_t)s * kOut + 4 * j;
+        o[0] = v[0], o[1] = v[1], o[2] = v[2], o[3] = v[3];
     }
 }
 
@@ -82,10 +192,14 @@ __device__ __forceinline__ void stage_params(float* __restrict__ lds, const floa
     __syncthreads();
 }
 
-__device__ __forceinline__ void layer1(const float* __restrict__ w, const float (&x)[3], float (&a1)[kH]) {
+__device__ __forceinline__ void layer1(const float* __restrict__ w, const float (&x)[kIn], float (&a1)[kH]) {
 #pragma unroll
-    for (int i = 0; i < kH; ++i)
-        a1[i] = tanhf(w[kB1 + i] + w[kW1 + 3 * i] * x[0] + w[kW1 + 3 * i + 1] * x[1] + w[kW1 + 3 * i + 2] * x[2]);
+    for (int i = 0; i < kH; ++i) {
+        float acc = w[kB1 + i];
+#pragma unroll
+        for (int c = 0; c < kIn; ++c) acc += w[kW1 + kIn * i + c] * x[c];
+        a1[i] = tanhf(acc);
+    }
 }
 
 __device__ __forceinline__ void layer2(const float* __restrict__ w, const float (&a1)[kH], float (&a2)[kH]) {
@@ -113,16 +227,19 @@ __device__ __forceinline__ float layer3(const float* __restrict__ w, const float
     return acc;
 }
 
-__device__ __forceinline__ void normalise_input(const float* __restrict__ norm, float df, float u, float b1,
-                                                float (&x)[3]) {
-    x[0] = (df - norm[0]) / norm[3];
-    x[1] = (u - norm[1]) / norm[4];
-    x[2] = (b1 - norm[2]) / norm[5];
+// The network's inputs from a spin's (f, b): norm = c_f, h_f, c_b, h_b, S. The features' phases 2 pi f q S are taken
+// in turns by sincospif (|2 f q S| stays within a few turns over a box, where its float32 result is accurate).
+__device__ __forceinline__ void features(const float* __restrict__ norm, float f, float b, float (&x)[kIn]) {
+    x[0] = (f - norm[0]) / norm[1];
+    x[1] = (b - norm[2]) / norm[3];
+    const float t = 2.0f * f * norm[4];
+#pragma unroll
+    for (int q = 1; q <= kFourier; ++q) sincospif(t * (float)q, &x[2 * q], &x[2 * q + 1]);
 }
 
 // One sample per lane: forward, the loss's output deltas d3 = 2 (out - t) / (n n_out) (t the normalised target),
 // and backpropagation to d2, d1. Stores the layer inputs (with a ones row for the biases) and the deltas as (rows, n)
-// arrays for the weight-gradient kernel: ws = [A0 (4, n) | A1 (65, n) | A2 (65, n) | D1 (64, n) | D2 (64, n) |
+// arrays for the weight-gradient kernel: ws = [A0 (kIn + 1, n) | A1 (65, n) | A2 (65, n) | D1 (64, n) | D2 (64, n) |
 // D3 (n_out, n)].
 __global__ __launch_bounds__(256) void mlp_fwd_bwd_kernel(const float* __restrict__ params,
                                                           const float* __restrict__ norm, const float* __restrict__ x,
@@ -132,16 +249,16 @@ __global__ __launch_bounds__(256) void mlp_fwd_bwd_kernel(const float* __restric
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     float* A0 = ws;
-    float* A1 = A0 + (size_t)4 * n;
+    float* A1 = A0 + (size_t)(kIn + 1) * n;
     float* A2 = A1 + (size_t)(kH + 1) * n;
     float* D1 = A2 + (size_t)(kH + 1) * n;
     float* D2 = D1 + (size_t)kH * n;
     float* D3 = D2 + (size_t)kH * n;
-    float xi[3];
-    normalise_input(norm, x[3 * s], x[3 * s + 1], x[3 * s + 2], xi);
+    float xi[kIn];
+    features(norm, x[kRaw * s], x[kRaw * s + 1], xi);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) A0[(size_t)c * n + s] = xi[c];
-    A0[(size_t)3 * n + s] = 1.0f;
+    for (int c = 0; c < kIn; ++c) A0[(size_t)c * n + s] = xi[c];
+    A0[(size_t)kIn * n + s] = 1.0f;
     float a1[kH], a2[kH], g2[kH];
     layer1(w, xi, a1);
     layer2(w, a1, a2);
@@ -154,7 +271,7 @@ __global__ __launch_bounds__(256) void mlp_fwd_bwd_kernel(const float* __restric
     A1[(size_t)kH * n + s] = 1.0f;
     A2[(size_t)kH * n + s] = 1.0f;
     const float scale = 2.0f / ((float)n * (float)kOut);
-    const float* mu = norm + 6;
+    const float* mu = norm + kNormHead;
     const float* sd = mu + kOut;
     for (int k = 0; k < kOut; ++k) {
         const float t = (y[(size_t)s * kOut + k] - mu[k]) / sd[k];
@@ -190,6 +307,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 // partial[chunk][parameter], summed over the chunks in order by mlp_adam_kernel. Tiles: layer 1 4 x 1, layer 2
 // 4 x 5, layer 3 3 x 5 (rows and columns past the layer read as zeros).
 constexpr int kTiles = 4 * 1 + 4 * 5 + 3 * 5;
+static_assert(kIn + 1 <= 16, "layer 1's weights and bias fit one column tile");
 
 __device__ __forceinline__ int param_index(int layer, int r, int c, int k_in) {
     const int w_off = layer == 0 ? kW1 : layer == 1 ? kW2 : kW3, b_off = layer == 0 ? kB1 : layer == 1 ? kB2 : kB3;
@@ -200,7 +318,7 @@ __global__ __launch_bounds__(64) void mlp_wgrad_kernel(const float* __restrict__
                                                        float* __restrict__ partial) {
     const int lane = threadIdx.x, t = blockIdx.x, chunk = blockIdx.y;
     const float* A0 = ws;
-    const float* A1 = A0 + (size_t)4 * n;
+    const float* A1 = A0 + (size_t)(kIn + 1) * n;
     const float* A2 = A1 + (size_t)(kH + 1) * n;
     const float* D1 = A2 + (size_t)(kH + 1) * n;
     const float* D2 = D1 + (size_t)kH * n;
@@ -211,7 +329,7 @@ __global__ __launch_bounds__(64) void mlp_wgrad_kernel(const float* __restrict__
     else layer = 2, rt = (t - 24) / 5, ct = (t - 24) % 5;
     const float* D = layer == 0 ? D1 : layer == 1 ? D2 : D3;
     const float* A = layer == 0 ? A0 : layer == 1 ? A1 : A2;
-    const int rows = layer == 2 ? kOut : kH, k_in = layer == 0 ? 3 : kH, cols = k_in + 1;
+    const int rows = layer == 2 ? kOut : kH, k_in = layer == 0 ? kIn : kH, cols = k_in + 1;
     const int m = lane & 15, kq = lane >> 4;
     const int row = rt * 16 + m, col = ct * 16 + m;
     const float* dr = D + (size_t)(row < rows ? row : 0) * n;
@@ -258,11 +376,12 @@ __global__ __launch_bounds__(256) void mlp_eval_kernel(const float* __restrict__
     stage_params(w, params);
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    float xi[3], a1[kH], a2[kH];
-    normalise_input(norm, x[3 * s], x[3 * s + 1], x[3 * s + 2], xi);
+    float xi[kIn], a1[kH], a2[kH];
+    features(norm, x[kRaw * s], x[kRaw * s + 1], xi);
     layer1(w, xi, a1);
     layer2(w, a1, a2);
-    for (int k = 0; k < kOut; ++k) y[(size_t)s * kOut + k] = norm[6 + k] + norm[6 + kOut + k] * layer3(w, a2, k);
+    for (int k = 0; k < kOut; ++k)
+        y[(size_t)s * kOut + k] = norm[kNormHead + k] + norm[kNormHead + kOut + k] * layer3(w, a2, k);
 }
 
 // ---- per-spin maps from the network ----
@@ -294,9 +413,48 @@ __device__ __forceinline__ void rotate_map(double (&A)[3][3], double (&c)[3], co
     }
 }
 
+// v <- the rotation of unit quaternion q applied to v.
+__device__ __forceinline__ void qrot(const Quat& q, double (&v)[3]) {
+    const double tx = 2.0 * (q.y * v[2] - q.z * v[1]), ty = 2.0 * (q.z * v[0] - q.x * v[2]),
+                 tz = 2.0 * (q.x * v[1] - q.y * v[0]);
+    v[0] += q.w * tx + (q.y * tz - q.z * ty);
+    v[1] += q.w * ty + (q.z * tx - q.x * tz);
+    v[2] += q.w * tz + (q.x * ty - q.y * tx);
+}
+
+// The split's correction of one block (see the top of the file): A += Rz(theta/2) Q~ (R2 - R1) (S(H) - T(G))
+// Rz(theta/2) A0, c += the same on c0 + Rz(theta/2) Q~ R1 PD (H_y, -H_x, -tr(G) / 2), with (A0, c0) the map before the
+// block and e = (H_x, H_y, G_xx, G_xy, G_yy) for this spin.
+__device__ __forceinline__ void split_correction(double (&A)[3][3], double (&c)[3], const double (&A0)[3][3],
+                                                 const double (&c0)[3], const Quat& h, const Quat& q,
+                                                 const double (&e)[5], double r1, double r2, double pd) {
+    const double k = r2 - r1, hx = e[0], hy = e[1], gxx = e[2], gxy = e[3], gyy = e[4];
+#pragma unroll
+    for (int col = 0; col < 4; ++col) {
+        double u[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) u[i] = col < 3 ? A0[i][col] : c0[i];
+        qrot(h, u);
+        double v[3] = {k * (hy * u[2] + gyy * u[0] - gxy * u[1]), k * (-hx * u[2] - gxy * u[0] + gxx * u[1]),
+                       k * (hy * u[0] - hx * u[1] - (gxx + gyy) * u[2])};
+        if (col == 3) v[0] += r1 * pd * hy, v[1] -= r1 * pd * hx, v[2] -= 0.5 * r1 * pd * (gxx + gyy);
+        qrot(q, v);
+        qrot(h, v);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (col < 3)
+                A[i][col] += v[i];
+            else
+                c[i] += v[i];
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void surrogate_maps_kernel(hb_case cs, const float* __restrict__ params,
-                                                             const float* __restrict__ norm, hb_sur_pulse sp,
-                                                             float* __restrict__ maps, int* __restrict__ outside) {
+                                                             const float* __restrict__ norm,
+                                                             const float* __restrict__ first, int n_grid,
+                                                             hb_sur_pulse sp, float* __restrict__ maps,
+                                                             int* __restrict__ outside) {
     __shared__ __align__(16) float w[kParams];
     stage_params(w, params);
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -305,9 +463,9 @@ __global__ __launch_bounds__(256) void surrogate_maps_kernel(hb_case cs, const f
     const double u = ((double)p.px + (double)p.lx) * sp.g[0] + ((double)p.py + (double)p.ly) * sp.g[1] +
                      ((double)p.pz + (double)p.lz) * sp.g[2];
     const double f = (double)p.db0 + u;
-    float xi[3];
-    normalise_input(norm, p.db0, (float)u, p.b1, xi);
-    const bool out = fabsf(xi[0]) > 1.0f || fabsf(xi[1]) > 1.0f || fabsf(xi[2]) > 1.0f;
+    float xi[kIn];
+    features(norm, (float)f, p.b1, xi);
+    const bool out = fabsf(xi[0]) > 1.0f || fabsf(xi[1]) > 1.0f;
     outside[j] = (j < cs.n && out) ? 1 : 0;
     float a1[kH], a2[kH];
     layer1(w, xi, a1);
@@ -316,18 +474,36 @@ __global__ __launch_bounds__(256) void surrogate_maps_kernel(hb_case cs, const f
     const double pd = p.pd, r1 = p.r1, r2 = p.r2;
     for (int b = 0; b < kM; ++b) {
         Quat q;
-        q.w = norm[6 + 4 * b] + norm[6 + kOut + 4 * b] * layer3(w, a2, 4 * b);
-        q.x = norm[7 + 4 * b] + norm[6 + kOut + 4 * b + 1] * layer3(w, a2, 4 * b + 1);
-        q.y = norm[8 + 4 * b] + norm[6 + kOut + 4 * b + 2] * layer3(w, a2, 4 * b + 2);
-        q.z = norm[9 + 4 * b] + norm[6 + kOut + 4 * b + 3] * layer3(w, a2, 4 * b + 3);
+        const float* mu = norm + kNormHead + 4 * b;
+        const float* sd = mu + kOut;
+        float q1[4];
+        first_quat(first, n_grid, norm, (float)f, p.b1 * norm[5], b, q1);
+        q.w = q1[0] + (mu[0] + sd[0] * layer3(w, a2, 4 * b));
+        q.x = q1[1] + (mu[1] + sd[1] * layer3(w, a2, 4 * b + 1));
+        q.y = q1[2] + (mu[2] + sd[2] * layer3(w, a2, 4 * b + 2));
+        q.z = q1[3] + (mu[3] + sd[3] * layer3(w, a2, 4 * b + 3));
         const double inv = 1.0 / sqrt(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
         q.w *= inv, q.x *= inv, q.y *= inv, q.z *= inv;
         const double tau = sp.blocks[2 * b], area = sp.blocks[2 * b + 1];
         const Quat h = rot_z(-M_PI * (f * tau + area));  // Rz(theta / 2), theta = -2 pi (f tau + B)
         const double e1 = exp(-0.5 * tau * r1), e2 = exp(-0.5 * tau * r2);
+        double A0[3][3], c0[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) A0[i][k] = A[i][k];
+            c0[i] = c[i];
+        }
         relax_map(A, c, e1, e2, pd);
         rotate_map(A, c, qmul(h, qmul(q, h)));
         relax_map(A, c, e1, e2, pd);
+        if (first) {
+            float v[7];
+            first_lookup(first, n_grid, norm, (float)f, b, v);
+            const double bh = (double)p.b1 * norm[5];
+            const double e[5] = {bh * v[2], bh * v[3], bh * bh * v[4], bh * bh * v[5], bh * bh * v[6]};
+            split_correction(A, c, A0, c0, h, q, e, r1, r2, pd);
+        }
     }
     float* o = maps + (size_t)sp.pid * 12 * np + j;
 #pragma unroll
@@ -344,7 +520,7 @@ int launch_train_grad(const hb_mlp* net, const float* x, const float* y, int32_t
     // one wave per workgroup: a batch of 8,192 samples is 128 workgroups (the kernel runs at one wave per SIMD)
     hipLaunchKernelGGL(mlp_fwd_bwd_kernel, dim3((n + 63) / 64), dim3(64), 0, st, net->params, net->norm, x, y, n,
                        work);
-    float* partial = work + (size_t)(4 + 2 * (kH + 1) + 2 * kH + kOut) * n;
+    float* partial = work + (size_t)(kIn + 1 + 2 * (kH + 1) + 2 * kH + kOut) * n;
     hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(kTiles, chunks), dim3(64), 0, st, work, n, partial);
     return hipGetLastError();
 }
@@ -355,13 +531,14 @@ using namespace hb;
 
 static int check_net(const hb_mlp* net) {
     if (!net || !net->params || !net->norm || net->hidden != kH || net->n_out != kOut) return hipErrorInvalidValue;
+    if (net->first && net->n_grid < 2) return hipErrorInvalidValue;
     return hipSuccess;
 }
 
 extern "C" int64_t hb_mlp_workspace(int32_t n) {
     if (n < 1) return 0;
     const int64_t chunks = (n + kChunk - 1) / kChunk;
-    return (int64_t)(4 + 2 * (kH + 1) + 2 * kH + kOut) * n + chunks * kParams;
+    return (int64_t)(kIn + 1 + 2 * (kH + 1) + 2 * kH + kOut) * n + chunks * kParams;
 }
 
 extern "C" int hb_spinor_blocks(const double* table, const int32_t* bounds, const float* x, int32_t n, float* y,
@@ -411,7 +588,7 @@ extern "C" int hb_mlp_train_steps(const hb_mlp* net, const float* x, const float
         const int t = step0 + i + 1;
         const double lr = lr1 + 0.5 * (lr0 - lr1) * (1.0 + cos(M_PI * (t - 1) / (double)total_steps));
         const float c1 = (float)(1.0 / (1.0 - pow(0.9, t))), c2 = (float)(1.0 / (1.0 - pow(0.999, t)));
-        if (int e = launch_train_grad(net, x + (size_t)i * batch * 3, y + (size_t)i * batch * kOut, batch, work,
+        if (int e = launch_train_grad(net, x + (size_t)i * batch * kRaw, y + (size_t)i * batch * kOut, batch, work,
                                       stream))
             return e;
         hipLaunchKernelGGL(mlp_adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, (hipStream_t)stream, partial,
@@ -436,6 +613,24 @@ extern "C" int hb_surrogate_maps(const hb_case* cs, const hb_mlp* net, const hb_
         sp->pid >= cs->n_pulses)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(surrogate_maps_kernel, dim3(cs->n_pad / 256), dim3(256), 0, (hipStream_t)stream, *cs,
-                       net->params, net->norm, *sp, maps, outside);
+                       net->params, net->norm, net->first, net->n_grid, *sp, maps, outside);
+    return hipGetLastError();
+}
+
+extern "C" int hb_sur_first_order(const double* table, const int32_t* bounds, double f_lo, double df, int32_t n_grid,
+                                  float* first, void* stream) {
+    if (!table || !bounds || !first || n_grid < 2 || !(df > 0.0)) return hipErrorInvalidValue;
+    const int total = kM * n_grid;
+    hipLaunchKernelGGL(first_order_kernel, dim3((total + 255) / 256), dim3(256), 0, (hipStream_t)stream, table, bounds,
+                       f_lo, df, n_grid, first);
+    return hipGetLastError();
+}
+
+extern "C" int hb_sur_first_quats(const hb_mlp* net, const float* x, int32_t n, float* q, void* stream) {
+    if (int e = check_net(net)) return e;
+    if (!x || !q || n < 0) return hipErrorInvalidValue;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(first_quats_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, net->first,
+                       net->n_grid, net->norm, x, n, q);
     return hipGetLastError();
 }

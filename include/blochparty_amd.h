@@ -182,25 +182,47 @@ int hb_pulse_map_dlnb1(const hb_case* cs, const double* table, const int32_t* in
                        float* maps_d, void* stream);
 
 /* ---- The learned RF surrogate (no counterpart in the reference: it stands where the reference composes a reused
- * pulse per spin, cuda/pulse.cu.in:3-32). A pulse is cut into HB_SUR_BLOCKS blocks; a network 3 -> 64 -> 64 -> 4 M
- * (tanh) maps a spin's (df [Hz], u = g . r [Hz], b1) to the blocks' rotations as unit quaternions (w, x, y, z) in the
- * frame turning with the off-resonance precession; the spin's map is composed from them in float64 with the
- * relaxation put back half a block before and after each rotation. Float32 parameters, float32 arithmetic. */
+ * pulse per spin, cuda/pulse.cu.in:3-32). A pulse is cut into HB_SUR_BLOCKS blocks; a network
+ * HB_MLP_INPUTS -> 64 -> 64 -> 4 M (tanh) maps a spin's offset during the pulse f = df + g . r [Hz] and its B1 scale
+ * times the pulse's peak b [Hz] to the blocks' rotations as unit quaternions (w, x, y, z) in the frame turning with
+ * the off-resonance precession; the spin's map is composed from them in float64 with the relaxation put back half a
+ * block before and after each rotation. The network's inputs are features of (f, b): (f - c_f) / h_f,
+ * (b - c_b) / h_b, and sin, cos of 2 pi f q S for q = 1 .. HB_MLP_FOURIER (S = the mean block length over
+ * 2 HB_MLP_FOURIER: frequencies inside a block's half-width, the terms a block's rotation is made of to first order
+ * in the RF). The network learns each block's quaternion less its first-order part: the small-tip rotation
+ * sum_k Rz(-phi_k) w_k (w_k = -2 pi dt_k b rf_k the substeps' transverse rotation vectors, phi_k the precession
+ * angle from the block's middle to substep k's), linear in b and, per unit b, a function of f alone, tabulated per
+ * network over its box's f range (HB_MLP_FIRST_GRID points) and interpolated linearly. The relaxation, put back
+ * half a block before and after each rotation, gets that split's correction (first order in the rates, second in the
+ * RF) from the blocks' time moments H_j = sum_k s_k Rz(-phi_k) w_k (s_k the substep's time from the block's middle)
+ * and G_j = int v v^T ds - (tau/2) v(tau) v(tau)^T (v(s) the rotation vector accumulated up to s), tabulated beside
+ * F_j.
+ * Float32 parameters, float32 arithmetic. */
 #define HB_MLP_HIDDEN 64
 #define HB_SUR_BLOCKS 10
-#define HB_MLP_PARAMS (HB_MLP_HIDDEN * 3 + HB_MLP_HIDDEN + HB_MLP_HIDDEN * HB_MLP_HIDDEN + HB_MLP_HIDDEN + \
+#define HB_MLP_FOURIER 4
+#define HB_MLP_INPUTS (2 + 2 * HB_MLP_FOURIER)
+#define HB_MLP_PARAMS (HB_MLP_HIDDEN * HB_MLP_INPUTS + HB_MLP_HIDDEN + HB_MLP_HIDDEN * HB_MLP_HIDDEN + HB_MLP_HIDDEN + \
                        4 * HB_SUR_BLOCKS * HB_MLP_HIDDEN + 4 * HB_SUR_BLOCKS)
+#define HB_MLP_NORM_HEAD 6
+#define HB_MLP_FIRST_GRID 2049
 
-/* A network. params: (HB_MLP_PARAMS,) W1 (64, 3), b1 (64), W2 (64, 64), b2 (64), W3 (n_out, 64), b3 (n_out);
- * adam_m, adam_v: Adam's moments (training only); norm: (6 + 2 n_out,) the input box's centre (3) and half-widths
- * (3), the outputs' means (n_out) and scales (n_out): the network sees (x - centre) / half-width and is trained on
- * (y - mean) / scale. hidden must be HB_MLP_HIDDEN and n_out 4 HB_SUR_BLOCKS. */
+/* A network. params: (HB_MLP_PARAMS,) W1 (64, HB_MLP_INPUTS), b1 (64), W2 (64, 64), b2 (64), W3 (n_out, 64),
+ * b3 (n_out); adam_m, adam_v: Adam's moments (training only); norm: (HB_MLP_NORM_HEAD + 2 n_out,) c_f, h_f, c_b,
+ * h_b (the input box's centres and half-widths), S [s] (the features' base time), s_b (an input b times s_b is b in
+ * Hz: 1 for the training inputs, the pulse's peak for a spin's B1 scale), then the outputs' means (n_out) and scales
+ * (n_out): the network is trained on (y - mean) / scale; first: (HB_SUR_BLOCKS, n_grid, 8) the blocks' first-order
+ * rotation vectors F_j (x, y) and first time moments H_j (x, y) per Hz of b and second moments G_j (xx, xy, yy) per
+ * Hz^2 of b, then 0, at n_grid offsets spaced evenly over [c_f - h_f, c_f + h_f] (hb_sur_first_order), or NULL for a
+ * network without a first-order part (and no split correction). hidden must be HB_MLP_HIDDEN and n_out 4 HB_SUR_BLOCKS. */
 typedef struct hb_mlp {
     float* params;
     float* adam_m;
     float* adam_v;
     const float* norm;
     int32_t hidden, n_out;
+    const float* first;
+    int32_t n_grid, reserved; /* reserved: 0 */
 } hb_mlp;
 
 /* One reused pulse for hb_surrogate_maps: blocks: (HB_SUR_BLOCKS, 2) float64 each block's length tau [s] and
@@ -211,16 +233,26 @@ typedef struct hb_sur_pulse {
     int32_t pid, pad;
 } hb_sur_pulse;
 
-/* Training data: for n inputs x (n, 3) (df, u, b1), the exact interaction-frame block quaternions y (n, 4 M),
- * products of the substeps' rotations in float64. table as hb_pulse_maps' (one pulse, rows from 0); bounds:
- * (M + 1,) the blocks' first substeps and the end. */
+/* Training data: for n inputs x (n, 2) (f [Hz], b [Hz]: the RF columns of table are multiplied by b), the exact
+ * interaction-frame block quaternions y (n, 4 M), products of the substeps' rotations in float64. table as
+ * hb_pulse_maps' (one pulse, rows from 0); bounds: (M + 1,) the blocks' first substeps and the end. */
 int hb_spinor_blocks(const double* table, const int32_t* bounds, const float* x, int32_t n, float* y, void* stream);
+
+/* The first-order table of a pulse (table, bounds as hb_spinor_blocks'): first[(j * n_grid + i) * 8 + {0..7}]
+ * block j's F_j (x, y), H_j (x, y), G_j (xx, xy, yy) and 0 at offset f_lo + i df, sums over its substeps in
+ * float64. */
+int hb_sur_first_order(const double* table, const int32_t* bounds, double f_lo, double df, int32_t n_grid,
+                       float* first, void* stream);
+
+/* q (n, 4 M): the first-order block quaternions exp(b F_j(f) / 2) (w, x, y, 0) at inputs x (n, 2) (f, b in the
+ * network's units), F_j interpolated in net->first. */
+int hb_sur_first_quats(const hb_mlp* net, const float* x, int32_t n, float* q, void* stream);
 
 /* Floats of workspace hb_mlp_grad and hb_mlp_train_step need for a batch of n. */
 int64_t hb_mlp_workspace(int32_t n);
 
-/* grad (HB_MLP_PARAMS,): the gradient of the mean squared error between the network's outputs on (x - centre) /
- * half-width and (y - mean) / scale, over the n samples and n_out outputs; per-chunk sums added in a fixed order. */
+/* grad (HB_MLP_PARAMS,): the gradient of the mean squared error between the network's outputs on the features of
+ * x (n, 2) and (y - mean) / scale, over the n samples and n_out outputs; per-chunk sums added in a fixed order. */
 int hb_mlp_grad(const hb_mlp* net, const float* x, const float* y, int32_t n, float* work, float* grad, void* stream);
 
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected for step = 1, 2, ...) on that gradient. */
@@ -232,10 +264,12 @@ int hb_mlp_train_step(const hb_mlp* net, const float* x, const float* y, int32_t
 int hb_mlp_train_steps(const hb_mlp* net, const float* x, const float* y, int32_t batch, int32_t n_steps, float* work,
                        int32_t step0, int32_t total_steps, float lr0, float lr1, void* stream);
 
-/* y (n, n_out): the network's de-normalised outputs mean + scale * net((x - centre) / half-width). */
+/* y (n, n_out): the network's de-normalised outputs mean + scale * net(features of x (n, 2)) (without the
+ * first-order part). */
 int hb_mlp_eval(const hb_mlp* net, const float* x, int32_t n, float* y, void* stream);
 
-/* Maps of pulse sp->pid for every spin from the network (the maps format of hb_pulse_maps), and outside[spin] = 1
+/* Maps of pulse sp->pid for every spin from the network, its block quaternions the first-order part plus the
+ * network's outputs, normalised (the maps format of hb_pulse_maps), and outside[spin] = 1
  * for spins (< n) whose inputs lie outside the trained box (their maps are to be composed exactly, hb_pulse_maps
  * with their list), else 0. */
 int hb_surrogate_maps(const hb_case* cs, const hb_mlp* net, const hb_sur_pulse* sp, float* maps, int32_t* outside,

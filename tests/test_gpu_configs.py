@@ -237,15 +237,16 @@ def test_multislice_surrogate_against_exact_and_oracle(gpu, multislice_sample):
     rep = sur.surrogate
     print("multislice judge:", [(r["pulse"], f"{r['judge_max_map_err']:.1e}", f"{r['train_s']:.1f}s")
                                 for r in rep["pulses"]], flush=True)
-    assert rep["covered"] == 5 and not rep["rejected"]
-    assert max(r["judge_max_map_err"] for r in rep["pulses"]) < 1e-4
     exact = bp.simulate(ph, seq, coils, engine="gpu", pulses="reuse")
     step = bp.simulate(ph, seq, coils, engine="gpu")
     errs = {name: signal_err(o, ref) for name, o in (("surrogate", sur), ("reuse", exact), ("step", step))}
     errs["surrogate_vs_exact_maps"] = rel(sur.signal.to(torch.complex128), exact.signal.to(torch.complex128))
     print("multislice signal vs oracle:", {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    assert rep["covered"] == 5 and not rep["rejected"]
     assert errs["step"] < 1e-5 and errs["reuse"] < 1e-5
     assert errs["surrogate"] < 1e-4 and errs["surrogate_vs_exact_maps"] < 1e-4
+    # the largest map-entry difference on the judged spins, well inside the judge's tolerance
+    assert max(r["judge_max_map_err"] for r in rep["pulses"]) < 0.5 * rep["tolerance"]
 
 
 def test_multislice_fallback_for_spins_outside_the_box(gpu, multislice_sample):
@@ -254,7 +255,7 @@ def test_multislice_fallback_for_spins_outside_the_box(gpu, multislice_sample):
     case = driver.prepare_case(ph, seq, coils, surrogate=True)
     nets = case.extra["surrogate"]
     far = bp.Phantom(ph.pos, ph.pd, ph.t1, ph.t2, ph.db0.clone(), b1=ph.b1)
-    far.db0[::5] += 3000.0
+    far.db0[::5] += 20000.0  # beyond the +-6.7 kHz of offsets the slice-select gradient spans over the slices
     case_far = driver.prepare_case(far, seq, coils, reuse=True)
     case_far.extra["surrogate"] = nets  # the networks of the unshifted spins, not trained anew
     driver.run_forward(case_far)

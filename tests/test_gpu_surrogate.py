@@ -27,26 +27,57 @@ def slab():
 
 
 def box_inputs(box, n, seed):
+    """(n, 2) float32 inputs (f, b1 x peak) drawn uniformly from a box."""
     g = np.random.default_rng(seed)
     c, h = np.array(box[0]), np.array(box[1])
-    return ((g.uniform(-1, 1, (n, 3)) * h + c)).astype(np.float32)
+    return ((g.uniform(-1, 1, (n, 2)) * h + c)).astype(np.float32)
+
+
+def net_features(net, x):
+    """The oracle's feature map of float32 inputs x (n, 2) with the network's float32 normalisation head."""
+    head = net.norm[:5].double().cpu().numpy()
+    return S.features(x[:, 0].astype(np.float64), x[:, 1].astype(np.float64), head, sur.FOURIER)
 
 
 def test_spinor_blocks_match_the_oracle(gpu, slab):
     *_, shape = slab
-    x = box_inputs(((0.0, 0.0, 1.0), (150.0, 1200.0, 0.3)), 2000, 0)
+    x = box_inputs(((0.0, 1.0), (1350.0, 0.3)), 2000, 0)
     y = sur.spinor_blocks(shape, torch.from_numpy(x).cuda()).cpu().numpy()
     xd = x.astype(np.float64)
-    ref = S.block_quats(shape.table, shape.bounds, xd[:, 0] + xd[:, 1] + 0.0, xd[:, 2]).reshape(len(x), -1)
+    ref = S.block_quats(shape.table, shape.bounds, xd[:, 0], xd[:, 1]).reshape(len(x), -1)
     assert np.abs(y - ref).max() < 2e-7  # float64 products rounded to float32
 
 
+def test_first_order_table_and_quaternions_match_the_oracle(gpu, slab):
+    """hb_sur_first_order (F_j, H_j, G_j) at its grid points and hb_sur_first_quats between them (linear
+    interpolation on 2049 points) against the oracle's float64 first order and time moments."""
+    *_, shape = slab
+    box = ((-40.0, shape.scale), (1300.0, 0.3 * shape.scale))
+    first = sur.first_order(shape, box, "cuda")
+    lo, hi = np.float32(box[0][0]) - np.float32(box[1][0]), np.float32(box[0][0]) + np.float32(box[1][0])
+    grid = np.linspace(float(lo), float(hi), sur.FIRST_GRID)
+    mine = first.double().cpu().numpy().transpose(1, 0, 2)  # (G, M, 4)
+    ref = S.first_order(shape.table, shape.bounds, grid)  # (G, M, 2)
+    assert np.abs(mine[..., :2] - ref).max() < 1e-6 * np.abs(ref).max()
+    ref_h = S.first_moment(shape.table, shape.bounds, grid)
+    assert np.abs(mine[..., 2:4] - ref_h).max() < 1e-6 * np.abs(ref_h).max()
+    ref_g = S.second_moment(shape.table, shape.bounds, grid)
+    assert np.abs(mine[..., 4:7] - ref_g).max() < 1e-5 * np.abs(ref_g).max()
+    assert not mine[..., 7].any()
+    net = sur.new_net(box, shape.fourier_base, torch.zeros(sur.N_OUT), torch.ones(sur.N_OUT), 0, "cuda", first)
+    x = box_inputs(box, 3000, 11)
+    q = sur.first_quats(net, torch.from_numpy(x).cuda()).double().cpu().numpy().reshape(len(x), -1, 4)
+    xd = x.astype(np.float64)
+    qr = S.first_quats(S.first_order(shape.table, shape.bounds, xd[:, 0]), xd[:, 1])
+    assert np.abs(q - qr).max() < 2e-6
+
+
 def random_net(seed=1):
-    box = ((3.0, -20.0, 1.0), (120.0, 900.0, 0.25))
+    box = ((-17.0, 1.0), (1020.0, 0.25))
     rng = np.random.default_rng(seed)
     mu = torch.from_numpy(rng.normal(size=sur.N_OUT)).float()
     sd = torch.from_numpy(rng.uniform(0.01, 0.5, sur.N_OUT)).float()
-    net = sur.new_net(box, mu.cuda(), sd.cuda(), seed, "cuda")
+    net = sur.new_net(box, 2.5e-5, mu.cuda(), sd.cuda(), seed, "cuda")
     net.params.copy_(torch.from_numpy(rng.normal(size=sur.N_PARAMS) * 0.3).float())
     return net, box, mu.double().numpy(), sd.double().numpy()
 
@@ -57,7 +88,7 @@ def test_mlp_eval_and_gradient_match_the_oracle(gpu):
     x = box_inputs(box, n, 2)
     y = np.random.default_rng(3).normal(size=(n, sur.N_OUT)).astype(np.float32)
     p = net.params.double().cpu().numpy()
-    xn = (x.astype(np.float64) - np.array(box[0])) / np.array(box[1])
+    xn = net_features(net, x)
     out, _ = S.mlp_forward(p, xn, sur.HIDDEN, sur.N_OUT)
     mine = sur.evaluate(net, torch.from_numpy(x).cuda()).double().cpu().numpy()
     assert rel(torch.from_numpy(mine), torch.from_numpy(mu + sd * out)) < 1e-5
@@ -111,12 +142,12 @@ def test_trained_network_against_the_exact_maps(gpu, trained):
     assert net.final_loss < 1e-3
     x = box_inputs(box, 4000, 7)
     xd = x.astype(np.float64)
-    f = xd[:, 0] + xd[:, 1]
-    q = sur.evaluate(net, torch.from_numpy(x).cuda()).double().cpu().numpy().reshape(len(x), -1, 4)
+    f = xd[:, 0]
+    q = sur.block_quats(net, torch.from_numpy(x).cuda()).double().cpu().numpy().reshape(len(x), -1, 4)
     rng = np.random.default_rng(8)
     pd, r1, r2 = rng.uniform(0.7, 1.0, len(x)), 1 / rng.uniform(0.8, 4.0, len(x)), 1 / rng.uniform(0.08, 2, len(x))
     D, v = S.block_map(q, shape.blocks[:, 0], f, pd, r1, r2, shape.blocks[:, 1])
-    exact_q = S.block_quats(shape.table, shape.bounds, f, xd[:, 2])
+    exact_q = S.block_quats(shape.table, shape.bounds, f, xd[:, 1])
     De, ve = S.block_map(exact_q, shape.blocks[:, 0], f, pd, r1, r2, shape.blocks[:, 1])
     err = max(np.abs(D - De).max(), np.abs(v - ve).max())
     print(f"map error on {len(x)} box inputs: {err:.2e}")
@@ -217,13 +248,19 @@ def test_ablation_training_and_cost(gpu, trained, slab):
     case = driver.prepare_case(ph, seq, coils, surrogate=True)
     driver.run_forward(case)
     trained_err = driver.read_result(case).surrogate["pulses"][0]["judge_max_map_err"]
-    mu, sd = net.norm[6 : 6 + sur.N_OUT], net.norm[6 + sur.N_OUT :]
-    raw = sur.new_net(box, mu, sd, 0, "cuda")
+    mu, sd = net.norm[sur.NORM_HEAD : sur.NORM_HEAD + sur.N_OUT], net.norm[sur.NORM_HEAD + sur.N_OUT :]
+    raw = sur.new_net(box, shape.fourier_base, mu, sd, 0, "cuda", net.first)
     case.extra["surrogate"] = [(0, shape, raw)]
     driver.run_forward(case)
     raw_err = driver.read_result(case).surrogate["pulses"][0]["judge_max_map_err"]
-    print(f"judge map error: trained {trained_err:.2e}, untrained {raw_err:.2e}")
+    first_only = sur.new_net(box, shape.fourier_base, mu, sd, 0, "cuda", net.first)
+    first_only.params.zero_()  # outputs = the residuals' means: the first-order part alone
+    case.extra["surrogate"] = [(0, shape, first_only)]
+    driver.run_forward(case)
+    first_err = driver.read_result(case).surrogate["pulses"][0]["judge_max_map_err"]
+    print(f"judge map error: trained {trained_err:.2e}, first order alone {first_err:.2e}, untrained {raw_err:.2e}")
     assert raw_err > 100 * trained_err
+    assert first_err > 10 * trained_err
     # time: network maps vs exact substep composition over a 1 M-spin case
     big_ph, big_seq, big_c = W.gre3d(64)
     big = driver.prepare_case(big_ph, W.first_trs(big_seq, 2), big_c, reuse=True)
@@ -282,3 +319,63 @@ def test_a_flip_change_reuses_the_network(gpu, slab):
     nets = {id(n) for _, _, n in tcase.extra["surrogate"]}
     assert tcase.n_pulses == 5 and len(tcase.extra["surrogate"]) == 5
     assert len(nets) == 2  # the excitation, and one network for the four refocusing flips
+
+
+def test_pulse_flip_against_the_oracle(gpu, slab):
+    """The flip angle the training sampler follows: the network-free composition of the exact block quaternions in
+    the lab frame against the oracle's block map without relaxation (R = I + D, flip = arccos R_zz), and the slab
+    pulse's 10 degrees on resonance."""
+    *_, shape = slab
+    f = np.linspace(-1500.0, 1500.0, 61).astype(np.float32)
+    flip = sur.pulse_flip(shape, torch.from_numpy(f).cuda(), shape.scale)
+    fd = f.astype(np.float64)
+    b = np.full(len(f), np.float32(shape.scale), dtype=np.float64)
+    qs = S.block_quats(shape.table, shape.bounds, fd, b)
+    D, _ = S.block_map(qs, shape.blocks[:, 0], fd, np.ones(len(f)), np.zeros(len(f)), np.zeros(len(f)),
+                       shape.blocks[:, 1])
+    ref = np.arccos(np.clip(1.0 + D[:, 2, 2], -1.0, 1.0))
+    assert np.abs(flip - ref).max() < 1e-5
+    assert abs(flip[30] - np.deg2rad(10.0)) < 1e-4  # f = 0
+
+
+def test_surrogate_maps_compose_like_the_oracle(gpu):
+    """The maps kernel's composition against the oracle's block_map with the split correction: a network whose
+    outputs are zero stands for the first-order quaternions alone, on the multi-slice workload's edge-slice pulse
+    (offsets up to +-6.7 kHz, T2 down to 80 ms, B1 0.7-1.3)."""
+    import ctypes
+
+    from blochparty_amd.engines import hip
+
+    ph, seq, coils = W.multislice(16, lines=2)
+    case = driver.prepare_case(ph, seq, coils, reuse=True)
+    table = case.pulse_table.cpu().numpy()
+    off, k = case.pulse_index.cpu().numpy()[4]
+    shape = sur.shape_of(table[off : off + k])
+    box = sur.padded_box(sur.input_range(case.spins, case.pos_lo, case.n, shape.g, shape.scale))
+    net = sur.new_net(box, shape.fourier_base, torch.zeros(sur.N_OUT), torch.ones(sur.N_OUT), 0, "cuda",
+                      sur.first_order(shape, box, "cuda"))
+    net.params.zero_()
+    norm = sur.pulse_norm(net, shape)
+    maps = torch.zeros_like(case.maps)
+    outside = torch.zeros(case.n_pad, dtype=torch.int32, device="cuda")
+    blocks = torch.from_numpy(shape.blocks).cuda()
+    sp = sur.HbSurPulse(blocks.data_ptr(), (ctypes.c_double * 3)(*shape.g), 4, 0)
+    lib = hip.load()
+    assert lib.hb_surrogate_maps(ctypes.byref(case.struct()), ctypes.byref(net.struct(norm)), ctypes.byref(sp),
+                                 maps.data_ptr(), outside.data_ptr(), sur.stream()) == 0
+    n = case.n
+    mine = maps.view(case.n_pulses, 12, case.n_pad)[4, :, :n].double().cpu().numpy()
+    assert int(outside[:n].sum()) == 0
+    spins = case.spins[:, :n].double().cpu().numpy()
+    pos = spins[:3] + case.pos_lo[:, :n].double().cpu().numpy()
+    f = spins[6] + (shape.g[:, None] * pos).sum(0)
+    b = spins[7] * shape.scale
+    pd, r1, r2 = spins[3], spins[4], spins[5]
+    qs = S.first_quats(S.first_order(shape.table, shape.bounds, f), b)
+    mom = (S.first_moment(shape.table, shape.bounds, f) * b[:, None, None],
+           S.second_moment(shape.table, shape.bounds, f) * (b * b)[:, None, None])
+    D, v = S.block_map(qs, shape.blocks[:, 0], f, pd, r1, r2, shape.blocks[:, 1], mom)
+    ref = np.concatenate([D.reshape(n, 9), v], 1).T
+    err = np.abs(mine - ref).max()
+    print(f"maps kernel vs oracle block map (first order, split correction): {err:.2e}")
+    assert err < 3e-6

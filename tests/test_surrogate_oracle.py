@@ -130,7 +130,7 @@ def test_block_map_with_relaxation(spins):
 def test_mlp_gradient_by_finite_differences():
     rng = np.random.default_rng(2)
     h, n_out = 8, 5
-    p = rng.normal(size=S.mlp_size(h, n_out)) * 0.5
+    p = rng.normal(size=S.mlp_size(h, n_out, 3)) * 0.5
     x = rng.uniform(-1, 1, (40, 3))
     y = rng.normal(size=(40, n_out))
     g = S.mlp_grad(p, x, y, h, n_out)
@@ -180,11 +180,26 @@ def test_pulse_shape_key_and_norm_fold_the_amplitude():
     bent = a.table.copy()
     bent[5, 1] *= 1.01
     assert sur.shape_of(bent).key() != a.key()
-    net = sur.Net(torch.zeros(1), torch.zeros(1), torch.zeros(1), torch.tensor([1.0, 2.0, 30.0, 4.0, 5.0, 3.0]),
-                  ((1.0, 2.0, 30.0), (4.0, 5.0, 3.0)))
+    net = sur.Net(torch.zeros(1), torch.zeros(1), torch.zeros(1), torch.tensor([1.0, 4.0, 30.0, 3.0, 1e-5, 0.0]),
+                  ((1.0, 30.0), (4.0, 3.0)))
     norm = sur.pulse_norm(net, b)
     b1 = 0.93
-    assert abs((b1 - float(norm[2])) / float(norm[5]) - (b1 * b.scale - 30.0) / 3.0) < 1e-5  # float32 norm
+    assert abs((b1 - float(norm[2])) / float(norm[3]) - (b1 * b.scale - 30.0) / 3.0) < 1e-5  # float32 norm
+    assert float(norm[0]) == 1.0 and float(norm[1]) == 4.0 and float(norm[4]) == np.float32(1e-5)
+
+
+def test_features_pinned_by_corners_harmonics_and_quarter_turns():
+    head = (100.0, 400.0, 50.0, 5.0, 2e-5)
+    q_max = 4
+    x = S.features([100.0 - 400.0, 100.0 + 400.0], [45.0, 55.0], head, q_max)
+    np.testing.assert_allclose(x[:, :2], [[-1.0, -1.0], [1.0, 1.0]], rtol=0, atol=1e-15)
+    f = np.linspace(-3000.0, 3000.0, 7)
+    for q in range(1, q_max + 1):
+        a = S.features(f, np.full(7, 50.0), head, q_max)[:, 2 * q : 2 * q + 2]
+        one = S.features(q * f, np.full(7, 50.0), head, q_max)[:, 2:4]
+        np.testing.assert_allclose(a, one, rtol=0, atol=1e-12)  # harmonic q at f = harmonic 1 at q f
+        quarter = S.features([1.0 / (4 * q * head[4])], [50.0], head, q_max)[0, 2 * q : 2 * q + 2]
+        np.testing.assert_allclose(quarter, [1.0, 0.0], rtol=0, atol=1e-12)
 
 
 def test_mlp_forward_and_grad_against_torch_nn():
@@ -195,13 +210,13 @@ def test_mlp_forward_and_grad_against_torch_nn():
 
     h, n_out = 16, 10
     rng = np.random.default_rng(7)
-    p = rng.normal(size=S.mlp_size(h, n_out)) * 0.4
+    p = rng.normal(size=S.mlp_size(h, n_out, 3)) * 0.4
     x = rng.uniform(-1, 1, (50, 3))
     y = rng.normal(size=(50, n_out))
     net = torch.nn.Sequential(torch.nn.Linear(3, h), torch.nn.Tanh(), torch.nn.Linear(h, h), torch.nn.Tanh(),
                               torch.nn.Linear(h, n_out)).double()
     with torch.no_grad():
-        for t, v in zip(net.parameters(), S.mlp_unpack(p, h, n_out)):
+        for t, v in zip(net.parameters(), S.mlp_unpack(p, h, n_out, 3)):
             t.copy_(torch.from_numpy(v))
     out, _ = S.mlp_forward(p, x, h, n_out)
     ref = net(torch.from_numpy(x))
@@ -211,3 +226,83 @@ def test_mlp_forward_and_grad_against_torch_nn():
     auto = np.concatenate([t.grad.numpy().ravel() for t in net.parameters()])
     np.testing.assert_allclose(S.mlp_grad(p, x, y, h, n_out), auto, rtol=0, atol=1e-12)
     assert abs(S.mlp_loss(p, x, y, h, n_out) - float(loss)) < 1e-12
+
+
+def test_first_order_pinned_by_a_hard_pulse_and_the_residual_order():
+    """A hard pulse on resonance: the first order is exact. A shaped pulse off resonance: block_quats - first_quats
+    falls by more than 3.5 per halving of b (the first order is right), while the planted mistake (the precession
+    phase's sign flipped) leaves a first-order residual (about 2 per halving)."""
+    k, m = 40, 4
+    hard = np.zeros((k, 8))
+    hard[:, 0], hard[:, 1], hard[:, 2] = 5e-6, 0.6, -0.3
+    bounds = S.block_bounds(k, m)
+    f0 = np.zeros(3)
+    b = np.array([100.0, 250.0, 400.0])
+    exact = S.block_quats(hard, bounds, f0, b)
+    mine = S.first_quats(S.first_order(hard, bounds, f0), b)
+    assert np.abs(exact - mine).max() < 1e-13
+    from blochparty_amd import workloads as W
+    from blochparty_amd.engines import pulses, surrogate as sur
+
+    seq = W.first_trs(W.gre3d_sequence(4), 2)
+    table, index = pulses.pulse_table(seq, pulses.reused_pulses(seq))
+    off, kk = index[0]
+    shape = sur.shape_of(table[off : off + kk])
+    f = np.array([-900.0, -250.0, 130.0, 700.0])
+    bounds = S.block_bounds(len(shape.table), 10)
+
+    def residual(bb, sign=1.0):
+        q = S.block_quats(shape.table, bounds, f, np.full(len(f), bb))
+        q1 = S.first_quats(S.first_order(shape.table, bounds, f, sign), np.full(len(f), bb))
+        return np.abs(q - q1).max()
+
+    peak = shape.scale
+    assert residual(peak) / residual(peak / 2) > 3.5
+    bad = residual(peak, -1.0) / residual(peak / 2, -1.0)
+    assert 1.6 < bad < 2.4  # the planted mistake is caught
+
+
+def test_block_map_split_correction_pinned_against_the_substeps():
+    """A slice-selective pulse at offsets up to 6.6 kHz (blocks that precess many turns) with T2 = 80 ms: the corrected
+    split is at least 10x closer to the substep-exact map than the plain one and at least second order in the block
+    length, and the planted mistake (the correction's sign flipped) is further off than the plain split. The
+    symmetric slab pulse near its band: within 1e-6 of the substeps."""
+    from blochparty_amd import workloads as W
+    from blochparty_amd.engines import pulses, surrogate as sur
+
+    seq = W.multislice_sequence(16, lines=1, slices=5)
+    table, index = pulses.pulse_table(seq, pulses.reused_pulses(W.concat([seq, seq])))
+    off, k = index[4]
+    shape = sur.shape_of(table[off : off + k])
+    f = np.linspace(-6600.0, 6600.0, 133)
+    n = len(f)
+    b = np.full(n, shape.scale)
+    pd, r1, r2 = np.ones(n), np.full(n, 1 / 0.8), np.full(n, 1 / 0.08)
+
+    def split(m, moments=True, sign=1.0):
+        bounds = S.block_bounds(k, m)
+        q = S.block_quats(shape.table, bounds, f, b)
+        taus = np.add.reduceat(shape.table[:, 0], bounds[:-1])
+        areas = np.add.reduceat(shape.table[:, 0] * shape.table[:, 6], bounds[:-1])
+        mom = None
+        if moments:
+            mom = (S.first_moment(shape.table, bounds, f) * b[:, None, None],
+                   S.second_moment(shape.table, bounds, f) * (b * b)[:, None, None])
+        return S.block_map(q, taus, f, pd, r1, r2, areas, mom, sign)
+
+    exact = split(k, moments=False)  # one substep per block: the substeps themselves
+    err = lambda out: max(np.abs(out[0] - exact[0]).max(), np.abs(out[1] - exact[1]).max())
+    plain, fixed = err(split(10, moments=False)), err(split(10))
+    assert fixed < plain / 10
+    assert fixed / err(split(20)) > 3.0
+    assert err(split(10, sign=-1.0)) > plain  # the planted mistake is caught
+    slab_seq = W.first_trs(W.gre3d_sequence(8), 2)
+    table, index = pulses.pulse_table(slab_seq, pulses.reused_pulses(slab_seq))
+    off, k = index[0]
+    shape = sur.shape_of(table[off : off + k])
+    f = np.linspace(-1300.0, 1300.0, 53)
+    n = len(f)
+    b = np.full(n, 1.3 * shape.scale)
+    pd, r1, r2 = np.ones(n), np.full(n, 1 / 0.5), np.full(n, 1 / 0.05)
+    exact = split(k, moments=False)
+    assert err(split(10)) < 1e-6
