@@ -9,7 +9,10 @@
 //      unspoil, add the ADC sample's source term sum_c h_c conj(S_c) and the coil gradients h_c conj(Mxy), reverse
 //      the step (adjoint_device.h).
 // Per-spin gradients are summed in registers and added to float64 accumulators at every segment index that is a
-// multiple of 64 (when something was added since the last time); per-segment gradients are summed over each wave (DPP
+// multiple of 64 (when something was added since the last time); the accumulators (8 per-spin gradients and the 2 NC
+// coil gradients) live in LDS for the whole launch, each lane its own slots, read from HBM at the start and written
+// back at the end (one read and one write per interval instead of one of each per flush: the same float64 additions
+// in the same order); per-segment gradients are summed over each wave (DPP
 // + one MFMA per segment), then over the workgroup's 4 waves in LDS in a fixed order, one partial per workgroup and
 // segment, and over workgroups in a fixed order by seg_reduce. Nothing depends on the interval length (a multiple of
 // 64) or on scheduling: the gradients are the same bit for bit for any checkpoint_every and every run.
@@ -250,10 +253,18 @@ template <int NC, bool SEG, bool NU>
 __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8 ? 2 : 1))) void adjoint_kernel(
     hb_case cs, hb_adjoint ad, int s0, int s1) {
     __shared__ float segbuf[2][kWavesA][8][8];  // alternate buffers: one barrier per workgroup sum
+    __shared__ double accs[(8 + 2 * NC) * kBlockA];  // float64 accumulators: slot a of lane tid at a * kBlockA + tid
     int sbuf = 0;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = blockIdx.x * kBlockA + tid;
     const int np = cs.n_pad;
+    const bool coil_grads = NC > 0 && ad.coil_grads;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) accs[a * kBlockA + tid] = ad.gs[(size_t)a * np + j];
+    if (coil_grads) {
+#pragma unroll
+        for (int c = 0; c < 2 * NC; ++c) accs[(8 + c) * kBlockA + tid] = ad.gc[(size_t)c * np + j];
+    }
     const Spin p = load_spin(cs.spins, cs.pos_lo, np, j);
     const float b1sq = p.b1 * p.b1;
     const bool rf_only = SEG && ad.seg_grads == 2;
@@ -463,14 +474,14 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
             dirty = false;
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
-                ad.gs[(size_t)a * np + j] += (double)acc.v[a];
+                accs[a * kBlockA + tid] += (double)acc.v[a];
                 acc.v[a] = 0.0f;
             }
-            if (NC > 0 && ad.coil_grads) {
+            if (coil_grads) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
-                    ad.gc[(size_t)(2 * c) * np + j] += (double)gr[c];
-                    ad.gc[(size_t)(2 * c + 1) * np + j] += (double)gi[c];
+                    accs[(8 + 2 * c) * kBlockA + tid] += (double)gr[c];
+                    accs[(9 + 2 * c) * kBlockA + tid] += (double)gi[c];
                     gr[c] = gi[c] = 0.0f;
                 }
             }
@@ -478,12 +489,12 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
     }
     // an interval that does not start at a multiple of 64 leaves a remainder (only possible for the first one)
 #pragma unroll
-    for (int a = 0; a < 8; ++a) ad.gs[(size_t)a * np + j] += (double)acc.v[a];
-    if (NC > 0 && ad.coil_grads) {
+    for (int a = 0; a < 8; ++a) ad.gs[(size_t)a * np + j] = accs[a * kBlockA + tid] + (double)acc.v[a];
+    if (coil_grads) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-            ad.gc[(size_t)(2 * c) * np + j] += (double)gr[c];
-            ad.gc[(size_t)(2 * c + 1) * np + j] += (double)gi[c];
+            ad.gc[(size_t)(2 * c) * np + j] = accs[(8 + 2 * c) * kBlockA + tid] + (double)gr[c];
+            ad.gc[(size_t)(2 * c + 1) * np + j] = accs[(9 + 2 * c) * kBlockA + tid] + (double)gi[c];
         }
     }
     ad.lam[j] = lx, ad.lam[np + j] = ly, ad.lam[2 * np + j] = lz;
