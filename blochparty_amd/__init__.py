@@ -19,6 +19,7 @@ from .builder import (
 )
 from .coils import birdcage_sensitivities
 from .exchange import BoundPool
+from .fields import GradientNonlinearity, Scanner
 from .engines import available, simulate
 from .library import gre_2d, gre_3d, gre_cartesian, tse_2d
 from .phantom import (
@@ -39,6 +40,8 @@ from .sequence import Sequence
 __all__ = [
     "ADC",
     "BoundPool",
+    "GradientNonlinearity",
+    "Scanner",
     "LABELS",
     "MAX_RF_STEP",
     "TISSUES",

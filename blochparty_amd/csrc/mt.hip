@@ -1,9 +1,14 @@
-// The bound-pool (magnetization transfer, rate model) forward kernel. Replaces the reference's V = 1 variant of
-// bloch_chunk_mma for a rate bound pool (src/blochparty/engines/gpu/shaders/common.metal:113-208: lineshape, the
-// exchange step, step_mt).
+// The full-variant forward kernel: a rate bound pool (magnetization transfer) and / or a scanner's concomitant fields.
+// Replaces the reference's V = 1 variant of bloch_chunk_mma (src/blochparty/engines/gpu/shaders/common.metal:113-208:
+// lineshape, the exchange step, step_mt; :563-601 the field model; engines/core.py:115-121, 190-217).
 //
 // One lane per isochromat, workgroups of 256 (4 waves of 64), every lane walking the same plain segment records. Per
-// spin the free pool's (Mx, My, Mz) and the bound pool's Mzb; Mz and Mzb carry two-sum low parts.
+// spin the free pool's (Mx, My, Mz) and, with a bound pool, its Mzb; Mz and Mzb carry two-sum low parts. Without a
+// bound pool the longitudinal step is plain T1 recovery.
+// - Concomitant field f_c [Hz] of a segment with gradient g: leading order c . (gx^2, gy^2, gz^2, gx gy, gx gz,
+//   gy gz), or exact |B| - f0 - lin = t^2 / (sqrt(u^2 + t^2) + u) (t^2 = (ax . g)^2 + (ay . g)^2, u = f0 + lin, lin the
+//   rest of the offset); it adds to an RF segment's offset (and so to the bound pool's saturation offset) and, times
+//   the segment's length, to a free segment's float64 phase.
 // - RF segment: Strang split. Half a segment of transverse decay and of longitudinal exchange, with the bound pool
 //   saturated at W = (2 pi b1 |rf|)^2 G(f_rf - bz - dw); the water's rotation (only the precession about z for RF
 //   that only the bound pool feels); the other half.
@@ -84,6 +89,18 @@ __device__ __forceinline__ void rotate_state(State& m, float wx, float wy, float
     two_sum(m.mz, m.lz, a * cz + b * ez);
 }
 
+// The spin's concomitant field in segment q, lin the rest of its offset there [Hz]; cc the spin's coefficients.
+__device__ __forceinline__ float conc_field(int mode, const float (&cc)[7], const hb_segment& q, float lin) {
+    const float gx = q.g_x, gy = q.g_y, gz = q.g_z;
+    if (mode == HB_CONC_LEADING)
+        return cc[0] * gx * gx + cc[1] * gy * gy + cc[2] * gz * gz + cc[3] * gx * gy + cc[4] * gx * gz +
+               cc[5] * gy * gz;
+    const float tx = cc[0] * gx + cc[1] * gy + cc[2] * gz, ty = cc[3] * gx + cc[4] * gy + cc[5] * gz;
+    const float t2 = tx * tx + ty * ty, u = cc[6] + lin;
+    const float den = sqrtf(u * u + t2) + u;  // > 0 for any real field (u = f0 + lin > 0)
+    return den > 0.0f ? t2 / den : 0.0f;
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -98,38 +115,67 @@ __global__ __launch_bounds__(kBlockM) void forward_mt_kernel(hb_case cs, hb_mt m
     const int np = cs.n_pad;
     const Spin p = load_spin(cs.spins, cs.pos_lo, np, j);
     const Pos64 r64 = pos64(p);
-    const float f = mt.pool[j], k = mt.pool[np + j], r1b = mt.pool[2 * np + j], t2b = mt.pool[3 * np + j];
-    const float dw = mt.pool[4 * np + j];
+    const bool pool = mt.pool != nullptr;  // uniform
+    const float f = pool ? mt.pool[j] : 0.0f, k = pool ? mt.pool[np + j] : 0.0f;
+    const float r1b = pool ? mt.pool[2 * np + j] : 1.0f, t2b = pool ? mt.pool[3 * np + j] : 1.0f;
+    const float dw = pool ? mt.pool[4 * np + j] : 0.0f;
     const float b1sq = p.b1 * p.b1;
     State m = load_state(cs.state, np, j);
-    float mb = mt.mb[j], lb = mt.mb[np + j];
+    float mb = pool ? mt.mb[j] : 0.0f, lb = pool ? mt.mb[np + j] : 0.0f;
+    const int cmode = mt.conc_mode;
+    float cc[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (cmode != HB_CONC_NONE) {
+#pragma unroll
+        for (int i = 0; i < 7; ++i) cc[i] = mt.conc[(size_t)i * np + j];
+    }
     const int C = cs.n_coils, vals = 16 * cs.n_tiles;
     for (int s = s0; s < s1; ++s) {
         const hb_segment q = cs.segs[s];
         const int fl = q.flags;
         if (fl & HB_RF) {
-            const float bz = offset(p, q);
-            const float w1sq = HB_TWO_PI * HB_TWO_PI * b1sq * (q.rf_x * q.rf_x + q.rf_y * q.rf_y);
-            const float sat = w1sq * lineshape_g(mt, mt.rf_freq[s] - bz - dw, t2b);
-            const XStep x = xstep(0.5f * q.dt, p.pd, p.r1, f, k, r1b, sat);
+            float bz = offset(p, q);
+            const float fc = cmode != HB_CONC_NONE ? conc_field(cmode, cc, q, bz) : 0.0f;
+            bz += fc;
             const float e2 = expm1x(-0.5f * q.dt * p.r2);
+            XStep x;
+            float e1 = 0.0f;
+            if (pool) {
+                const float w1sq = HB_TWO_PI * HB_TWO_PI * b1sq * (q.rf_x * q.rf_x + q.rf_y * q.rf_y);
+                const float sat = w1sq * lineshape_g(mt, mt.rf_freq[s] - bz - dw, t2b);
+                x = xstep(0.5f * q.dt, p.pd, p.r1, f, k, r1b, sat);
+            } else {
+                e1 = expm1x(-0.5f * q.dt * p.r1);
+            }
             m.mx += m.mx * e2, m.my += m.my * e2;
-            exchange(x, m, mb, lb);
-            const float wz = p.px * q.w_g_x + p.py * q.w_g_y + p.pz * q.w_g_z + (p.db0 * q.w + q.w_b0);
+            if (pool)
+                exchange(x, m, mb, lb);
+            else
+                two_sum(m.mz, m.lz, (m.mz - p.pd) * e1);
+            const float wz =
+                p.px * q.w_g_x + p.py * q.w_g_y + p.pz * q.w_g_z + (p.db0 * q.w + q.w_b0) + q.w * fc;
             if (fl & HB_BOUND_ONLY)
                 rotate_state(m, 0.0f, 0.0f, wz);
             else
                 rotate_state(m, p.b1 * q.w_rf_x, p.b1 * q.w_rf_y, wz);
             m.mx += m.mx * e2, m.my += m.my * e2;
-            exchange(x, m, mb, lb);
+            if (pool)
+                exchange(x, m, mb, lb);
+            else
+                two_sum(m.mz, m.lz, (m.mz - p.pd) * e1);
         } else {
+            double cyc = free_cycles64(r64, p, q);
+            if (cmode != HB_CONC_NONE) cyc += (double)(conc_field(cmode, cc, q, offset(p, q)) * q.dt);
+            cyc -= rint(cyc);
             float sn, cs_;
-            sincospi(-2.0f * free_cycles(r64, p, q), &sn, &cs_);
+            sincospi(-2.0f * (float)cyc, &sn, &cs_);
             const float e2 = expm1x(-q.dt * p.r2);
             const float nx = cs_ * m.mx - sn * m.my, ny = sn * m.mx + cs_ * m.my;
             m.mx = nx + nx * e2;
             m.my = ny + ny * e2;
-            exchange(xstep(q.dt, p.pd, p.r1, f, k, r1b, 0.0f), m, mb, lb);
+            if (pool)
+                exchange(xstep(q.dt, p.pd, p.r1, f, k, r1b, 0.0f), m, mb, lb);
+            else
+                two_sum(m.mz, m.lz, (m.mz - p.pd) * expm1x(-q.dt * p.r1));
         }
         if (fl & HB_ADC) {  // the workgroup's coil sums of this sample (uniform branch: every lane takes it)
             const int r = q.pad[0] - row0;
@@ -153,7 +199,7 @@ __global__ __launch_bounds__(kBlockM) void forward_mt_kernel(hb_case cs, hb_mt m
         if (fl & HB_SPOIL) m.mx = m.my = 0.0f;
     }
     store_state(cs.state, np, j, m);
-    mt.mb[j] = mb, mt.mb[np + j] = lb;
+    if (pool) mt.mb[j] = mb, mt.mb[np + j] = lb;
 }
 
 }  // namespace
@@ -162,14 +208,18 @@ __global__ __launch_bounds__(kBlockM) void forward_mt_kernel(hb_case cs, hb_mt m
 extern "C" int hb_forward_mt(const hb_case* cs, const hb_mt* mt, int32_t s0, int32_t s1, float* partial,
                              int32_t rows, int32_t row0, void* stream) {
     using namespace hb;
-    if (!cs || !mt || !mt->pool || !mt->mb || !mt->sens || !mt->rf_freq) return hipErrorInvalidValue;
+    if (!cs || !mt || !mt->sens) return hipErrorInvalidValue;
+    if (mt->pool && (!mt->mb || !mt->rf_freq)) return hipErrorInvalidValue;
+    if (mt->conc_mode < HB_CONC_NONE || mt->conc_mode > HB_CONC_EXACT || (mt->conc_mode && !mt->conc))
+        return hipErrorInvalidValue;
     if (cs->n_pad <= 0 || cs->n_pad % kBlockM || cs->n < 0 || cs->n > cs->n_pad) return hipErrorInvalidValue;
     if (cs->n_coils < 1 || cs->n_coils > kMaxCoilsM || cs->n_tiles < 1 || 8 * cs->n_tiles < cs->n_coils)
         return hipErrorInvalidValue;
     if (s0 < 0 || s1 < s0 || s1 > cs->n_segments || rows < 0 || row0 < 0) return hipErrorInvalidValue;
-    if (mt->lineshape == HB_LINE_SUPERLORENTZIAN && (!mt->table || mt->n_table < 2 || !(mt->dlx > 0.0f)))
+    if (mt->pool && mt->lineshape == HB_LINE_SUPERLORENTZIAN && (!mt->table || mt->n_table < 2 || !(mt->dlx > 0.0f)))
         return hipErrorInvalidValue;
-    if (mt->lineshape != HB_LINE_SUPERLORENTZIAN && mt->lineshape != HB_LINE_LORENTZIAN) return hipErrorInvalidValue;
+    if (mt->pool && mt->lineshape != HB_LINE_SUPERLORENTZIAN && mt->lineshape != HB_LINE_LORENTZIAN)
+        return hipErrorInvalidValue;
     if (s1 == s0) return hipSuccess;
     hipLaunchKernelGGL(forward_mt_kernel, dim3(cs->n_pad / kBlockM), dim3(kBlockM), 0, (hipStream_t)stream, *cs,
                        *mt, s0, s1, partial, rows, row0);

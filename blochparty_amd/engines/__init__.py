@@ -16,7 +16,7 @@ import dataclasses
 import torch
 from torch import Tensor
 
-from ..interop import from_reference
+from ..interop import from_reference, scanner_from_reference
 from ..phantom import Phantom
 from ..result import SimResult
 from ..sequence import Sequence
@@ -83,8 +83,11 @@ def simulate(
 
     coils: (N, C) complex receive sensitivities (default one uniform coil). m_init: (N, 3) starting magnetization
     (default equilibrium). checkpoint_every: the adjoint stores the state every this many segments (a multiple of 64;
-    default about sqrt(segments)). merge_free_runs must stay True (the core's option). scanner / maps: refused
-    (NotImplementedError) unless None. seed: only meaningful for diffusion, which is refused.
+    default about sqrt(segments)). merge_free_runs must stay True (the core's option). scanner: the field model
+    (``fields.Scanner``, or the reference's): a gradient nonlinearity encodes every spin at f(r) (forward and with
+    gradients, the coefficients' through autograd); concomitant fields (forward only) take the full-variant kernel,
+    with pulses="step" and readouts="direct". maps: refused (NotImplementedError) unless None. seed: only
+    meaningful for diffusion, which is refused.
     options: ``chunk`` (segments per dispatch, default 32768), ``group`` (256), ``mma`` (the matrix cores always
     contract the coils here; any value is accepted), ``readouts`` ("direct", or "nufft": readouts of >= 16 equal
     samples as a type-1 NUFFT with ``nufft_width`` kernel cells (6) and ``nufft_terms`` decay terms (3); under
@@ -103,6 +106,7 @@ def simulate(
     """
     name = resolve(engine)
     phantom, seq = from_reference(phantom), from_reference(seq)
+    scanner = scanner_from_reference(scanner)
     if unknown := sorted(set(options) - set(GPU_OPTIONS)):
         raise TypeError(f"simulate(engine={engine!r}) got unexpected keyword argument(s) {', '.join(unknown)}")
     if not merge_free_runs:
@@ -125,6 +129,14 @@ def simulate(
     rf_grad = options.get("rf_grad")
     if rf_grad not in (None, "scale"):
         raise ValueError(f"rf_grad must be None or 'scale', not {rf_grad!r}")
+    check(name, features(phantom, seq, m_init, scanner, maps), grad=False)
+    conc = None
+    if scanner is not None:  # the spins at their encoding positions; concomitant coefficients at the true ones
+        from ..fields import spin_fields
+
+        fields = spin_fields(scanner, phantom.pos)
+        phantom = dataclasses.replace(phantom, pos=fields.pos)
+        conc = (fields.conc, fields.mode) if fields.mode else None
     grad = torch.is_grad_enabled() and requires_grad(phantom, seq, coils, m_init)
     check(name, features(phantom, seq, m_init, scanner, maps), grad=grad)
     if grad:
@@ -135,7 +147,7 @@ def simulate(
 
     with torch.no_grad():
         return forward(phantom, seq, coils, m_init, chunk=chunk, reuse=pulses == "reuse",
-                       surrogate=pulses == "surrogate", nufft=nufft)
+                       surrogate=pulses == "surrogate", nufft=nufft, conc=conc)
 
 
 __all__ = ["ENGINES", "GPU_OPTIONS", "available", "resolve", "simulate"]

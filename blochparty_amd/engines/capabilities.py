@@ -1,9 +1,10 @@
 """What the GPU engine accepts, forward and with gradients: one table, one check, one wording.
 
 Reference: ``src/blochparty/engines/capabilities.py:12-91``. The feature names and the message are the reference's;
-the ``hip`` row is this engine's: the plain Bloch path with B1+ and an initial magnetization, forward and with
-gradients; a rate-model bound pool and bound-only RF segments forward. Everything else (the generalized bound pool,
-free pools, diffusion, motion, the scanner field model, field maps) is refused with ``NotImplementedError``.
+the ``hip`` row is this engine's: the plain Bloch path with B1+, an initial magnetization and a gradient
+nonlinearity (encoding positions), forward and with gradients; a rate-model bound pool, bound-only RF segments and a
+scanner's concomitant fields forward. Everything else (the generalized bound pool, free pools, diffusion, motion,
+field maps) is refused with ``NotImplementedError``.
 """
 
 from __future__ import annotations
@@ -16,13 +17,14 @@ FEATURES = {
     "pools": "free exchanging pools (Phantom.pools)",
     "diffusion": "diffusion (Phantom.diffusion)",
     "motion": "motion (Phantom.motion)",
-    "scanner": "the scanner field model (scanner=)",
+    "scanner": "the scanner's concomitant fields (scanner= with a field strength)",
+    "nonlinearity": "the gradient nonlinearity (Scanner.nonlinearity)",
     "bound_only": "bound-pool-only RF segments (Sequence.bound_only)",
     "m_init": "an initial magnetization (m_init)",
 }
 SUPPORTED = {
-    ("hip", "forward"): frozenset({"b1", "m_init", "bound", "bound_only"}),
-    ("hip", "gradients"): frozenset({"b1", "m_init"}),
+    ("hip", "forward"): frozenset({"b1", "m_init", "bound", "bound_only", "scanner", "nonlinearity"}),
+    ("hip", "gradients"): frozenset({"b1", "m_init", "nonlinearity"}),
 }
 
 
@@ -38,7 +40,9 @@ def features(phantom, seq, m_init=None, scanner=None, maps=None) -> frozenset:
         "pools": bool(getattr(phantom, "pools", None)),
         "diffusion": getattr(phantom, "diffusion", None) is not None,
         "motion": getattr(phantom, "motion", None) is not None,
-        "scanner": scanner is not None,
+        "scanner": scanner is not None and getattr(scanner, "concomitant", None) is not None
+        and getattr(scanner, "field_strength", None) is not None,
+        "nonlinearity": getattr(scanner, "nonlinearity", None) is not None,
         "bound_only": bound_only is not None and any(bound_only),
         "m_init": m_init is not None,
     }

@@ -395,20 +395,22 @@ def check_inputs(phantom: Phantom, coils) -> None:
 
 def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, chunk: int = DEFAULT_CHUNK,
                  device=None, partial_bytes: int = PARTIAL_BYTES, aligned: bool = False,
-                 reuse: bool = False, surrogate: bool = False, nufft=None) -> DeviceCase:
+                 reuse: bool = False, surrogate: bool = False, nufft=None, conc=None) -> DeviceCase:
     """Upload a case: the buffers of the module docstring, the dispatch plan and the work buffers. ``aligned``:
     dispatches are exactly [c chunk, (c + 1) chunk) (the adjoint's checkpoint intervals), the partial buffer sized
     for the largest one. ``reuse``: reused pulses applied as per-spin maps; ``surrogate``: those maps from the
     learned surrogate (trained here on first use); ``nufft``: (width, terms) of NUFFT readouts (the spins are then
-    sorted along the readout's phase; ``read_result`` restores their order)."""
+    sorted along the readout's phase; ``read_result`` restores their order). ``conc``: (coefficients (N, 6 or 7),
+    mode) of a scanner's concomitant fields (``fields.spin_fields``), or None."""
+    from . import mt
     from . import nufft as nu_mod
 
     reuse = reuse or surrogate
     check_inputs(phantom, coils)
-    bound = phantom.bound is not None  # the bound-pool kernel (engines/mt.py): plain records only
-    if bound and (reuse or nufft is not None or aligned):
-        raise NotImplementedError("hip engine: a bound pool runs forward only, with pulses='step' and "
-                                  "readouts='direct'")
+    full = mt.needed(phantom, seq, conc)  # the full-variant kernel (engines/mt.py): plain records only
+    if full and (reuse or nufft is not None or aligned):
+        raise NotImplementedError("hip engine: a bound pool, concomitant fields or bound-only RF run forward only, "
+                                  "with pulses='step' and readouts='direct'")
     dev = device if device is not None else hip.HIP.device
     n = len(phantom)
     n_pad = max(BLOCK, math.ceil(n / BLOCK) * BLOCK)
@@ -471,10 +473,8 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
     adc_cum = np.concatenate([[0], np.cumsum(rows_w)])
     row_starts = [int(adc_cum[s]) for s in starts]
     rows = [int(adc_cum[e] - adc_cum[s]) for s, e in zip(starts, ends)]
-    rec = segment_table(seq, starts, reused, runs=not aligned and not bound, nufft=plan)
-    if bound:
-        from . import mt
-
+    rec = segment_table(seq, starts, reused, runs=not aligned and not full, nufft=plan)
+    if full:
         mt.mark_bound_only(rec, seq)
     covered = np.zeros(len(seq), dtype=bool)  # segments applied as reused-pulse maps
     for _, k, occ in reused:
@@ -510,8 +510,8 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
     )
     if plan is not None:
         attach_nufft(case, plan, order, seq, adc_np, starts, ends, dev)
-    if bound:
-        mt.attach(case, phantom, seq, coils, m_init, n, n_pad, dev)
+    if full:
+        mt.attach(case, phantom, seq, coils, m_init, n, n_pad, dev, conc)
     if surrogate and case.n_pulses:
         from . import surrogate as sur
 
@@ -565,7 +565,8 @@ def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> l
     if mt_bufs is not None:
         from . import mt
 
-        mt_bufs["mb"].copy_(mt_bufs["mb0"])
+        if mt_bufs["mb"] is not None:
+            mt_bufs["mb"].copy_(mt_bufs["mb0"])
     st = case.struct()
     n_blocks = case.n_pad // BLOCK
     ckpts = [] if keep else None
@@ -619,7 +620,7 @@ def read_result(case: DeviceCase) -> SimResult:
     signal = torch.complex(sig[..., 0], sig[..., 1]).T.to(torch.complex64).contiguous()
     st = case.state.cpu().double()
     m = (st[:3, : case.n] + st[3:, : case.n]).T.float().contiguous()
-    if case.extra.get("mt") is not None:
+    if case.extra.get("mt") is not None and case.extra["mt"]["mb"] is not None:
         mb = case.extra["mt"]["mb"].cpu().double()
         m = torch.cat([m, (mb[0, : case.n] + mb[1, : case.n]).float()[:, None]], 1).contiguous()
     if case.perm is not None:  # back to the caller's spin order
@@ -635,8 +636,9 @@ def read_result(case: DeviceCase) -> SimResult:
 
 
 def forward(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, chunk: int = DEFAULT_CHUNK,
-            reuse: bool = False, surrogate: bool = False, nufft=None) -> SimResult:
-    case = prepare_case(phantom, seq, coils, m_init, chunk=chunk, reuse=reuse, surrogate=surrogate, nufft=nufft)
+            reuse: bool = False, surrogate: bool = False, nufft=None, conc=None) -> SimResult:
+    case = prepare_case(phantom, seq, coils, m_init, chunk=chunk, reuse=reuse, surrogate=surrogate, nufft=nufft,
+                        conc=conc)
     run_forward(case)
     hip.HIP.synchronize()
     return read_result(case)

@@ -45,4 +45,19 @@ def from_reference(obj):
     return cls(**{k: v for k, v in values.items() if k in ours})
 
 
-__all__ = ["from_reference"]
+def scanner_from_reference(obj):
+    """This package's ``fields.Scanner`` from the reference's (``src/blochparty/fields.py:108-135``) or any object
+    with its attributes (field_strength, concomitant, z_asymmetry, nonlinearity with alpha, beta, r0); tensors shared.
+    None stays None."""
+    from .fields import GradientNonlinearity, Scanner
+
+    if obj is None or isinstance(obj, Scanner):
+        return obj
+    nl = getattr(obj, "nonlinearity", None)
+    if nl is not None and not isinstance(nl, GradientNonlinearity):
+        nl = GradientNonlinearity(nl.alpha, nl.beta, nl.r0)
+    return Scanner(field_strength=getattr(obj, "field_strength", None), concomitant=getattr(obj, "concomitant", None),
+                   z_asymmetry=getattr(obj, "z_asymmetry", 0.5), nonlinearity=nl)
+
+
+__all__ = ["from_reference", "scanner_from_reference"]

@@ -1,10 +1,12 @@
-/* blochparty_amd: the bound-pool (magnetization transfer, rate model) forward path.
+/* blochparty_amd: the full-variant forward path, a bound pool and / or the scanner's concomitant fields.
  *
- * Replaces the reference's V = 1 GPU variant for Phantom.bound with the rate model (step_mt and lineshape of
- * src/blochparty/engines/gpu/shaders/common.metal:113-208, exchange.py:209-256): the free pool keeps its Bloch
- * dynamics, the bound pool only Mzb (equilibrium f PD), the two exchange longitudinal magnetization, and RF saturates
- * the bound pool at W = (2 pi b1 |rf|)^2 G(f_rf - bz - dw). Conventions as blochparty_amd.h: every pointer is device
- * memory the caller owns, every entry point enqueues on `stream` and returns a hipError_t (0 = success). */
+ * Replaces the reference's V = 1 / "_full" GPU variant (engines/gpu/driver.py:520, shaders/common.metal:113-208 and
+ * :563-601): for Phantom.bound with the rate model (step_mt and lineshape, exchange.py:209-256) the free pool keeps
+ * its Bloch dynamics, the bound pool only Mzb (equilibrium f PD), the two exchange longitudinal magnetization, and RF
+ * saturates the bound pool at W = (2 pi b1 |rf|)^2 G(f_rf - bz - dw); for a Scanner's concomitant fields (fields.py,
+ * engines/core.py:115-121, 190-217) every segment's offset gains the spin's concomitant field. Conventions as
+ * blochparty_amd.h: every pointer is device memory the caller owns, every entry point enqueues on `stream` and returns
+ * a hipError_t (0 = success). */
 #ifndef BLOCHPARTY_AMD_MT_H
 #define BLOCHPARTY_AMD_MT_H
 
@@ -23,26 +25,35 @@ extern "C" {
 #define HB_LINE_LORENTZIAN 1
 #define HB_LINE_SUPERLORENTZIAN 2
 
-/* The bound pool of a case. pool: (5, n_pad) f, k [Hz], R1b [1/s], T2b [s], dw [Hz]; mb: (2, n_pad) Mzb and its
- * two-sum low part, in / out; sens: (2 n_coils, n_pad) Re S_0, Im S_0, Re S_1, ...; rf_freq: (n_segments,) each
- * segment's RF frequency [Hz] (a pulse turning as exp(-i 2 pi f t) is resonant at offset f); table: n_table values of
- * ln F(x) at ln x = lx0 + i dlx (the super-Lorentzian integral, G = T2b F(2 pi max(|delta|, cutoff) T2b), 0 past the
- * table's end), NULL for the Lorentzian. */
+/* Concomitant-field modes (the reference's SpinFields.mode). */
+#define HB_CONC_NONE 0
+#define HB_CONC_LEADING 1 /* conc rows c_0..c_5: f_c = c . (gx^2, gy^2, gz^2, gx gy, gx gz, gy gz) [Hz] */
+#define HB_CONC_EXACT 2   /* conc rows ax (3), ay (3), f0: f_c = |B| - f0 - lin, (Bx, By) = (ax . g, ay . g) */
+
+/* A case's bound pool and concomitant fields. pool: (5, n_pad) f, k [Hz], R1b [1/s], T2b [s], dw [Hz], or NULL
+ * without a bound pool (then mb, rf_freq and table are unused); mb: (2, n_pad) Mzb and its two-sum low part, in / out;
+ * sens: (2 n_coils, n_pad) Re S_0, Im S_0, Re S_1, ...; rf_freq: (n_segments,) each segment's RF frequency [Hz] (a
+ * pulse turning as exp(-i 2 pi f t) is resonant at offset f); table: n_table values of ln F(x) at ln x = lx0 + i dlx
+ * (the super-Lorentzian integral, G = T2b F(2 pi max(|delta|, cutoff) T2b), 0 past the table's end), NULL for the
+ * Lorentzian; conc: (7, n_pad) the spins' concomitant coefficients for conc_mode (HB_CONC_*; rows past the mode's
+ * unused), NULL for HB_CONC_NONE. */
 typedef struct hb_mt {
     const float* pool;
     float* mb;
     const float* sens;
     const float* rf_freq;
     const float* table;
+    const float* conc;
     float lx0, dlx, cutoff;
-    int32_t n_table, lineshape; /* HB_LINE_* */
-    int32_t reserved;           /* 0 */
+    int32_t n_table, lineshape; /* HB_LINE_* (ignored without a bound pool) */
+    int32_t conc_mode;          /* HB_CONC_* */
 } hb_mt;
 
-/* The bound-pool event loop over segments [s0, s1) of plain records (no runs, trains, merged runs, reused pulses or
- * NUFFT runs in the table), one lane per spin: every ADC sample's coil sums per workgroup of 256 spins go to
- * partial[(block * rows + r) * 16 * n_tiles + 2 c + {re, im}], r = the segment's pad[0] - row0 (< rows), as
- * hb_forward's, for hb_signal_reduce. Replaces the reference's bloch_chunk_mma with V = 1 and a rate bound pool. */
+/* The full-variant event loop (a bound pool, concomitant fields, bound-only RF: any of them) over segments
+ * [s0, s1) of plain records (no runs, trains, merged runs, reused pulses or NUFFT runs in the table), one lane per
+ * spin: every ADC sample's coil sums per workgroup of 256 spins go to partial[(block * rows + r) * 16 * n_tiles +
+ * 2 c + {re, im}], r = the segment's pad[0] - row0 (< rows), as hb_forward's, for hb_signal_reduce. Replaces the
+ * reference's bloch_chunk_mma with V = 1 (a rate bound pool and / or concomitant fields). */
 int hb_forward_mt(const hb_case* cs, const hb_mt* mt, int32_t s0, int32_t s1, float* partial, int32_t rows,
                   int32_t row0, void* stream);
 

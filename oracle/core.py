@@ -63,17 +63,20 @@ def rotate(mx, my, mz, wx, wy, wz):
 class Spins:
     """The per-spin inputs of the plain path, float64, vectorised over N."""
 
-    def __init__(self, pos, db0, pd, t1, t2, sens_re, sens_im, b1):
+    def __init__(self, pos, db0, pd, t1, t2, sens_re, sens_im, b1, field=None):
         self.pos, self.db0, self.pd = pos, db0, pd
         self.r1, self.r2 = 1.0 / t1, 1.0 / t2
         self.sens_re, self.sens_im, self.b1 = sens_re, sens_im, b1
+        self.field = field  # oracle.fields.Field: the concomitant field (core.py:115-121), or None
 
 
 def rf_segment(mx, my, mz, dt, rf, g, off, sp: Spins):
-    """One RF segment by Strang splitting (core.py:124-150)."""
+    """One RF segment by Strang splitting (core.py:124-150); the concomitant field adds to the offset (:128-129)."""
     bz = sp.pos @ g + sp.db0
     if off is not None:
         bz = bz + off
+    if sp.field is not None:
+        bz = bz + sp.field.at(g, bz)
     mx, my, mz = relax(mx, my, mz, 0.5 * dt, sp.pd, sp.r1, sp.r2)
     w = -TWO_PI * dt
     wx, wy = w * rf[0], w * rf[1]
@@ -98,6 +101,10 @@ def free_run(mx, my, mz, dt, g, off, psi, adc_idx, sp: Spins):
     t = torch.cumsum(dt, 0)
     k = torch.cumsum(g * dt[:, None], 0)
     extra = None if off is None else torch.cumsum(off * dt, 0)
+    if sp.field is not None:  # per spin and segment (core.py:190-217): the field of each segment times its length
+        lin = sp.pos @ g.T + sp.db0[:, None] + (0.0 if off is None else off[None])
+        conc = torch.cumsum(sp.field.at(g, lin) * dt[None], 1)  # (N, n)
+        extra = conc if extra is None else conc + extra[None]
     sig = None
     if adc_idx:
         idx = torch.tensor(adc_idx, device=dt.device)
@@ -110,7 +117,7 @@ def free_run(mx, my, mz, dt, g, off, psi, adc_idx, sp: Spins):
             tj, kj = ta[j : j + block], ka[j : j + block]
             cyc = sp.pos @ kj.T + sp.db0[:, None] * tj
             if extra is not None:
-                cyc = cyc + extra[idx[j : j + block]]
+                cyc = cyc + extra[..., idx[j : j + block]]
             phase = TWO_PI * cyc
             decay = torch.exp(-sp.r2[:, None] * tj)
             e_re, e_im = decay * torch.cos(phase), -decay * torch.sin(phase)
@@ -120,7 +127,7 @@ def free_run(mx, my, mz, dt, g, off, psi, adc_idx, sp: Spins):
         sig = torch.stack([s_re * c + s_im * s, s_im * c - s_re * s]).permute(2, 0, 1)  # (n, 2, C)
     cyc = sp.pos @ k[-1] + sp.db0 * t[-1]
     if extra is not None:
-        cyc = cyc + extra[-1]
+        cyc = cyc + extra[..., -1]
     phi = -TWO_PI * cyc
     c, s = torch.cos(phi), torch.sin(phi)
     mx, my = c * mx - s * my, s * mx + c * my
@@ -143,8 +150,10 @@ def steps(is_rf, spoil, merge: bool) -> list[tuple[int, int]]:
 
 
 def simulate(phantom, seq, coils: Tensor | None = None, *, m_init: Tensor | None = None,
-             merge_free_runs: bool = True, dtype=torch.float64):
-    """The oracle's ``simulate`` (core.py:487-608, plain path): returns (signal (C, n_adc) complex, m (N, 3))."""
+             merge_free_runs: bool = True, dtype=torch.float64, field=None, enc_pos=None):
+    """The oracle's ``simulate`` (core.py:487-608, plain path): returns (signal (C, n_adc) complex, m (N, 3)).
+    ``field``: an ``oracle.fields.Field`` (the concomitant field of a scanner); ``enc_pos``: encoding positions in
+    place of the phantom's (a gradient nonlinearity)."""
     pd = phantom.pd.to(dtype)
     dev = pd.device
     conv = lambda t: None if t is None else t.to(dtype=dtype, device=dev)
@@ -157,8 +166,9 @@ def simulate(phantom, seq, coils: Tensor | None = None, *, m_init: Tensor | None
     else:
         cplx = coils.to(torch.complex128 if dtype == torch.float64 else torch.complex64)
         sens_re, sens_im = cplx.real, cplx.imag
-    sp = Spins(conv(phantom.pos), conv(phantom.db0), pd, conv(phantom.t1), conv(phantom.t2), sens_re, sens_im,
-               conv(phantom.b1))
+    pos = conv(phantom.pos) if enc_pos is None else conv(enc_pos)
+    sp = Spins(pos, conv(phantom.db0), pd, conv(phantom.t1), conv(phantom.t2), sens_re, sens_im, conv(phantom.b1),
+               field)
     if m_init is None:
         mx, my, mz = torch.zeros_like(pd), torch.zeros_like(pd), pd
     else:

@@ -90,10 +90,13 @@ class Pool:
 
 def rf_segment_bound(mx, my, mz, mb, dt, rf, g, off, f_rf, sp: core.Spins, pool: Pool, bound_only=False):
     """One RF segment with the bound pool (core.py:124-150): the bound pool saturated at W = w1^2 G(f_rf - bz - dw)
-    over the whole segment, Strang-split around the water's rotation (no rotation for bound-only RF)."""
+    over the whole segment, Strang-split around the water's rotation (no rotation for bound-only RF); a scanner's
+    concomitant field adds to bz (core.py:128-129)."""
     bz = sp.pos @ g + sp.db0
     if off is not None:
         bz = bz + off
+    if sp.field is not None:
+        bz = bz + sp.field.at(g, bz)
     w1sq = (2 * math.pi) ** 2 * (rf[0] ** 2 + rf[1] ** 2)
     if sp.b1 is not None:
         w1sq = w1sq * sp.b1 * sp.b1
@@ -113,7 +116,7 @@ def rf_segment_bound(mx, my, mz, mb, dt, rf, g, off, f_rf, sp: core.Spins, pool:
     return half(mx, my, mz, mb)
 
 
-def simulate_bound(phantom, seq, coils=None, *, m_init=None, rf_freq=None, dtype=torch.float64):
+def simulate_bound(phantom, seq, coils=None, *, m_init=None, rf_freq=None, dtype=torch.float64, field=None):
     """``core.simulate`` with ``phantom.bound`` (rate model): returns (signal (C, n_adc), m (N, 4): mx, my, mz, mzb).
     ``rf_freq``: (S,) each segment's RF frequency (default ``seq.rf_freq``, else the package's ``rf_frequency``
     rule restated: minus the mean phase step per time to the RF neighbours)."""
@@ -126,7 +129,7 @@ def simulate_bound(phantom, seq, coils=None, *, m_init=None, rf_freq=None, dtype
         c = coils.to(torch.complex128)
         sens_re, sens_im = c.real, c.imag
     sp = core.Spins(conv(phantom.pos), conv(phantom.db0), pd, conv(phantom.t1), conv(phantom.t2), sens_re, sens_im,
-                    conv(phantom.b1))
+                    conv(phantom.b1), field)
     pool = Pool(phantom.bound, n, dtype)
     dt, rf, g, psi = conv(seq.dt), conv(seq.rf), conv(seq.grad), conv(seq.adc_phase)
     off_all = conv(seq.b0_offset)

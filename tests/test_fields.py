@@ -1,0 +1,196 @@
+"""The scanner field model on the CPU: the oracle's concomitant field and nonlinearity pinned by closed forms
+(Bernstein's leading-order formula, Maxwell's equations, an extended-precision |B|, hand-written low-order solid
+harmonics), the package's functions against the oracle's, the coefficient-file reader, and the oracle's simulate
+with a field against Bernstein's phase of a bipolar gradient pair."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from blochparty_amd import fields as F
+from oracle import core
+from oracle import fields as OF
+
+f64 = torch.float64
+
+
+def bernstein(pos, g, f0):
+    """Bernstein et al. (MRM 1998): the leading-order concomitant field of the symmetric coil [Hz], written out."""
+    x, y, z = pos.unbind(-1)
+    gx, gy, gz = g
+    return ((gx * gx + gy * gy) * z * z + gz * gz * (x * x + y * y) / 4 - gx * gz * x * z - gy * gz * y * z) / (2 * f0)
+
+
+def random_pos(n, seed):
+    return (torch.rand(n, 3, generator=torch.Generator().manual_seed(seed), dtype=f64) - 0.5) * 0.4
+
+
+def test_oracle_leading_order_is_bernsteins_formula_and_obeys_maxwell():
+    pos = random_pos(200, 0)
+    f0 = F.GAMMA_BAR * 1.5
+    for g in (torch.tensor([3e5, -1e5, 2e5], dtype=f64), torch.tensor([0.0, 0.0, 4e5], dtype=f64)):
+        mine = OF.concomitant(1, pos, g, None, f0, 0.5)
+        assert float((mine - bernstein(pos, g, f0)).abs().max() / bernstein(pos, g, f0).abs().max()) < 1e-13
+    # (Bx, By, Bz = g.r) is curl- and divergence-free: its Jacobian in r is symmetric and traceless, for any a
+    g = torch.tensor([2e5, 1.5e5, -3e5], dtype=f64)
+    for a in (0.5, 0.3):
+        r = torch.tensor([0.05, -0.07, 0.03], dtype=f64)
+        jac = torch.autograd.functional.jacobian(
+            lambda q: torch.stack([*OF.transverse_field(q[None], g, a), (q @ g)[None]]).reshape(3), r)
+        assert abs(float(jac.trace())) < 1e-9 * float(g.abs().max())
+        assert torch.allclose(jac, jac.T, atol=1e-9 * float(g.abs().max()))
+
+
+def test_oracle_exact_mode_against_extended_precision_and_the_leading_order():
+    pos = random_pos(64, 1)
+    g = torch.tensor([3e5, 2e5, -4e5], dtype=f64)
+    lin = pos @ g + 37.0
+    bx, by = OF.transverse_field(pos, g, 0.5)
+    for field in (0.05, 0.55, 3.0):
+        f0 = F.GAMMA_BAR * field
+        mine = OF.concomitant(2, pos, g, lin, f0, 0.5).numpy()
+        u = (np.float64(f0) + lin.numpy()).astype(np.longdouble)
+        t2 = (bx.numpy() ** 2 + by.numpy() ** 2).astype(np.longdouble)
+        ref = (np.sqrt(u * u + t2) - u).astype(np.float64)
+        assert np.abs(mine - ref).max() < 1e-9 * np.abs(ref).max()
+    # at high field exact -> leading, the difference falling as 1 / f0^2
+    diffs = []
+    for field in (3.0, 6.0, 12.0):
+        f0 = F.GAMMA_BAR * field
+        diffs.append(float((OF.concomitant(2, pos, g, lin, f0, 0.5) - OF.concomitant(1, pos, g, lin, f0, 0.5))
+                           .abs().max()))
+    assert 3.6 < diffs[0] / diffs[1] < 4.4 and 3.6 < diffs[1] / diffs[2] < 4.4
+
+
+def test_oracle_displacement_low_orders_by_hand():
+    pos = random_pos(50, 2)
+    x, y, z = pos.unbind(-1)
+    r0 = 0.25
+    alpha, beta = np.zeros((3, 3, 3)), np.zeros((3, 3, 3))
+    alpha[0, 1, 1] = 0.7  # x coil: N_11 P_1^1 cos(phi) r = sqrt(3)/2 x
+    alpha[1, 1, 0] = -0.2  # y coil: P_1 r = z
+    alpha[2, 2, 0] = 0.3  # z coil: R0 (r/R0)^2 (3 cos^2 - 1) / 2 = (2 z^2 - x^2 - y^2) / (2 R0)
+    beta[2, 2, 1] = 0.1  # z coil: R0 (r/R0)^2 N_21 P_2^1 sin(phi) = sqrt(5/12) 3 y z / R0
+    d = OF.displacement(pos, alpha, beta, r0)
+    expect = torch.stack([0.7 * math.sqrt(3) / 2 * x, -0.2 * z,
+                          0.3 * (2 * z * z - x * x - y * y) / (2 * r0) + 0.1 * math.sqrt(5 / 12) * 3 * y * z / r0], 1)
+    assert torch.allclose(d, expect, atol=1e-14)
+
+
+def test_package_concomitant_coefficients_match_the_oracle():
+    pos = random_pos(300, 3)
+    f0 = F.GAMMA_BAR * 0.55
+    gen = torch.Generator().manual_seed(4)
+    for a in (0.5, 0.37):
+        c = F.concomitant_coefficients(pos, f0, a)
+        t = F.transverse_coefficients(pos, a)
+        for _ in range(5):
+            g = (torch.rand(3, generator=gen, dtype=f64) - 0.5) * 8e5
+            mine = c @ F.gradient_monomials(g)
+            ref = OF.concomitant(1, pos, g, None, f0, a)
+            assert float((mine - ref).abs().max() / ref.abs().max()) < 1e-13
+            lin = pos @ g
+            mine_x = F.exact_excess(lin, t[:, :3] @ g, t[:, 3:] @ g, f0)
+            ref_x = OF.concomitant(2, pos, g, lin, f0, a)
+            assert float((mine_x - ref_x).abs().max() / ref_x.abs().max()) < 1e-8
+
+
+def test_package_solid_harmonics_match_scipy():
+    pos = random_pos(120, 5)
+    gen = np.random.default_rng(6)
+    alpha, beta = gen.normal(size=(3, 6, 6)) * 1e-3, gen.normal(size=(3, 6, 6)) * 1e-3
+    nl = F.GradientNonlinearity(torch.from_numpy(alpha), torch.from_numpy(beta), 0.25)
+    mine = nl.displacement(pos)
+    ref = OF.displacement(pos, np.tril(alpha), np.tril(beta), 0.25)  # entries m > n ignored by both
+    assert torch.allclose(mine, ref, rtol=0, atol=1e-13)
+    assert torch.allclose(nl.encoding_positions(pos), pos + ref, rtol=0, atol=1e-13)
+
+
+def test_coefficient_files(tmp_path):
+    siemens = tmp_path / "coil.grad"
+    siemens.write_text("# a Siemens-style file\n 0.25 m = R0 of the expansion\n"
+                       "  1 A( 3, 0)  -0.0123 z\n  2 A( 3, 1)   0.0045 x\n  3 B( 3, 1)  -0.0067 y\n  4 A( 5, 2) 1.5e-4 x\n")
+    nl = F.GradientNonlinearity.from_file(siemens)
+    assert nl.r0 == 0.25 and nl.alpha.shape == (3, 6, 6)
+    assert float(nl.alpha[2, 3, 0]) == -0.0123 and float(nl.alpha[0, 3, 1]) == 0.0045
+    assert float(nl.beta[1, 3, 1]) == -0.0067 and float(nl.alpha[0, 5, 2]) == 1.5e-4
+    named = tmp_path / "coil.coef"
+    named.write_text("R0 = 250 mm\nalpha_z 3 0 -0.0123\nbeta_y 3 1 -0.0067;\n")
+    nl2 = F.GradientNonlinearity.from_file(named)
+    assert nl2.r0 == 0.25 and float(nl2.alpha[2, 3, 0]) == -0.0123 and float(nl2.beta[1, 3, 1]) == -0.0067
+    (tmp_path / "bad.grad").write_text(" 1 A( 1, 2) 0.1 x\n")
+    with pytest.raises(ValueError):
+        F.GradientNonlinearity.from_file(tmp_path / "bad.grad", r0=0.2)
+    with pytest.raises(ValueError):
+        F.Scanner(field_strength=1.5, concomitant="quadratic")
+
+
+def test_spin_fields_modes():
+    pos = random_pos(10, 7)
+    assert F.spin_fields(None, pos).mode == 0
+    sc = F.Scanner(field_strength=3.0)
+    sf = F.spin_fields(sc, pos)
+    assert sf.mode == 1 and sf.conc.shape == (10, 6) and torch.equal(sf.pos, pos)
+    sf = F.spin_fields(F.Scanner(field_strength=3.0, concomitant="exact"), pos)
+    assert sf.mode == 2 and sf.conc.shape == (10, 7) and float(sf.conc[0, 6]) == F.GAMMA_BAR * 3.0
+    nl = F.GradientNonlinearity(torch.zeros(3, 2, 2, dtype=f64), torch.zeros(3, 2, 2, dtype=f64), 0.2)
+    nl.alpha[0, 1, 1] = 0.01
+    sf = F.spin_fields(F.Scanner(nonlinearity=nl), pos)
+    assert sf.mode == 0 and not torch.equal(sf.pos, pos)
+
+
+def bipolar(g, delta, n_steps=4):
+    """Free segments: +g for delta, then -g for delta (each in n_steps segments), ideal elsewhere."""
+    from blochparty_amd.sequence import Sequence
+
+    k = 2 * n_steps
+    dt = torch.full((k,), delta / n_steps, dtype=f64)
+    grad = torch.cat([g.expand(n_steps, 3), -g.expand(n_steps, 3)])
+    return Sequence(dt=dt, rf=torch.zeros(k, 2, dtype=f64), grad=grad, adc_phase=torch.zeros(k, dtype=f64),
+                    is_rf=[False] * k, adc=[False] * (k - 1) + [True], spoil=[False] * k)
+
+
+def test_oracle_simulate_bipolar_phase_is_bernsteins():
+    """The linear phases of +g and -g cancel, the concomitant field (quadratic in g) does not: the magnetization
+    turns by -2 pi f_c(g) 2 delta (Bernstein's analytic phase), in free runs and stepped segment by segment."""
+    from blochparty_amd.phantom import Phantom
+
+    pos = random_pos(40, 8)
+    n = len(pos)
+    ph = Phantom(pos=pos, pd=torch.ones(n, dtype=f64), t1=torch.full((n,), 1e9, dtype=f64),
+                 t2=torch.full((n,), 1e9, dtype=f64), db0=torch.zeros(n, dtype=f64))
+    g, delta, f0 = torch.tensor([1.2e5, -0.8e5, 2.0e5], dtype=f64), 5e-3, F.GAMMA_BAR * 0.55
+    m0 = torch.zeros(n, 3, dtype=f64)
+    m0[:, 0] = 1.0
+    seq = bipolar(g, delta)
+    field = OF.Field(pos, f0, 0.5, 1)
+    expect = torch.exp(-2j * math.pi * bernstein(pos, g, f0) * 2 * delta)
+    for merge in (True, False):
+        _, m = core.simulate(ph, seq, m_init=m0, field=field, merge_free_runs=merge)
+        got = torch.complex(m[:, 0], m[:, 1])
+        assert float((got - expect).abs().max()) < 1e-10
+    _, m = core.simulate(ph, seq, m_init=m0)
+    assert float((torch.complex(m[:, 0], m[:, 1]) - 1).abs().max()) < 1e-10  # an ideal scanner: no phase
+
+
+def test_capabilities_of_the_field_model():
+    """Concomitant fields: forward only; a nonlinearity alone: forward and gradients."""
+    from blochparty_amd.engines.capabilities import check, features
+    from blochparty_amd.phantom import Phantom
+
+    pos = random_pos(4, 9)
+    ph = Phantom(pos=pos, pd=torch.ones(4, dtype=f64), t1=torch.ones(4, dtype=f64), t2=torch.ones(4, dtype=f64),
+                 db0=torch.zeros(4, dtype=f64))
+    seq = bipolar(torch.tensor([1e5, 0.0, 0.0], dtype=f64), 1e-3)
+    used = features(ph, seq, None, F.Scanner(field_strength=1.5), None)
+    assert "scanner" in used
+    check("hip", used, grad=False)
+    with pytest.raises(NotImplementedError, match="concomitant"):
+        check("hip", used, grad=True)
+    nl = F.GradientNonlinearity(torch.zeros(3, 2, 2, dtype=f64), torch.zeros(3, 2, 2, dtype=f64), 0.2)
+    used = features(ph, seq, None, F.Scanner(nonlinearity=nl), None)
+    assert used == {"nonlinearity"}
+    check("hip", used, grad=True)
+    assert features(ph, seq, None, F.Scanner(), None) == frozenset()  # no field strength, no nonlinearity
