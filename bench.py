@@ -199,6 +199,62 @@ def fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, steps: int,
     return leg
 
 
+def value_leg(line, ph, seq, coils, nufft, lib, driver, sur, steps: int, warmup: int, work: int) -> dict:
+    """The headline: the forward with the learned surrogate's maps (the network trained first, outside the timed
+    region), with the forward kernel's roofline."""
+    t0 = time.perf_counter()
+    case_s = driver.prepare_case(ph, seq, coils, surrogate=True, nufft=nufft)
+    torch.cuda.synchronize()
+    nets = case_s.extra["surrogate"]
+    log(f"surrogate: case prepared and {len(nets)} network(s) trained in {time.perf_counter() - t0:.1f} s")
+    lead = forward_leg(case_s, lib, driver, steps, warmup, work, "learned RF surrogate")
+    line["value"], line["ms_per_step"] = lead["value"], lead["ms_per_step"]
+    line["roofline"] = {
+        "bound": "valu", "kernel": "forward_kernel", "achieved": lead["kernel_tflops"], "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s", "frac": lead["kernel_tflops"] / FP32_PEAK_TFLOPS, "traffic": profiled_traffic(),
+        "mean_launch_ms": lead["kernel_mean_ms"], "launches": lead["launches"],
+        "flops_per_launch": lead["flops_per_launch"],
+    }
+    line["surrogate"] = {"forward": lead, "judge": sur.report(case_s)}
+    del case_s
+    torch.cuda.empty_cache()
+    return lead
+
+
+def copy_probe(mib: int) -> None:
+    """A plain copy of known size for scaling byte counters: x * 1 into y reads and writes ``mib`` MiB, three times."""
+    x = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    log(f"copy probe: 3 x {mib} MiB read + {mib} MiB written")
+
+
+def adjoint_legs(args, line, emit, ph, seq, coils, nufft, lib, adjoint, driver, adj_k, adj_w, work) -> None:
+    """The forward + adjoint legs: the surrogate's maps (or, with --adjoint-pulses reuse, the exact reused maps) with
+    NUFFT readouts, then per-sample readouts, then exact RF."""
+    every = args.checkpoint_every
+    maps = dict(surrogate=True) if args.adjoint_pulses == "surrogate" else dict(reuse=True)
+    name = "learned RF surrogate" if args.adjoint_pulses == "surrogate" else "reused exact maps"
+    tag = "surrogate" if args.adjoint_pulses == "surrogate" else "reuse"
+    legs = [(f"fwd_adjoint_{tag}_nufft", dict(maps, nufft=nufft), f"{name}, NUFFT readouts")] if nufft else []
+    legs += [(f"fwd_adjoint_{tag}", dict(maps), name), ("fwd_adjoint", {}, "exact RF")]
+    for key, kw, label in legs:
+        if key == "fwd_adjoint" and args.no_exact:
+            continue
+        case_a = driver.prepare_case(ph, seq, coils, chunk=every, aligned=True, **kw)
+        line[key] = fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, adj_k, adj_w, work, label)
+        del case_a
+        torch.cuda.empty_cache()
+        best = line.get(f"fwd_adjoint_{tag}_nufft") or line.get(f"fwd_adjoint_{tag}")
+        if line.get("fwd_adjoint") and best:
+            line["fwd_adjoint_speedup_vs_exact_rf"] = best["value"] / line["fwd_adjoint"]["value"]
+        emit()
+    if args.copy_probe and args.adjoint_only:  # the counters' scale, as in the full run
+        copy_probe(args.copy_probe)
+
+
 def multislice_leg(W, driver, sur, lib, steps: int, warmup: int, nufft) -> dict:
     """The learned surrogate's own workload (multi-slice 2D GRE, 5 slices x 256^2 x 8 isochromats, 3 ms sinc at
     30 deg, B1 0.7-1.3, +-200 Hz): forward with the surrogate's maps against stepping every RF substep, both with
@@ -300,6 +356,14 @@ def main() -> None:
                     help="segments per checkpoint interval of the forward + adjoint legs (a multiple of 64; 4 x the "
                          "library's sqrt-of-segments default: 256 checkpoints of 24 B per spin and a 6.5 GB replay "
                          "scratch fit one GPU's HBM many times, and fewer, longer launches shorten the tails)")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="no network trained (counter runs): skip the surrogate's value leg, and the forward + adjoint "
+                         "legs take the exact reused maps")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the forward + adjoint legs (profiling runs: no value, CPU, multi-slice or end-to-end legs)")
+    ap.add_argument("--adjoint-pulses", choices=("surrogate", "reuse"), default="surrogate",
+                    help="the pulses of the non-exact forward + adjoint legs: the learned surrogate's maps, or the exact "
+                         "reused maps (no network trained: for counter runs)")
     ap.add_argument("--cpu-trs", type=int, default=32)
     ap.add_argument("--cpu-spins", type=int, default=65536)
     args = ap.parse_args()
@@ -309,6 +373,11 @@ def main() -> None:
         sys.exit(2)
     if args.only_value:
         args.no_cpu = args.no_adjoint = args.no_exact = args.no_e2e = args.no_multislice = True
+    if args.adjoint_only:
+        args.no_cpu = args.no_e2e = args.no_multislice = True
+    if args.no_surrogate:
+        args.no_e2e = args.no_multislice = True
+        args.adjoint_pulses = "reuse"
     if args.checkpoint_every % 64:
         raise SystemExit("--checkpoint-every must be a multiple of 64")
 
@@ -353,22 +422,11 @@ def main() -> None:
         print(json.dumps(line), flush=True)
 
     # -- the value: the surrogate leg
-    t0 = time.perf_counter()
-    case_s = driver.prepare_case(ph, seq, coils, surrogate=True, nufft=nufft)
-    torch.cuda.synchronize()
-    nets = case_s.extra["surrogate"]
-    log(f"surrogate: case prepared and {len(nets)} network(s) trained in {time.perf_counter() - t0:.1f} s")
-    lead = forward_leg(case_s, lib, driver, K, Wm, work, "learned RF surrogate")
-    line["value"], line["ms_per_step"] = lead["value"], lead["ms_per_step"]
-    line["roofline"] = {
-        "bound": "valu", "kernel": "forward_kernel", "achieved": lead["kernel_tflops"], "peak": FP32_PEAK_TFLOPS,
-        "unit": "TFLOP/s", "frac": lead["kernel_tflops"] / FP32_PEAK_TFLOPS, "traffic": profiled_traffic(),
-        "mean_launch_ms": lead["kernel_mean_ms"], "launches": lead["launches"],
-        "flops_per_launch": lead["flops_per_launch"],
-    }
-    line["surrogate"] = {"forward": lead, "judge": sur.report(case_s)}
-    del case_s
-    torch.cuda.empty_cache()
+    if args.adjoint_only:
+        return adjoint_legs(args, line, emit, ph, seq, coils, nufft, lib, adjoint, driver, adj_k, adj_w, work)
+    lead = None
+    if not args.no_surrogate:
+        lead = value_leg(line, ph, seq, coils, nufft, lib, driver, sur, K, Wm, work)
 
     # -- the CPU baseline (bounded sample); the GPU's parity on the same sample comes after the device legs
     if not args.no_cpu:
@@ -402,13 +460,15 @@ def main() -> None:
             case_e = driver.prepare_case(ph, seq, coils, nufft=nufft)
             line["exact_rf_nufft_forward"] = forward_leg(case_e, lib, driver, leg_k, leg_w, work,
                                                          "exact RF, NUFFT readouts")
-            line["speedup_vs_exact_rf_same_readouts"] = lead["value"] / line["exact_rf_nufft_forward"]["value"]
+            if lead is not None:
+                line["speedup_vs_exact_rf_same_readouts"] = lead["value"] / line["exact_rf_nufft_forward"]["value"]
             del case_e
             torch.cuda.empty_cache()
             emit()
         case_e = driver.prepare_case(ph, seq, coils)
         line["exact_rf_forward"] = forward_leg(case_e, lib, driver, leg_k, leg_w, work, "exact RF")
-        line["speedup_vs_exact_rf"] = lead["value"] / line["exact_rf_forward"]["value"]
+        if lead is not None:
+            line["speedup_vs_exact_rf"] = lead["value"] / line["exact_rf_forward"]["value"]
         del case_e
         torch.cuda.empty_cache()
         emit()
@@ -420,22 +480,7 @@ def main() -> None:
 
     # -- forward + adjoint: the surrogate's maps with the value's readouts first, then exact RF
     if not args.no_adjoint:
-        every = args.checkpoint_every
-        legs = [("fwd_adjoint_surrogate_nufft", dict(surrogate=True, nufft=nufft),
-                 "learned RF surrogate, NUFFT readouts")] if nufft else []
-        legs += [("fwd_adjoint_surrogate", dict(surrogate=True), "learned RF surrogate"),
-                 ("fwd_adjoint", {}, "exact RF")]
-        for key, kw, label in legs:
-            if key == "fwd_adjoint" and args.no_exact:
-                continue
-            case_a = driver.prepare_case(ph, seq, coils, chunk=every, aligned=True, **kw)
-            line[key] = fwd_adjoint_leg(case_a, seq, coils, every, lib, adjoint, driver, adj_k, adj_w, work, label)
-            del case_a
-            torch.cuda.empty_cache()
-            best = line.get("fwd_adjoint_surrogate_nufft") or line.get("fwd_adjoint_surrogate")
-            if line.get("fwd_adjoint") and best:
-                line["fwd_adjoint_speedup_vs_exact_rf"] = best["value"] / line["fwd_adjoint"]["value"]
-            emit()
+        adjoint_legs(args, line, emit, ph, seq, coils, nufft, lib, adjoint, driver, adj_k, adj_w, work)
 
     # -- simulate() end to end
     if not args.no_e2e:
@@ -456,14 +501,8 @@ def main() -> None:
                                                         if k.endswith("signal_rel_err")))
         emit()
 
-    if args.copy_probe:  # a plain copy of known size: x * 1 into y reads and writes copy_probe MiB each
-        x = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(3):
-            torch.mul(x, 1.0, out=y)
-        torch.cuda.synchronize()
-        log(f"copy probe: 3 x {args.copy_probe} MiB read + {args.copy_probe} MiB written")
-        del x, y
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
 
 
 if __name__ == "__main__":
