@@ -84,9 +84,9 @@ def simulate(
     coils: (N, C) complex receive sensitivities (default one uniform coil). m_init: (N, 3) starting magnetization
     (default equilibrium). checkpoint_every: the adjoint stores the state every this many segments (a multiple of 64;
     default about sqrt(segments)). merge_free_runs must stay True (the core's option). scanner: the field model
-    (``fields.Scanner``, or the reference's): a gradient nonlinearity encodes every spin at f(r) (forward and with
-    gradients, the coefficients' through autograd); concomitant fields (forward only) take the full-variant kernel,
-    with pulses="step" and readouts="direct". maps: refused (NotImplementedError) unless None. seed: only
+    (``fields.Scanner``, or the reference's): a gradient nonlinearity encodes every spin at f(r) (the coefficients'
+    gradients through autograd); concomitant fields take the full-variant kernels (as a bound pool and bound-only RF
+    do), with pulses="step" and readouts="direct", forward and with gradients. maps: refused (NotImplementedError) unless None. seed: only
     meaningful for diffusion, which is refused.
     options: ``chunk`` (segments per dispatch, default 32768), ``group`` (256), ``mma`` (the matrix cores always
     contract the coils here; any value is accepted), ``readouts`` ("direct", or "nufft": readouts of >= 16 equal
@@ -140,6 +140,15 @@ def simulate(
     grad = torch.is_grad_enabled() and requires_grad(phantom, seq, coils, m_init)
     check(name, features(phantom, seq, m_init, scanner, maps), grad=grad)
     if grad:
+        from . import mt
+
+        if mt.needed(phantom, seq, conc):
+            if pulses != "step" or nufft is not None:
+                raise NotImplementedError("hip engine: a bound pool, concomitant fields or bound-only RF run with "
+                                          "pulses='step' and readouts='direct'")
+            from .full_adjoint import simulate_grad as simulate_grad_full
+
+            return simulate_grad_full(phantom, seq, coils, m_init, checkpoint_every, conc)
         from .adjoint import simulate_grad
 
         return simulate_grad(phantom, seq, coils, m_init, checkpoint_every, pulses, nufft, rf_grad)

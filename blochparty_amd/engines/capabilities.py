@@ -1,9 +1,9 @@
 """What the GPU engine accepts, forward and with gradients: one table, one check, one wording.
 
 Reference: ``src/blochparty/engines/capabilities.py:12-91``. The feature names and the message are the reference's;
-the ``hip`` row is this engine's: the plain Bloch path with B1+, an initial magnetization and a gradient
-nonlinearity (encoding positions), forward and with gradients; a rate-model bound pool, bound-only RF segments and a
-scanner's concomitant fields forward. Everything else (the generalized bound pool, free pools, diffusion, motion,
+the ``hip`` row is this engine's: the plain Bloch path with B1+, an initial magnetization, a gradient nonlinearity
+(encoding positions), a rate-model bound pool, bound-only RF segments and a scanner's concomitant fields, forward and
+with gradients. Everything else (the generalized bound pool, free pools, diffusion, motion,
 field maps) is refused with ``NotImplementedError``.
 """
 
@@ -24,7 +24,7 @@ FEATURES = {
 }
 SUPPORTED = {
     ("hip", "forward"): frozenset({"b1", "m_init", "bound", "bound_only", "scanner", "nonlinearity"}),
-    ("hip", "gradients"): frozenset({"b1", "m_init", "nonlinearity"}),
+    ("hip", "gradients"): frozenset({"b1", "m_init", "bound", "bound_only", "scanner", "nonlinearity"}),
 }
 
 

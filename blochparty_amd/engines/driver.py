@@ -407,10 +407,10 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
 
     reuse = reuse or surrogate
     check_inputs(phantom, coils)
-    full = mt.needed(phantom, seq, conc)  # the full-variant kernel (engines/mt.py): plain records only
-    if full and (reuse or nufft is not None or aligned):
-        raise NotImplementedError("hip engine: a bound pool, concomitant fields or bound-only RF run forward only, "
-                                  "with pulses='step' and readouts='direct'")
+    full = mt.needed(phantom, seq, conc)  # the full-variant kernels (engines/mt.py): plain records only
+    if full and (reuse or nufft is not None):
+        raise NotImplementedError("hip engine: a bound pool, concomitant fields or bound-only RF run with "
+                                  "pulses='step' and readouts='direct'")
     dev = device if device is not None else hip.HIP.device
     n = len(phantom)
     n_pad = max(BLOCK, math.ceil(n / BLOCK) * BLOCK)
@@ -583,8 +583,9 @@ def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> l
         nu_arg = hip.HbNufft(nu["nu"].data_ptr(), nu["nu_win"].data_ptr(), nu["part"].data_ptr(), nu["runs_max"],
                              nu["cols"], nu["plan"].terms, 0)
     for d, (s0, s1, r0, nr) in enumerate(zip(case.starts, case.ends, case.row_starts, case.rows)):
-        if keep:
-            ckpts.append(case.state.clone())
+        if keep:  # with a bound pool, its Mzb and low part as rows 6, 7
+            mb = None if mt_bufs is None else mt_bufs["mb"]
+            ckpts.append(case.state.clone() if mb is None else torch.cat([case.state, mb]))
         st.rf_free = int(bool(case.rf_free) and case.rf_free[d])
         if timer:
             timer("begin")
@@ -614,20 +615,21 @@ def run_forward(case: DeviceCase, lib=None, keep: bool = False, timer=None) -> l
 
 
 def read_result(case: DeviceCase) -> SimResult:
-    """(C, n_adc) complex64 signal and (N, 3) float32 magnetization on the CPU (the GPU engines' contract); (N, 4)
-    with a bound pool, its Mzb last."""
+    """(C, n_adc) complex64 signal and (N, 3) float32 magnetization on the CPU (the GPU engines' contract), with a
+    bound pool its Mz in ``m_bound`` (N,) (the reference's driver.py:266)."""
     sig = case.sig[: case.n_adc].cpu()
     signal = torch.complex(sig[..., 0], sig[..., 1]).T.to(torch.complex64).contiguous()
     st = case.state.cpu().double()
     m = (st[:3, : case.n] + st[3:, : case.n]).T.float().contiguous()
+    m_bound = None
     if case.extra.get("mt") is not None and case.extra["mt"]["mb"] is not None:
         mb = case.extra["mt"]["mb"].cpu().double()
-        m = torch.cat([m, (mb[0, : case.n] + mb[1, : case.n]).float()[:, None]], 1).contiguous()
-    if case.perm is not None:  # back to the caller's spin order
+        m_bound = (mb[0, : case.n] + mb[1, : case.n]).float().contiguous()
+    if case.perm is not None:  # back to the caller's spin order (NUFFT readouts; never with a bound pool)
         inv = np.empty_like(case.perm)
         inv[case.perm] = np.arange(len(case.perm))
         m = m[torch.from_numpy(inv)].contiguous()
-    res = SimResult(signal, m)
+    res = SimResult(signal, m, m_bound)
     if case.extra.get("surrogate") is not None:
         from . import surrogate
 

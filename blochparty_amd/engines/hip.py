@@ -15,7 +15,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent.parent / "libblochparty_amd.so"
-ABI_VERSION = 22
+ABI_VERSION = 23
 MAX_COILS = 128  # 16 tiles of 8 coils (the forward kernel's LDS budget)
 
 

@@ -3,8 +3,8 @@ and bound-only RF, through ``hb_forward_mt`` (``csrc/mt.hip``, ``include/blochpa
 
 Reference: the V = 1 / ``_full`` variant of the reference's GPU engine (``engines/gpu/driver.py:435-520`` with
 ``shaders/common.metal:113-208, 563-601``), the physics of ``exchange.py`` and ``fields.py`` (restated in
-``blochparty_amd/exchange.py`` and ``blochparty_amd/fields.py``). Forward only here: gradients through a bound pool
-or concomitant fields raise ``NotImplementedError`` (capabilities). The case keeps the plain records (no runs, merged
+``blochparty_amd/exchange.py`` and ``blochparty_amd/fields.py``); gradients through ``hb_adjoint_full``
+(``engines/full_adjoint.py``). The case keeps the plain records (no runs, merged
 runs, reused pulses or NUFFT runs), the sensitivities in rows (2 C, n_pad) and, as present, the bound pool's per-spin
 parameters (f, k, R1b, T2b, dw), its Mzb with a low part, every segment's RF frequency and the super-Lorentzian's
 ln F table, and the spins' concomitant coefficients (7, n_pad); the signal goes through ``hb_signal_reduce`` as the
@@ -38,6 +38,9 @@ def bind(lib) -> None:
     lib.hb_forward_mt.restype = ctypes.c_int
     lib.hb_forward_mt.argtypes = [ctypes.POINTER(hip.HbCase), ctypes.POINTER(HbMt), ctypes.c_int32, ctypes.c_int32,
                                   ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]
+    lib.hb_adjoint_full.restype = ctypes.c_int
+    lib.hb_adjoint_full.argtypes = [ctypes.POINTER(hip.HbCase), ctypes.POINTER(HbMt), ctypes.c_void_p, ctypes.c_int32,
+                                    ctypes.c_int32, ctypes.c_void_p]
 
 
 def check_pool(bound: X.BoundPool) -> None:

@@ -119,6 +119,8 @@ def rf_frequency(seq) -> Tensor:
     resonant with spins at offset f): minus the phase step per unit time to the RF neighbours, the mean of the two
     steps where both neighbours carry RF, 0 with none. ``seq.rf_freq`` when it is set."""
     if seq.rf_freq is not None:
+        if seq.rf_freq.shape != seq.dt.shape:
+            raise ValueError(f"Sequence.rf_freq has shape {tuple(seq.rf_freq.shape)}, expected {tuple(seq.dt.shape)}")
         return seq.rf_freq
     rf, dt = seq.rf, seq.dt
     live = torch.tensor(seq.is_rf, device=dt.device) & ((rf[:, 0] ** 2 + rf[:, 1] ** 2) > 0)

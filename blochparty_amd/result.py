@@ -11,8 +11,8 @@ from torch import Tensor
 class SimResult:
     """``signal``: (C, n_adc) complex k-space in ADC order; ``m``: (N, 3) magnetization after the last segment.
 
-    ``m_bound`` / ``m_pools`` keep the reference's field names; the GPU engine here never fills them (bound and
-    free exchanging pools are refused by its capability check). ``surrogate`` (not in the reference): with
+    ``m_bound``: (N,) the bound pool's Mz after the last segment (with ``Phantom.bound``); ``m_pools`` keeps the
+    reference's field name (free exchanging pools are refused by the capability check). ``surrogate`` (not in the reference): with
     ``pulses="surrogate"``, the judge's report of the run (largest map difference against the exact composition on a
     sample of spins, spins outside the trained box, whether the exact maps replaced the network)."""
 

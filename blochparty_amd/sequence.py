@@ -28,8 +28,8 @@ class Sequence:
     spoil: list[bool]  # ideal crusher: transverse magnetization zeroed at the end of the segment
     meta: dict = field(default_factory=dict)
     b0_offset: Tensor | None = None  # (S,) uniform field added to every spin's off-resonance [Hz]; None = zeros
-    rf_freq: Tensor | None = None  # (S,) RF carrier [Hz] (bound-pool saturation; kept for the interface)
-    bound_only: list[bool] | None = None  # RF felt by the bound pool only (refused by this engine)
+    rf_freq: Tensor | None = None  # (S,) RF frequency [Hz] of the bound pool's saturation; None: exchange.rf_frequency
+    bound_only: list[bool] | None = None  # RF felt by the bound pool only (the water ignores it)
 
     def __len__(self) -> int:
         return int(self.dt.shape[0])

@@ -57,6 +57,37 @@ typedef struct hb_mt {
 int hb_forward_mt(const hb_case* cs, const hb_mt* mt, int32_t s0, int32_t s1, float* partial, int32_t rows,
                   int32_t row0, void* stream);
 
+/* The full variant's reverse sweep of one checkpoint interval [s0, s1) of plain records (replaces the reference's V = 1
+ * replay and adjoint, shaders/adjoint_common.metal:66-203, per interval of engines/gpu/adjoint.py:179-220; last
+ * interval first). One lane per spin, float64: the interval replayed from its checkpoint with the state before every
+ * segment kept in traj, then reversed segment by segment.
+ * ckpt: (6, n_pad) the forward's state at s0 (hb_case.state's layout); ckpt_mb: (2, n_pad) Mzb and its low part there
+ * (with a bound pool; else unused); traj: (s1 - s0, 4, n_pad) float32 scratch; lam: (4, n_pad) dL/d(Mx, My, Mz, Mzb)
+ * after s1 in, before s0 out; gs: (HB_FULL_GRADS, n_pad) float64 per-spin gradients, added to, rows px py pz (encoding
+ * positions), db0, pd, R1, R2, b1, the 7 concomitant coefficients (hb_mt.conc's rows), then the bound pool's f, k,
+ * R1b, T2b, dw (the reference's N_SPIN_GRADS rows, adjoint.py:42); gc: (2 nc, n_pad) float64 coil gradients (Re, Im),
+ * added to (with coil_grads); gseg: (n_pad / 256, k_max, 8) each workgroup's sums per segment of dL/d(dt, rf_x, rf_y,
+ * g_x, g_y, g_z, b0_offset, rf_freq) (with seg_grads; every row of the interval written), for hb_seg_reduce;
+ * h: (n_adc, nc, 2) dL/dsignal turned by exp(+i psi); sens: (2 nc, n_pad) Re S_0, Im S_0, ... (nc <= 16; more coils
+ * take one sweep per tile of 16, the sweep being linear in its sources). */
+typedef struct hb_full_adjoint {
+    const float* ckpt;
+    const float* ckpt_mb;
+    float* traj;
+    float* lam;
+    double* gs;
+    double* gc;
+    float* gseg;
+    const float* h;
+    const float* sens;
+    int32_t nc, k_max, coil_grads, seg_grads;
+} hb_full_adjoint;
+
+#define HB_FULL_GRADS 20
+
+int hb_adjoint_full(const hb_case* cs, const hb_mt* mt, const hb_full_adjoint* ad, int32_t s0, int32_t s1,
+                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif

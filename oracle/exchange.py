@@ -165,12 +165,16 @@ def simulate_bound(phantom, seq, coils=None, *, m_init=None, rf_freq=None, dtype
 
 
 def _rf_frequency(is_rf, rf, dt):
-    """Minus the phase step per unit time to each RF neighbour, averaged over the neighbours that carry RF."""
-    z = (rf[:, 0] + 1j * rf[:, 1]).numpy()
-    d = dt.numpy()
-    live = np.asarray(is_rf, dtype=bool) & (np.abs(z) > 0)
+    """Minus the phase step per unit time to each RF neighbour, averaged over the neighbours that carry RF
+    (exchange.py:182-210), differentiable in rf and dt: the step between s and s + 1 is
+    -atan2(Im, Re)(z_{s+1} conj z_s) / (pi (dt_s + dt_{s+1})), with safe arguments where a neighbour has no RF."""
+    live = torch.as_tensor(np.asarray(is_rf, dtype=bool)) & ((rf[:, 0] ** 2 + rf[:, 1] ** 2) > 0)
     both = live[1:] & live[:-1]
-    step = np.where(both, -np.angle(z[1:] * np.conj(z[:-1])) / (math.pi * (d[1:] + d[:-1])), 0.0)
-    total = np.concatenate([[0.0], step]) + np.concatenate([step, [0.0]])
-    count = np.concatenate([[0.0], both]) + np.concatenate([both, [0.0]])
-    return torch.as_tensor(total / np.maximum(count, 1), dtype=dt.dtype)
+    re = rf[1:, 0] * rf[:-1, 0] + rf[1:, 1] * rf[:-1, 1]
+    im = rf[1:, 1] * rf[:-1, 0] - rf[1:, 0] * rf[:-1, 1]
+    re, im = torch.where(both, re, torch.ones_like(re)), torch.where(both, im, torch.zeros_like(im))
+    step = torch.where(both, -torch.atan2(im, re) / (math.pi * (dt[1:] + dt[:-1])), torch.zeros_like(re))
+    zero = dt.new_zeros(1)
+    total = torch.cat([zero, step]) + torch.cat([step, zero])
+    count = torch.cat([zero, both.to(dt.dtype)]) + torch.cat([both.to(dt.dtype), zero])
+    return total / count.clamp_min(1)

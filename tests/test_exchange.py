@@ -107,3 +107,40 @@ def test_no_pool_is_the_plain_path():
     s, m = E.simulate_bound(ph0, seq, coils)
     s_ref, m_ref = core.simulate(ph, seq, coils)
     assert (s - s_ref).abs().max() < 1e-12 * s_ref.abs().max() and (m[:, :3] - m_ref).abs().max() < 1e-12
+
+
+def test_rf_frequency_closed_form_and_gradient():
+    """A pulse turning as A exp(-i 2 pi f t) on a dt raster has frequency f at every RF segment (neighbours on one
+    side at its ends), 0 where no neighbour carries RF; the oracle's rule (atan2) equals the package's (the angle of
+    the complex product); its autograd gradient in rf and dt equals central differences."""
+    f, dt, k = 730.0, 10e-6, 12
+    t = torch.arange(k, dtype=f64) * dt
+    amp = 200.0 * (1.5 + torch.sin(torch.arange(k, dtype=f64)))
+    rf = torch.zeros(k + 3, 2, dtype=f64)
+    rf[:k, 0], rf[:k, 1] = amp * torch.cos(2 * math.pi * f * t), -amp * torch.sin(2 * math.pi * f * t)
+    rf[k + 1] = torch.tensor([50.0, 10.0], dtype=f64)  # an isolated RF segment
+    is_rf = [True] * k + [False, True, False]
+    dts = torch.full((k + 3,), dt, dtype=f64)
+    got = E._rf_frequency(is_rf, rf, dts)
+    assert torch.allclose(got[:k], torch.full((k,), f, dtype=f64), rtol=1e-12)
+    assert float(got[k:].abs().max()) == 0.0
+    seq = bp.Sequence(dts, rf, torch.zeros(k + 3, 3, dtype=f64), torch.zeros(k + 3, dtype=f64), is_rf,
+                      [False] * (k + 3), [False] * (k + 3))
+    assert torch.allclose(X.rf_frequency(seq), got, rtol=1e-12, atol=1e-9)
+    # gradients: L = w . f_rf against central differences in rf and dt
+    w = torch.linspace(-1.0, 2.0, k + 3, dtype=f64)
+    rf_l, dt_l = rf.clone().requires_grad_(), dts.clone().requires_grad_()
+    g_rf, g_dt = torch.autograd.grad((w * E._rf_frequency(is_rf, rf_l, dt_l)).sum(), [rf_l, dt_l])
+    for idx in [(3, 0), (3, 1), (k - 1, 1), (k + 1, 0)]:
+        h = 1e-3
+        p, m = rf.clone(), rf.clone()
+        p[idx] += h
+        m[idx] -= h
+        fd = float((w * (E._rf_frequency(is_rf, p, dts) - E._rf_frequency(is_rf, m, dts))).sum()) / (2 * h)
+        assert abs(fd - float(g_rf[idx])) < 1e-6 * max(1.0, abs(fd)), (idx, fd, float(g_rf[idx]))
+    h = 1e-12
+    p, m = dts.clone(), dts.clone()
+    p[4] += h
+    m[4] -= h
+    fd = float((w * (E._rf_frequency(is_rf, rf, p) - E._rf_frequency(is_rf, rf, m))).sum()) / (2 * h)
+    assert abs(fd - float(g_dt[4])) < 1e-5 * abs(fd), (fd, float(g_dt[4]))

@@ -176,7 +176,7 @@ def test_oracle_simulate_bipolar_phase_is_bernsteins():
 
 
 def test_capabilities_of_the_field_model():
-    """Concomitant fields: forward only; a nonlinearity alone: forward and gradients."""
+    """Concomitant fields and a nonlinearity: forward and gradients; a generalized bound pool: refused."""
     from blochparty_amd.engines.capabilities import check, features
     from blochparty_amd.phantom import Phantom
 
@@ -187,8 +187,9 @@ def test_capabilities_of_the_field_model():
     used = features(ph, seq, None, F.Scanner(field_strength=1.5), None)
     assert "scanner" in used
     check("hip", used, grad=False)
-    with pytest.raises(NotImplementedError, match="concomitant"):
-        check("hip", used, grad=True)
+    check("hip", used, grad=True)
+    with pytest.raises(NotImplementedError, match="generalized"):
+        check("hip", used | {"bound", "generalized"}, grad=True)
     nl = F.GradientNonlinearity(torch.zeros(3, 2, 2, dtype=f64), torch.zeros(3, 2, 2, dtype=f64), 0.2)
     used = features(ph, seq, None, F.Scanner(nonlinearity=nl), None)
     assert used == {"nonlinearity"}

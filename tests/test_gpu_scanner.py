@@ -58,8 +58,9 @@ def test_concomitant_fields_with_a_bound_pool(gpu):
     ph = replace(ph, bound=pool)
     out = bp.simulate(ph, seq, coils, engine="gpu", scanner=F.Scanner(field_strength=0.35))
     ref_s, ref_m = E.simulate_bound(ph, seq, coils, field=OF.Field(ph.pos.double(), F.GAMMA_BAR * 0.35, 0.5, 1))
-    assert out.m.shape == (N, 4)
-    e_s, e_m = rel(out.signal.to(torch.complex128), ref_s), rel(out.m.double(), ref_m)
+    assert out.m.shape == (N, 3) and out.m_bound.shape == (N,)
+    m4 = torch.cat([out.m, out.m_bound[:, None]], 1).double()
+    e_s, e_m = rel(out.signal.to(torch.complex128), ref_s), rel(m4, ref_m)
     print(f"concomitant + bound pool vs oracle: signal {e_s:.1e}, m {e_m:.1e}", flush=True)
     assert e_s < 1e-5 and e_m < 1e-5, (e_s, e_m)
 
@@ -117,12 +118,12 @@ def test_gradient_nonlinearity_forward_and_coefficient_gradient(gpu):
 
 
 def test_field_model_refusals():
-    """Concomitant fields run forward only, on the plain stepped path: gradients and reused pulses raise
-    NotImplementedError."""
+    """Concomitant fields run on the plain stepped path: reused pulses and NUFFT readouts raise NotImplementedError,
+    forward and with gradients."""
     ph, seq, coils = field_case(n=4)
     scanner = F.Scanner(field_strength=1.5)
     t1 = ph.t1.clone().requires_grad_()
     with pytest.raises(NotImplementedError):
-        bp.simulate(replace(ph, t1=t1), seq, coils, engine="gpu", scanner=scanner)
+        bp.simulate(replace(ph, t1=t1), seq, coils, engine="gpu", scanner=scanner, pulses="surrogate")
     with pytest.raises(NotImplementedError):
         bp.simulate(ph, seq, coils, engine="gpu", scanner=scanner, pulses="reuse")
