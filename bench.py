@@ -27,6 +27,9 @@ min(W, 1) warmup steps, the caps being --leg-steps and --adjoint-steps, and each
   to the exact-RF leg is ``fwd_adjoint_speedup_vs_exact_rf``). Each reports dL/dflip, which should agree;
 - the multi-slice workload of the learned surrogate (5 slices x 256^2 x 8 isochromats, 3 ms sinc at 30 deg, B1
   0.7-1.3): forward with the surrogate's maps and forward stepping every RF substep, same readouts (``multislice``);
+- ``full_variant``: the full variant (a super-Lorentzian bound pool and a 0.55 T scanner's concomitant fields,
+  workloads.mt_gre2d: 65,536 spins, 16,384 samples, 8 coils) forward, and forward + adjoint of L = sum |signal|^2
+  w.r.t. the flip angle, the T1 map, the bound-pool fraction and the coil maps (float64 reverse sweep);
 - ``end_to_end``: simulate() timed from the caller's Phantom / Sequence / coils to the CPU result, the way the
   reference's harness times it (benchmarks/bench.py:47-95; sequence and phantom built outside the timed region):
   the value's configuration forward, and forward + .backward() with the flip angle, the T1 map and the coils
@@ -288,6 +291,72 @@ def multislice_leg(W, driver, sur, lib, steps: int, warmup: int, nufft) -> dict:
     return out
 
 
+def full_variant_leg(W, driver, lib, steps: int, warmup: int, adj_steps: int, adj_warmup: int) -> dict:
+    """The full variant on workloads.mt_gre2d: hb_forward_mt forward (every launch bracketed by events on its
+    stream), then forward with checkpoints + hb_adjoint_full of L = sum |signal|^2 (dL/dflip, dL/dT1 map, dL/df map,
+    coil gradients)."""
+    from blochparty_amd import fields as F
+    from blochparty_amd.engines import adjoint
+    from blochparty_amd.engines import full_adjoint as FA
+
+    ph, seq, coils, scanner = W.mt_gre2d()
+    fl = F.spin_fields(scanner, ph.pos)
+    conc = (fl.conc, fl.mode)
+    S, n_adc = len(seq), seq.n_adc
+    work = len(ph) * n_adc
+    out = {"workload": "2D slice-selective spoiled GRE 128^2 x 4 isochromats through 3 mm, super-Lorentzian bound "
+                       "pool by tissue class, 0.55 T leading-order concomitant fields, 8 coils, 2 ms sinc (100 "
+                       "substeps) of 20 deg, TR 10 ms", "spins": len(ph), "segments": S, "n_adc": n_adc}
+    case = driver.prepare_case(ph, seq, coils, conc=conc)
+    events = []
+
+    def timer(_phase):
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        events.append(ev)
+
+    for _ in range(warmup):
+        driver.run_forward(case, lib)
+    t = timed(lambda: driver.run_forward(case, lib, timer=timer), steps, 0)
+    kernel_ms = [events[2 * i].elapsed_time(events[2 * i + 1]) for i in range(len(events) // 2)]
+    out["forward"] = {"value": work * steps / t, "ms_per_step": 1e3 * t / steps, "steps": steps, "warmup": warmup,
+                      "kernel": "forward_mt_kernel", "kernel_mean_ms": float(np.mean(kernel_ms)),
+                      "launches_per_step": len(kernel_ms) // steps}
+    log(f"full variant forward: {out['forward']['ms_per_step']:.1f} ms/step, {out['forward']['value']:.3e} "
+        f"spin-samples/s")
+    del case
+    every = adjoint.default_checkpoint_every(S)
+    case_a = driver.prepare_case(ph, seq, coils, chunk=every, aligned=True, conc=conc)
+    dev = case_a.spins.device
+    nc = adjoint.adjoint_width(case_a.n_coils)
+    sens = adjoint.adjoint_sens(coils, case_a.n_pad, nc).to(dev)
+    flip_shape = seq.rf.double().to(dev) / 20.0
+    res = {}
+
+    def step():
+        ckpts = driver.run_forward(case_a, lib, keep=True)
+        sig = case_a.sig[:n_adc]
+        gr, gi = 2 * sig[..., 0], 2 * sig[..., 1]
+        cp, sp = torch.cos(case_a.psi_rows)[:, None], torch.sin(case_a.psi_rows)[:, None]
+        h = torch.zeros(n_adc, nc, 2, dtype=torch.float32, device=dev)
+        h[:, : case_a.n_coils, 0] = (gr * cp - gi * sp).float()
+        h[:, : case_a.n_coils, 1] = (gr * sp + gi * cp).float()
+        lam = torch.zeros(4, case_a.n_pad, dtype=torch.float32, device=dev)
+        gs, gc, segg, _ = FA.run_backward(case_a, ckpts, every, h, lam, sens, nc, True, True)
+        res["flip"] = (segg.view(S, 8)[:, 1:3] * flip_shape).sum()
+        res["t1"], res["f"], res["coils"] = gs[5], gs[15], gc
+
+    t_a = timed(step, adj_steps, adj_warmup)
+    assert all(torch.isfinite(v).all() for v in res.values())
+    out["forward_adjoint"] = {"value": work * adj_steps / t_a, "ms_per_step": 1e3 * t_a / adj_steps,
+                              "steps": adj_steps, "warmup": adj_warmup, "checkpoint_every": every,
+                              "loss": "sum |signal|^2", "wrt": ["flip angle", "T1 map", "bound-pool f", "coil maps"],
+                              "dL_dflip_per_deg": float(res["flip"])}
+    log(f"full variant forward + adjoint: {out['forward_adjoint']['ms_per_step']:.1f} ms/step, "
+        f"{out['forward_adjoint']['value']:.3e} spin-samples/s, dL/dflip {float(res['flip']):.6e}")
+    return out
+
+
 def end_to_end(bp, W, args, ph, seq, coils, work: int, steps: int, warmup: int) -> dict:
     """simulate() timed from the caller's objects to the CPU result (host packing, upload, the GPU work, readback),
     as the reference's harness times it; the grad case adds .backward() of sum |signal|^2 with the flip angle, the
@@ -364,6 +433,9 @@ def main() -> None:
     ap.add_argument("--adjoint-pulses", choices=("surrogate", "reuse"), default="surrogate",
                     help="the pulses of the non-exact forward + adjoint legs: the learned surrogate's maps, or the exact "
                          "reused maps (no network trained: for counter runs)")
+    ap.add_argument("--no-full-variant", action="store_true", help="skip the full variant's legs")
+    ap.add_argument("--full-variant-only", action="store_true",
+                    help="only the full variant's legs (bound pool, concomitant fields), no JSON line")
     ap.add_argument("--cpu-trs", type=int, default=32)
     ap.add_argument("--cpu-spins", type=int, default=65536)
     args = ap.parse_args()
@@ -373,8 +445,9 @@ def main() -> None:
         sys.exit(2)
     if args.only_value:
         args.no_cpu = args.no_adjoint = args.no_exact = args.no_e2e = args.no_multislice = True
+        args.no_full_variant = True
     if args.adjoint_only:
-        args.no_cpu = args.no_e2e = args.no_multislice = True
+        args.no_cpu = args.no_e2e = args.no_multislice = args.no_full_variant = True
     if args.no_surrogate:
         args.no_e2e = args.no_multislice = True
         args.adjoint_pulses = "reuse"
@@ -390,6 +463,11 @@ def main() -> None:
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     hip.load()
     lib = hip.HIP.lib()
+    if args.full_variant_only:
+        fv = full_variant_leg(W, driver, lib, max(1, min(args.steps, args.leg_steps)), min(args.warmup, 1),
+                              max(1, min(args.steps, args.adjoint_steps)), min(args.warmup, 1))
+        log(json.dumps(fv))
+        return
     t_build = time.perf_counter()
     ph, seq, coils = W.gre3d(args.n, n_coils=args.coils, spins_per_voxel=(1, 2, 2) if args.iso == 4 else (1, 1, 1))
     S = len(seq)
@@ -476,6 +554,12 @@ def main() -> None:
     # -- the surrogate's own workload: multi-slice GRE, surrogate vs exact RF
     if not args.no_multislice:
         line["multislice"] = multislice_leg(W, driver, sur, lib, leg_k, leg_w, nufft)
+        emit()
+
+    # -- the full variant: a bound pool and concomitant fields, forward and forward + adjoint
+    if not args.no_full_variant:
+        line["full_variant"] = full_variant_leg(W, driver, lib, leg_k, leg_w, adj_k, adj_w)
+        torch.cuda.empty_cache()
         emit()
 
     # -- forward + adjoint: the surrogate's maps with the value's readouts first, then exact RF

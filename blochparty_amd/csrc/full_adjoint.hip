@@ -9,9 +9,11 @@
 //   2. reverse, last segment first: zero the transverse adjoint at a spoil, add an ADC sample's source
 //      sum_c h_c conj(S_c) and its coil gradients h_c conj(Mxy), then reverse the step from its state before
 //      (recomputed intermediates), its spin gradients summed in float64 registers;
-//   3. per-segment gradients (dt, rf, g, b0_offset, the RF frequency): summed over each wave by a float64 butterfly
-//      and over the workgroup's 4 waves in LDS in a fixed order, one partial per workgroup and segment, for
-//      hb_seg_reduce.
+//   3. per-segment gradients (dt, rf, g, b0_offset, the RF frequency): summed over each wave by a float64
+//      reduce-scatter and over the workgroup's 4 waves in LDS in a fixed order, one partial per workgroup and segment,
+//      for hb_seg_reduce.
+// Factors that depend only on a segment's length (decays, the unsaturated exchange) are kept while consecutive segments
+// of the kind have the same dt, in both sweeps.
 // Coil gradients sit in LDS for the launch (each lane its own slots), read from and written back to HBM once.
 #include "bloch_device.h"
 #include "full_device.h"
@@ -26,10 +28,32 @@ constexpr int kWavesF = kBlockF / 64;
 constexpr int kMaxCoilsF = 16;
 static_assert(full::N_G == HB_FULL_GRADS, "per-spin gradient rows");
 
-__device__ __forceinline__ double wave_sum_d(double v) {
+// The wave's sums of the N_S = 8 per-segment gradients in a fixed order: over lane bits 5, 4, 3 each step keeps half
+// the values (a lane with the bit set the upper half) and adds the partner's copy of them, then lane bits 2, 1, 0 by a
+// butterfly; gradient i ends on lanes 8 i .. 8 i + 7 (10 shuffles instead of 48).
+__device__ __forceinline__ double seg_scatter_sums(double (&v)[full::N_S], int lane) {
+    static_assert(full::N_S == 8, "three halving steps");
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    for (int k = 0; k < 4; ++k) {
+        const bool up = lane & 32;
+        const double send = up ? v[k] : v[k + 4], keep = up ? v[k + 4] : v[k];
+        v[k] = keep + __shfl_xor(send, 32, 64);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const bool up = lane & 16;
+        const double send = up ? v[k] : v[k + 2], keep = up ? v[k + 2] : v[k];
+        v[k] = keep + __shfl_xor(send, 16, 64);
+    }
+    {
+        const bool up = lane & 8;
+        const double send = up ? v[0] : v[1], keep = up ? v[1] : v[0];
+        v[0] = keep + __shfl_xor(send, 8, 64);
+    }
+    double r = v[0];
+#pragma unroll
+    for (int off = 4; off > 0; off >>= 1) r += __shfl_xor(r, off, 64);
+    return r;
 }
 
 __device__ __forceinline__ full::Seg seg_of(const hb_segment& r, const hb_mt& mt, int s) {
@@ -74,6 +98,7 @@ __global__ __launch_bounds__(kBlockF) void adjoint_full_kernel(hb_case cs, hb_mt
 
     // 1. replay
     {
+        full::Cache ck = full::no_cache();
         full::St m;
         m.x = (double)ad.ckpt[j] + (double)ad.ckpt[3 * np + j];
         m.y = (double)ad.ckpt[np + j] + (double)ad.ckpt[4 * np + j];
@@ -83,7 +108,7 @@ __global__ __launch_bounds__(kBlockF) void adjoint_full_kernel(hb_case cs, hb_mt
             float* tr = ad.traj + (size_t)(s - s0) * 4 * np;
             tr[j] = (float)m.x, tr[np + j] = (float)m.y, tr[2 * np + j] = (float)m.z, tr[3 * np + j] = (float)m.b;
             const hb_segment r = cs.segs[s];
-            m = full::step(md, p, seg_of(r, mt, s), m).after;
+            m = full::step(md, p, seg_of(r, mt, s), m, ck).after;
             if (r.flags & HB_SPOIL) m.x = m.y = 0.0;
         }
     }
@@ -94,12 +119,13 @@ __global__ __launch_bounds__(kBlockF) void adjoint_full_kernel(hb_case cs, hb_mt
 #pragma unroll
     for (int a = 0; a < full::N_G; ++a) g[a] = 0.0;
     int rbuf = 0;
+    full::Cache ck = full::no_cache();
     for (int s = s1 - 1; s >= s0; --s) {
         const hb_segment r = cs.segs[s];
         const full::Seg q = seg_of(r, mt, s);
         const float* tr = ad.traj + (size_t)(s - s0) * 4 * np;
         const full::St m0 = {tr[j], tr[np + j], tr[2 * np + j], tr[3 * np + j]};
-        const full::Step st = full::step(md, p, q, m0);
+        const full::Step st = full::step(md, p, q, m0, ck);
         if (r.flags & HB_SPOIL) l.x = l.y = 0.0;
         if (nc > 0 && (r.flags & HB_ADC)) {
             const float* hrow = ad.h + (size_t)r.pad[0] * 2 * nc;
@@ -120,22 +146,16 @@ __global__ __launch_bounds__(kBlockF) void adjoint_full_kernel(hb_case cs, hb_mt
         for (int a = 0; a < full::N_S; ++a) sg[a] = 0.0;
         full::step_reverse(md, p, q, m0, st, l, g, sg);
         if (SEG) {  // the workgroup's sums of this segment's gradients (uniform: every lane takes it)
-#pragma unroll
-            for (int a = 0; a < full::N_S; ++a) sg[a] = wave_sum_d(sg[a]);
-            if (lane < full::N_S) {
-                double v = sg[0];
-#pragma unroll
-                for (int a = 1; a < full::N_S; ++a) v = lane == a ? sg[a] : v;
-                red[rbuf][wave][lane] = v;
-            }
+            const double v = seg_scatter_sums(sg, lane);
+            if ((lane & 7) == 0) red[rbuf][wave][lane >> 3] = v;
             // the other buffer than the last segment's: a wave writes this one again only after every wave has
             // passed the next segment's barrier, i.e. after the reads below
             __syncthreads();
             if (tid < full::N_S) {
-                double v = red[rbuf][0][tid];
+                double t = red[rbuf][0][tid];
 #pragma unroll
-                for (int w = 1; w < kWavesF; ++w) v += red[rbuf][w][tid];
-                ad.gseg[((size_t)blockIdx.x * ad.k_max + (s - s0)) * 8 + tid] = (float)v;
+                for (int w = 1; w < kWavesF; ++w) t += red[rbuf][w][tid];
+                ad.gseg[((size_t)blockIdx.x * ad.k_max + (s - s0)) * 8 + tid] = (float)t;
             }
             rbuf ^= 1;
         }

@@ -289,7 +289,22 @@ HB_HD inline void longitudinal(const Model& md, const Params& p, const Xch& X, d
         z = p.pd + (z - p.pd) * e1;
 }
 
-HB_HD inline Step step(const Model& md, const Params& p, const Seg& q, const St& m0) {
+// Factors that depend only on a segment's length and the spin, kept while consecutive segments of the kind have the
+// same dt: the free segment's decay and unsaturated exchange, the RF segment's half-step decay.
+struct Cache {
+    double free_dt, rf_dt;  // the lengths the factors belong to (< 0: none yet)
+    double fe2, fe1, re2, re1;
+    Xch fx;
+};
+
+HB_HD inline Cache no_cache() {
+    Cache c;
+    c.free_dt = c.rf_dt = -1.0;
+    c.fe2 = c.fe1 = c.re2 = c.re1 = 0.0;
+    return c;
+}
+
+HB_HD inline Step step(const Model& md, const Params& p, const Seg& q, const St& m0, Cache& k) {
     Step s;
     s.lin = p.px * q.gx + p.py * q.gy + p.pz * q.gz + p.db0 + q.b0;
     s.cc = conc_field(md.conc, p.cc, q.gx, q.gy, q.gz, s.lin);
@@ -297,8 +312,12 @@ HB_HD inline Step step(const Model& md, const Params& p, const Seg& q, const St&
     St m = m0;
     if (q.flags & kRf) {
         const double h = 0.5 * q.dt;
-        s.e2 = exp(-h * p.r2);
-        s.e1 = md.pool ? 1.0 : exp(-h * p.r1);
+        if (k.rf_dt != q.dt) {
+            k.re2 = exp(-h * p.r2);
+            k.re1 = md.pool ? 1.0 : exp(-h * p.r1);
+            k.rf_dt = q.dt;
+        }
+        s.e2 = k.re2, s.e1 = k.re1;
         if (md.pool) {
             s.Wk = kTwoPi * kTwoPi * p.b1 * p.b1 * (q.rfx * q.rfx + q.rfy * q.rfy);
             s.G = line_g(md.line, q.frf - s.bz - p.dw, p.t2b, s.gd, s.gt);
@@ -321,15 +340,25 @@ HB_HD inline Step step(const Model& md, const Params& p, const Seg& q, const St&
         double cyc = p.px * q.mx + p.py * q.my + p.pz * q.mz + p.db0 * q.dt + q.b0a + s.cc.fc * q.dt;
         cyc -= rint(cyc);
         sincospi_d(-2.0 * cyc, &s.sn, &s.cs);
-        s.e2 = exp(-q.dt * p.r2);
-        s.e1 = md.pool ? 1.0 : exp(-q.dt * p.r1);
-        if (md.pool) s.X = xch_setup(q.dt, p.pd, p.r1, p.f, p.k, p.r1b, 0.0);
+        if (k.free_dt != q.dt) {
+            k.fe2 = exp(-q.dt * p.r2);
+            k.fe1 = md.pool ? 1.0 : exp(-q.dt * p.r1);
+            if (md.pool) k.fx = xch_setup(q.dt, p.pd, p.r1, p.f, p.k, p.r1b, 0.0);
+            k.free_dt = q.dt;
+        }
+        s.e2 = k.fe2, s.e1 = k.fe1;
+        if (md.pool) s.X = k.fx;
         s.ux = s.cs * m.x - s.sn * m.y, s.uy = s.sn * m.x + s.cs * m.y;
         m.x = s.e2 * s.ux, m.y = s.e2 * s.uy;
         longitudinal(md, p, s.X, s.e1, m.z, m.b);
     }
     s.after = m;
     return s;
+}
+
+HB_HD inline Step step(const Model& md, const Params& p, const Seg& q, const St& m0) {
+    Cache k = no_cache();
+    return step(md, p, q, m0, k);
 }
 
 // T1 recovery reversed (no bound pool): z' = pd + (z - pd) e1 over t

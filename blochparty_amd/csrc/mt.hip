@@ -17,8 +17,14 @@
 // - The exchange over t is exact: z_ss + exp(A t) (z - z_ss) for the 2 x 2 generator A of (Mz, Mzb), in increment
 //   form (exp(A t) - I)(z - z_ss) = (e^{mt} cosh(qt) - 1)(z - z_ss) + e^{mt} sinh(qt)/q (A - m I)(z - z_ss), with
 //   m +- q the eigenvalues and both factors from expm1 of m +- q, so that short steps keep their low bits.
-// - ADC: every sample's coil sums over the workgroup (wave butterflies, then the 4 waves in order in LDS) go to the
-//   partial rows that hb_signal_reduce sums in a fixed order: no atomics.
+// - Factors that depend only on the segment's length and the spin (the free segment's decay and unsaturated exchange,
+//   the RF segment's half-step decay) are kept across segments the records mark as having the previous one's dt
+//   (HB_SAME_FREE_DT, HB_SAME_RF_DT): the cache restarts with every dispatch, as the hints do.
+// - ADC: every sample's coil sums over the workgroup (over each wave, then the 4 waves in order in LDS) go to the
+//   partial rows that hb_signal_reduce sums in a fixed order: no atomics. Up to 16 coils the sensitivities sit in
+//   registers and the wave's 2 C sums are one reduce-scatter (each step halves the values a lane carries: 17
+//   shuffles for 8 coils, against 96 for a butterfly per value); more coils take a butterfly per value.
+// - The next segment record is loaded while the current one is stepped.
 #include "bloch_device.h"
 
 #include "../../include/blochparty_amd_mt.h"
@@ -107,6 +113,36 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
+
+// The wave's sums of V values (V a power of two <= 64), in a fixed order: over lane bits 5, 4, ... each step keeps
+// half the values (a lane with the bit set the upper half) and adds the partner's copy of them, then the remaining
+// low bits are summed by a butterfly. Value i ends on the lanes with lane >> (6 - log2 V) == i, in v[0].
+// (Each step is its own instantiation, so that every index into v is a compile-time constant and v stays in
+// registers without select chains.)
+template <int V, int ST>
+__device__ __forceinline__ void scatter_step(float (&v)[V], int lane) {
+    if constexpr ((V >> ST) > 1) {
+        constexpr int bit = 1 << (5 - ST), half = V >> (ST + 1);
+        const bool up = lane & bit;
+#pragma unroll
+        for (int k = 0; k < half; ++k) {
+            const float send = up ? v[k] : v[k + half], keep = up ? v[k + half] : v[k];
+            v[k] = keep + __shfl_xor(send, bit, 64);
+        }
+        scatter_step<V, ST + 1>(v, lane);
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void wave_scatter_sums(float (&v)[V], int lane) {
+    scatter_step<V, 0>(v, lane);
+#pragma unroll
+    for (int b = 5 - ilog2(V); b >= 0; --b) v[0] += __shfl_xor(v[0], 1 << b, 64);
+}
+
+// NC: the coils' register width (1, 2, 4, 8, 16; C <= NC, the rest zero), or 0 for any count read per sample.
+template <int NC>
 __global__ __launch_bounds__(kBlockM) void forward_mt_kernel(hb_case cs, hb_mt mt, int s0, int s1,
                                                              float* __restrict__ partial, int rows, int row0) {
     __shared__ float red[kWavesM][2 * kMaxCoilsM];
@@ -129,22 +165,36 @@ __global__ __launch_bounds__(kBlockM) void forward_mt_kernel(hb_case cs, hb_mt m
         for (int i = 0; i < 7; ++i) cc[i] = mt.conc[(size_t)i * np + j];
     }
     const int C = cs.n_coils, vals = 16 * cs.n_tiles;
+    float sr[NC > 0 ? NC : 1], si[NC > 0 ? NC : 1];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        sr[c] = c < C ? mt.sens[(size_t)(2 * c) * np + j] : 0.0f;
+        si[c] = c < C ? mt.sens[(size_t)(2 * c + 1) * np + j] : 0.0f;
+    }
+    XStep xf;                       // free segments: the unsaturated exchange over dt (with a bound pool)
+    float e1f = 0.0f, e2f = 0.0f;   // free segments: expm1 of -dt R1 (without a bound pool), -dt R2
+    float e1r = 0.0f, e2r = 0.0f;   // RF segments: expm1 of -dt/2 R1 (without a bound pool), -dt/2 R2
+    bool free_ok = false, rf_ok = false;
+    hb_segment nq = cs.segs[s0];
     for (int s = s0; s < s1; ++s) {
-        const hb_segment q = cs.segs[s];
+        const hb_segment q = nq;
+        if (s + 1 < s1) nq = cs.segs[s + 1];
         const int fl = q.flags;
         if (fl & HB_RF) {
             float bz = offset(p, q);
             const float fc = cmode != HB_CONC_NONE ? conc_field(cmode, cc, q, bz) : 0.0f;
             bz += fc;
-            const float e2 = expm1x(-0.5f * q.dt * p.r2);
+            if (!(rf_ok && (fl & HB_SAME_RF_DT))) {
+                e2r = expm1x(-0.5f * q.dt * p.r2);
+                if (!pool) e1r = expm1x(-0.5f * q.dt * p.r1);
+                rf_ok = true;
+            }
+            const float e2 = e2r, e1 = e1r;
             XStep x;
-            float e1 = 0.0f;
             if (pool) {
                 const float w1sq = HB_TWO_PI * HB_TWO_PI * b1sq * (q.rf_x * q.rf_x + q.rf_y * q.rf_y);
                 const float sat = w1sq * lineshape_g(mt, mt.rf_freq[s] - bz - dw, t2b);
                 x = xstep(0.5f * q.dt, p.pd, p.r1, f, k, r1b, sat);
-            } else {
-                e1 = expm1x(-0.5f * q.dt * p.r1);
             }
             m.mx += m.mx * e2, m.my += m.my * e2;
             if (pool)
@@ -168,21 +218,40 @@ __global__ __launch_bounds__(kBlockM) void forward_mt_kernel(hb_case cs, hb_mt m
             cyc -= rint(cyc);
             float sn, cs_;
             sincospi(-2.0f * (float)cyc, &sn, &cs_);
-            const float e2 = expm1x(-q.dt * p.r2);
+            if (!(free_ok && (fl & HB_SAME_FREE_DT))) {
+                e2f = expm1x(-q.dt * p.r2);
+                if (pool)
+                    xf = xstep(q.dt, p.pd, p.r1, f, k, r1b, 0.0f);
+                else
+                    e1f = expm1x(-q.dt * p.r1);
+                free_ok = true;
+            }
             const float nx = cs_ * m.mx - sn * m.my, ny = sn * m.mx + cs_ * m.my;
-            m.mx = nx + nx * e2;
-            m.my = ny + ny * e2;
+            m.mx = nx + nx * e2f;
+            m.my = ny + ny * e2f;
             if (pool)
-                exchange(xstep(q.dt, p.pd, p.r1, f, k, r1b, 0.0f), m, mb, lb);
+                exchange(xf, m, mb, lb);
             else
-                two_sum(m.mz, m.lz, (m.mz - p.pd) * expm1x(-q.dt * p.r1));
+                two_sum(m.mz, m.lz, (m.mz - p.pd) * e1f);
         }
         if (fl & HB_ADC) {  // the workgroup's coil sums of this sample (uniform branch: every lane takes it)
             const int r = q.pad[0] - row0;
-            for (int c = 0; c < C; ++c) {
-                const float sr = mt.sens[(size_t)(2 * c) * np + j], si = mt.sens[(size_t)(2 * c + 1) * np + j];
-                const float re = wave_sum(sr * m.mx - si * m.my), im = wave_sum(sr * m.my + si * m.mx);
-                if (lane == 0) red[wave][2 * c] = re, red[wave][2 * c + 1] = im;
+            if constexpr (NC > 0) {
+                float v[2 * NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    v[2 * c] = sr[c] * m.mx - si[c] * m.my;
+                    v[2 * c + 1] = sr[c] * m.my + si[c] * m.mx;
+                }
+                wave_scatter_sums<2 * NC>(v, lane);
+                constexpr int low = 6 - ilog2(2 * NC);
+                if ((lane & ((1 << low) - 1)) == 0) red[wave][lane >> low] = v[0];
+            } else {
+                for (int c = 0; c < C; ++c) {
+                    const float s_r = mt.sens[(size_t)(2 * c) * np + j], s_i = mt.sens[(size_t)(2 * c + 1) * np + j];
+                    const float re = wave_sum(s_r * m.mx - s_i * m.my), im = wave_sum(s_r * m.my + s_i * m.mx);
+                    if (lane == 0) red[wave][2 * c] = re, red[wave][2 * c + 1] = im;
+                }
             }
             __syncthreads();
             if (r >= 0 && r < rows && tid < vals) {
@@ -221,7 +290,20 @@ extern "C" int hb_forward_mt(const hb_case* cs, const hb_mt* mt, int32_t s0, int
     if (mt->pool && mt->lineshape != HB_LINE_SUPERLORENTZIAN && mt->lineshape != HB_LINE_LORENTZIAN)
         return hipErrorInvalidValue;
     if (s1 == s0) return hipSuccess;
-    hipLaunchKernelGGL(forward_mt_kernel, dim3(cs->n_pad / kBlockM), dim3(kBlockM), 0, (hipStream_t)stream, *cs,
-                       *mt, s0, s1, partial, rows, row0);
+    const dim3 grid(cs->n_pad / kBlockM), block(kBlockM);
+    hipStream_t st = (hipStream_t)stream;
+    const int C = cs->n_coils;
+    if (C <= 1)
+        hipLaunchKernelGGL(forward_mt_kernel<1>, grid, block, 0, st, *cs, *mt, s0, s1, partial, rows, row0);
+    else if (C <= 2)
+        hipLaunchKernelGGL(forward_mt_kernel<2>, grid, block, 0, st, *cs, *mt, s0, s1, partial, rows, row0);
+    else if (C <= 4)
+        hipLaunchKernelGGL(forward_mt_kernel<4>, grid, block, 0, st, *cs, *mt, s0, s1, partial, rows, row0);
+    else if (C <= 8)
+        hipLaunchKernelGGL(forward_mt_kernel<8>, grid, block, 0, st, *cs, *mt, s0, s1, partial, rows, row0);
+    else if (C <= 16)
+        hipLaunchKernelGGL(forward_mt_kernel<16>, grid, block, 0, st, *cs, *mt, s0, s1, partial, rows, row0);
+    else
+        hipLaunchKernelGGL(forward_mt_kernel<0>, grid, block, 0, st, *cs, *mt, s0, s1, partial, rows, row0);
     return hipGetLastError();
 }

@@ -11,6 +11,10 @@
   slice, B1 drawn uniformly from 0.7-1.3 per voxel, a smooth +-200 Hz off-resonance, 8 coils, a 3 ms windowed-sinc
   slice pulse (TBW 4, 300 substeps on a 10 us raster) of 30 deg shifted to each slice, TR 10 ms: the learned RF
   surrogate's workload (five distinct reused pulses, one per slice offset).
+- ``mt_gre2d``: the full variant's workload: one slice of the multi-slice phantom (n x n, 4 isochromats through its
+  3 mm), a super-Lorentzian bound pool by tissue class (f 0.15 / 0.10 / 0, k 30 Hz, T1b 1 s, T2b 12 us, dw -200 Hz),
+  a 0.55 T scanner's leading-order concomitant fields (z asymmetry 0.5), 8 coils, a spoiled GRE with a 2 ms sinc
+  slice pulse (TBW 4, 100 substeps on a 20 us raster) of 20 deg, TR 10 ms, RF spoiling, n x n samples.
 """
 
 from __future__ import annotations
@@ -117,6 +121,26 @@ def multislice_phantom(n: int = 256, slices: int = SLICES, iso: int = SLICE_ISO,
     rep = lambda v: v.repeat_interleave(iso)
     return Phantom(pos=pos.reshape(-1, 3), pd=rep(props[:, 0]) / iso, t1=rep(props[:, 1]), t2=rep(props[:, 2]),
                    db0=rep(db0), b1=rep(b1))
+
+
+MT_POOL_F = (0.15, 0.10, 0.0)  # bound-pool fraction per tissue class (the third, fluid, has none)
+
+
+def mt_gre2d(n: int = 128, n_coils: int = 8, iso: int = 4, field_strength: float = 0.55):
+    """(phantom with a bound pool, sequence, coils, scanner) of the full variant's workload (module docstring)."""
+    from dataclasses import replace
+
+    from .exchange import BoundPool
+    from .fields import Scanner
+
+    ph = multislice_phantom(n, slices=1, iso=iso)
+    t1_class = torch.tensor([c[1] for c in THREE_CLASSES], dtype=torch.float64)
+    label = (ph.t1[:, None] - t1_class[None]).abs().argmin(1)
+    pool = BoundPool(f=torch.tensor(MT_POOL_F, dtype=torch.float64)[label], k=30.0, t1=1.0, t2=12e-6, dw=-200.0)
+    seq = gre_cartesian((n, n), FOV, flip_deg=20.0, tr=10e-3, slice_thickness=SLICE_THICKNESS, rf_duration=2e-3,
+                        time_bw_product=4.0, spoiling="rf", system=System(rf_raster=20e-6))
+    return (replace(ph, bound=pool), seq, birdcage_sensitivities(ph.pos, n_coils),
+            Scanner(field_strength=field_strength))
 
 
 def multislice(n: int = 256, lines: int | None = None, n_coils: int = 8, slices: int = SLICES):
