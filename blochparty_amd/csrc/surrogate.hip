@@ -32,10 +32,10 @@
 // matters where a block's precession angle is large (spins far from the pulse's band, whose magnetization nutates to
 // and fro within a block); H and G are tabulated beside F.
 //
-// Kernels: the exact spinor products that make the training data (float64); the network's training step (a fused
-// per-sample forward + backward, per-chunk weight gradients on the matrix cores summed in a fixed order, Adam); its
-// evaluation; and the
-// per-spin maps, with spins outside the trained input box flagged for the exact composition (hb_pulse_maps' mask).
+// Kernels: the exact spinor products that make the training data (float64); the network's training step (forward,
+// backward and weight gradients of 64 samples per workgroup as matrix products on the matrix cores, one partial per
+// workgroup summed in a fixed order, Adam); its evaluation; and the per-spin maps, with spins outside the trained
+// input box flagged for the exact composition (hb_pulse_maps' mask).
 #include "bloch_device.h"
 
 namespace hb {
@@ -48,7 +48,6 @@ constexpr int kRaw = 2;  // floats per raw input: f [Hz], b [Hz]
 constexpr int kW1 = 0, kB1 = kW1 + kH * kIn, kW2 = kB1 + kH, kB2 = kW2 + kH * kH, kW3 = kB2 + kH,
               kB3 = kW3 + kOut * kH, kParams = kB3 + kOut;
 static_assert(kParams == HB_MLP_PARAMS, "parameter layout");
-constexpr int kChunk = 512;  // samples per weight-gradient partial
 
 struct Quat {
     double w, x, y, z;
@@ -237,128 +236,263 @@ __device__ __forceinline__ void features(const float* __restrict__ norm, float f
     for (int q = 1; q <= kFourier; ++q) sincospif(t * (float)q, &x[2 * q], &x[2 * q + 1]);
 }
 
-// One sample per lane: forward, the loss's output deltas d3 = 2 (out - t) / (n n_out) (t the normalised target),
-// and backpropagation to d2, d1. Stores the layer inputs (with a ones row for the biases) and the deltas as (rows, n)
-// arrays for the weight-gradient kernel: ws = [A0 (kIn + 1, n) | A1 (65, n) | A2 (65, n) | D1 (64, n) | D2 (64, n) |
-// D3 (n_out, n)].
-__global__ __launch_bounds__(256) void mlp_fwd_bwd_kernel(const float* __restrict__ params,
-                                                          const float* __restrict__ norm, const float* __restrict__ x,
-                                                          const float* __restrict__ y, int n, float* __restrict__ ws) {
-    __shared__ __align__(16) float w[kParams];
-    stage_params(w, params);
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    float* A0 = ws;
-    float* A1 = A0 + (size_t)(kIn + 1) * n;
-    float* A2 = A1 + (size_t)(kH + 1) * n;
-    float* D1 = A2 + (size_t)(kH + 1) * n;
-    float* D2 = D1 + (size_t)kH * n;
-    float* D3 = D2 + (size_t)kH * n;
-    float xi[kIn];
-    features(norm, x[kRaw * s], x[kRaw * s + 1], xi);
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// ---- the training gradient on the matrix cores ----
+// One workgroup of 4 waves per kGroup = 64 samples. Every layer is a matrix product on v_mfma_f32_16x16x4f32 with the
+// hidden units as rows and the samples as columns (lane l supplies A[l % 16][k0 + l / 16] and B[k0 + l / 16][l % 16]
+// and receives OUT[4 (l / 16) + r][l % 16], r = 0..3):
+//   phase 1, each wave on its 16 samples: Z1 = W1a X (W1a = [W1 | b1], X's row kIn the ones), A1 = tanh Z1,
+//     A2 = tanh(W2 A1 + b2), out = W3 A2 + b3, D3 = 2 (out - t) / (n n_out) (t the normalised target, 0 for samples
+//     past n), D2 = (W3^T D3) (1 - A2^2), D1 = (W2^T D2) (1 - A1^2);
+//   phase 2, the workgroup's 64 samples as the inner dimension: dW1a = D1 X^T, dW2 = D2 A1^T, dW3 = D3 A2^T (32 tiles of
+//     16 x 16, 8 per wave), the biases b2, b3 as in-order row sums of D2, D3;
+// one partial per workgroup, summed over the workgroups in order by mlp_adam_kernel.
+// LDS: the parameters with padded rows and the activations / deltas as [row][sample], rows kLd = 68 floats apart
+// (68 = 4 mod 64 banks: phase 2's reads of 16 rows x 4 consecutive samples hit 64 distinct banks).
+constexpr int kGroup = 64;              // samples per workgroup
+constexpr int kLd = 68;                 // LDS row stride (floats)
+constexpr int kInP = 16, kOutP = 48;    // inputs + ones, outputs, padded to whole tiles
+constexpr int kW1s = 12;                // W1a row: W1 (kIn), b1, 0
+constexpr int L_W1 = 0, L_W2 = L_W1 + kH * kW1s, L_B2 = L_W2 + kH * kLd, L_W3 = L_B2 + kH, L_B3 = L_W3 + kOutP * kLd,
+              L_PARAMS = L_B3 + kOutP;
+static_assert(kIn + 2 <= kW1s, "W1a row");
+
+// The parameters into the padded LDS layout: every thread's loads (float4, 8 per thread) are issued before its first
+// store, then each value goes to its row and column; the padding the kernel reads (W1a's last column, W3's and b3's
+// rows past n_out) is zeroed.
+__device__ __forceinline__ int padded_index(int i) {
+    if (i < kB1) return L_W1 + (i / kIn) * kW1s + i % kIn;
+    if (i < kW2) return L_W1 + (i - kB1) * kW1s + kIn;
+    if (i < kB2) return L_W2 + ((i - kW2) / kH) * kLd + (i - kW2) % kH;
+    if (i < kW3) return L_B2 + i - kB2;
+    if (i < kB3) return L_W3 + ((i - kW3) / kH) * kLd + (i - kW3) % kH;
+    return L_B3 + i - kB3;
+}
+
+static_assert(kParams % 4 == 0, "float4 staging");
+
+__device__ __forceinline__ void stage_params_padded(float* __restrict__ lw, const float* __restrict__ p) {
+    constexpr int kVec = kParams / 4, kPer = (kVec + 255) / 256;
+    const float4* p4 = reinterpret_cast<const float4*>(p);
+    float4 v[kPer];
 #pragma unroll
-    for (int c = 0; c < kIn; ++c) A0[(size_t)c * n + s] = xi[c];
-    A0[(size_t)kIn * n + s] = 1.0f;
-    float a1[kH], a2[kH], g2[kH];
-    layer1(w, xi, a1);
-    layer2(w, a1, a2);
-#pragma unroll
-    for (int j = 0; j < kH; ++j) {
-        A1[(size_t)j * n + s] = a1[j];
-        A2[(size_t)j * n + s] = a2[j];
-        g2[j] = 0.0f;
+    for (int j = 0; j < kPer; ++j) {
+        const int q = threadIdx.x + 256 * j;
+        v[j] = q < kVec ? p4[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    A1[(size_t)kH * n + s] = 1.0f;
-    A2[(size_t)kH * n + s] = 1.0f;
+    for (int i = threadIdx.x; i < kH; i += blockDim.x) lw[L_W1 + i * kW1s + kIn + 1] = 0.0f;
+    for (int i = threadIdx.x; i < (kOutP - kOut) * kLd; i += blockDim.x) lw[L_W3 + kOut * kLd + i] = 0.0f;
+    for (int i = threadIdx.x; i < kOutP - kOut; i += blockDim.x) lw[L_B3 + kOut + i] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int q = threadIdx.x + 256 * j;
+        if (q < kVec) {
+            lw[padded_index(4 * q)] = v[j].x;
+            lw[padded_index(4 * q + 1)] = v[j].y;
+            lw[padded_index(4 * q + 2)] = v[j].z;
+            lw[padded_index(4 * q + 3)] = v[j].w;
+        }
+    }
+}
+
+__device__ __forceinline__ void mfma(f4& acc, float a, float b) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(256) void mlp_grad_mfma_kernel(const float* __restrict__ params,
+                                                            const float* __restrict__ norm,
+                                                            const float* __restrict__ x, const float* __restrict__ y,
+                                                            int n, float* __restrict__ partial) {
+    __shared__ __align__(16) float lw[L_PARAMS];
+    __shared__ float X[kInP * kLd], A1[kH * kLd], A2[kH * kLd], D1[kH * kLd], D2[kH * kLd], D3[kOutP * kLd];
+    stage_params_padded(lw, params);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, kq = lane >> 4;
+    const int col = 16 * wave + m;  // this lane's sample column in phase 1 (the B operand's and the output's)
+    const int s = blockIdx.x * kGroup + col;
+    const bool valid = s < n;
+    if (kq == 0) {  // the wave's 16 samples' inputs, one lane each
+        float xi[kIn];
+        if (valid) {
+            features(norm, x[kRaw * s], x[kRaw * s + 1], xi);
+        } else {
+#pragma unroll
+            for (int c = 0; c < kIn; ++c) xi[c] = 0.0f;
+        }
+#pragma unroll
+        for (int c = 0; c < kInP; ++c) X[c * kLd + col] = c < kIn ? xi[c] : (c == kIn && valid) ? 1.0f : 0.0f;
+    }
+    __syncthreads();
+    // phase 1: forward
+    // (each layer's output tiles are independent accumulation chains, interleaved in the inner loop)
+    float a1v[4][4], a2v[4][4];
+    f4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int u = 0; u < kInP / 4; ++u) {
+        const int k = 4 * u + kq;
+        const float b = X[k * kLd + col];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) mfma(acc[t], k < kW1s ? lw[L_W1 + (16 * t + m) * kW1s + k] : 0.0f, b);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            a1v[t][r] = tanhf(acc[t][r]);
+            A1[(16 * t + 4 * kq + r) * kLd + col] = a1v[t][r];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[t][r] = lw[L_B2 + 16 * t + 4 * kq + r];
+    }
+#pragma unroll
+    for (int u = 0; u < kH / 4; ++u) {
+        const int k = 4 * u + kq;
+        const float b = A1[k * kLd + col];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) mfma(acc[t], lw[L_W2 + (16 * t + m) * kLd + k], b);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            a2v[t][r] = tanhf(acc[t][r]);
+            A2[(16 * t + 4 * kq + r) * kLd + col] = a2v[t][r];
+        }
+    }
+    __syncthreads();
     const float scale = 2.0f / ((float)n * (float)kOut);
     const float* mu = norm + kNormHead;
     const float* sd = mu + kOut;
-    for (int k = 0; k < kOut; ++k) {
-        const float t = (y[(size_t)s * kOut + k] - mu[k]) / sd[k];
-        const float d3 = scale * (layer3(w, a2, k) - t);
-        D3[(size_t)k * n + s] = d3;
-        const float4* row = reinterpret_cast<const float4*>(w + kW3 + kH * k);
 #pragma unroll
-        for (int i = 0; i < kH / 4; ++i) {
-            const float4 r = row[i];
-            g2[4 * i] += r.x * d3, g2[4 * i + 1] += r.y * d3, g2[4 * i + 2] += r.z * d3, g2[4 * i + 3] += r.w * d3;
+    for (int t = 0; t < kOutP / 16; ++t) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[t][r] = lw[L_B3 + 16 * t + 4 * kq + r];
+    }
+#pragma unroll
+    for (int u = 0; u < kH / 4; ++u) {
+        const int k = 4 * u + kq;
+        const float b = A2[k * kLd + col];
+#pragma unroll
+        for (int t = 0; t < kOutP / 16; ++t) mfma(acc[t], lw[L_W3 + (16 * t + m) * kLd + k], b);
+    }
+#pragma unroll
+    for (int t = 0; t < kOutP / 16; ++t) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int o = 16 * t + 4 * kq + r;
+            float d = 0.0f;
+            if (valid && o < kOut) d = scale * (acc[t][r] - (y[(size_t)s * kOut + o] - mu[o]) / sd[o]);
+            D3[o * kLd + col] = d;
         }
     }
+    __syncthreads();
+    // phase 1: backward
 #pragma unroll
-    for (int j = 0; j < kH; ++j) {
-        g2[j] *= 1.0f - a2[j] * a2[j];
-        D2[(size_t)j * n + s] = g2[j];
+    for (int t = 0; t < 4; ++t) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int u = 0; u < kOutP / 4; ++u) {
+        const int k = 4 * u + kq;
+        const float b = D3[k * kLd + col];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) mfma(acc[t], lw[L_W3 + k * kLd + 16 * t + m], b);
     }
 #pragma unroll
-    for (int i = 0; i < kH; ++i) {
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            D2[(16 * t + 4 * kq + r) * kLd + col] = acc[t][r] * (1.0f - a2v[t][r] * a2v[t][r]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int u = 0; u < kH / 4; ++u) {
+        const int k = 4 * u + kq;
+        const float b = D2[k * kLd + col];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) mfma(acc[t], lw[L_W2 + k * kLd + 16 * t + m], b);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            D1[(16 * t + 4 * kq + r) * kLd + col] = acc[t][r] * (1.0f - a1v[t][r] * a1v[t][r]);
+    }
+    __syncthreads();
+    // phase 2: the weight gradients over the workgroup's samples (tile i of 32: layer 1 rows 4, layer 2 4 x 4,
+    // layer 3 3 x 4), then the biases b2, b3
+    // (wave w takes tiles w, w + 4, ...: two at a time, as two interleaved accumulation chains)
+    float* out = partial + (size_t)blockIdx.x * kParams;
+    for (int i0 = wave; i0 < 32; i0 += 8) {
+        const float* Dm[2];
+        const float* Am[2];
+        int rt[2], ct[2], layer[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int i = i0 + 4 * h;
+            if (i < 4) layer[h] = 0, rt[h] = i, ct[h] = 0, Dm[h] = D1, Am[h] = X;
+            else if (i < 20) layer[h] = 1, rt[h] = (i - 4) >> 2, ct[h] = (i - 4) & 3, Dm[h] = D2, Am[h] = A1;
+            else layer[h] = 2, rt[h] = (i - 20) >> 2, ct[h] = (i - 20) & 3, Dm[h] = D3, Am[h] = A2;
+            acc[h] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+#pragma unroll
+        for (int u = 0; u < kGroup / 4; ++u) {
+            const int k = 4 * u + kq;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                mfma(acc[h], Dm[h][(16 * rt[h] + m) * kLd + k], Am[h][(16 * ct[h] + m) * kLd + k]);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * rt[h] + 4 * kq + r, c = 16 * ct[h] + m;
+                if (layer[h] == 0) {
+                    if (c < kIn) out[kW1 + row * kIn + c] = acc[h][r];
+                    else if (c == kIn) out[kB1 + row] = acc[h][r];
+                } else if (layer[h] == 1) {
+                    out[kW2 + row * kH + c] = acc[h][r];
+                } else if (row < kOut) {
+                    out[kW3 + row * kH + c] = acc[h][r];
+                }
+            }
+        }
+    }
+    if (tid < kH + kOut) {
+        const float* Dm = tid < kH ? D2 + tid * kLd : D3 + (tid - kH) * kLd;
         float acc = 0.0f;
-#pragma unroll
-        for (int j = 0; j < kH; ++j) acc += w[kW2 + kH * j + i] * g2[j];
-        D1[(size_t)i * n + s] = acc * (1.0f - a1[i] * a1[i]);
+        for (int k = 0; k < kGroup; ++k) acc += Dm[k];
+        out[tid < kH ? kB2 + tid : kB3 + tid - kH] = acc;
     }
 }
 
-typedef float f4 __attribute__((ext_vector_type(4)));
-
-// Weight gradients as matrix products on the matrix cores: for each layer, dW|db (rows x (k_in + 1)) =
-// D (rows x n) A^T (n x (k_in + 1)), A's last row the ones of the bias. One wave per 16 x 16 output tile and chunk of
-// kChunk samples, v_mfma_f32_16x16x4f32 over the chunk's samples 4 at a time (lane l supplies D[row m][s] and
-// A[col m][s] with m = l % 16, s = 4 u + l / 16; it receives D_out[4 (l / 16) + r][l % 16]); the tile's entries go to
-// partial[chunk][parameter], summed over the chunks in order by mlp_adam_kernel. Tiles: layer 1 4 x 1, layer 2
-// 4 x 5, layer 3 3 x 5 (rows and columns past the layer read as zeros).
-constexpr int kTiles = 4 * 1 + 4 * 5 + 3 * 5;
-static_assert(kIn + 1 <= 16, "layer 1's weights and bias fit one column tile");
-
-__device__ __forceinline__ int param_index(int layer, int r, int c, int k_in) {
-    const int w_off = layer == 0 ? kW1 : layer == 1 ? kW2 : kW3, b_off = layer == 0 ? kB1 : layer == 1 ? kB2 : kB3;
-    return c < k_in ? w_off + r * k_in + c : b_off + r;
-}
-
-__global__ __launch_bounds__(64) void mlp_wgrad_kernel(const float* __restrict__ ws, int n,
-                                                       float* __restrict__ partial) {
-    const int lane = threadIdx.x, t = blockIdx.x, chunk = blockIdx.y;
-    const float* A0 = ws;
-    const float* A1 = A0 + (size_t)(kIn + 1) * n;
-    const float* A2 = A1 + (size_t)(kH + 1) * n;
-    const float* D1 = A2 + (size_t)(kH + 1) * n;
-    const float* D2 = D1 + (size_t)kH * n;
-    const float* D3 = D2 + (size_t)kH * n;
-    int layer, rt, ct;
-    if (t < 4) layer = 0, rt = t, ct = 0;
-    else if (t < 24) layer = 1, rt = (t - 4) / 5, ct = (t - 4) % 5;
-    else layer = 2, rt = (t - 24) / 5, ct = (t - 24) % 5;
-    const float* D = layer == 0 ? D1 : layer == 1 ? D2 : D3;
-    const float* A = layer == 0 ? A0 : layer == 1 ? A1 : A2;
-    const int rows = layer == 2 ? kOut : kH, k_in = layer == 0 ? kIn : kH, cols = k_in + 1;
-    const int m = lane & 15, kq = lane >> 4;
-    const int row = rt * 16 + m, col = ct * 16 + m;
-    const float* dr = D + (size_t)(row < rows ? row : 0) * n;
-    const float* ar = A + (size_t)(col < cols ? col : 0) * n;
-    const int s0 = chunk * kChunk, s1 = min(n, s0 + kChunk);
-    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int s = s0 + kq; s < s0 + kChunk; s += 4) {
-        const bool ok = s < s1;
-        const float a = (ok && row < rows) ? dr[s] : 0.0f;
-        const float b = (ok && col < cols) ? ar[s] : 0.0f;
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int orow = rt * 16 + 4 * kq + r, ocol = ct * 16 + m;
-        if (orow < rows && ocol < cols) partial[(size_t)chunk * kParams + param_index(layer, orow, ocol, k_in)] = acc[r];
-    }
-}
-
-// grad = the chunks' partials summed in chunk order; then Adam (bias corrections c1 = 1/(1 - beta1^t),
+// grad = the workgroups' partials summed in a fixed order (64 parameters per workgroup: wave w sums the partials
+// c = w mod 4 in order, then the 4 waves' sums are added in order); then Adam (bias corrections c1 = 1/(1 - beta1^t),
 // c2 = 1/(1 - beta2^t) from the host) unless grad_out is set, which receives the gradient instead.
+constexpr int kAdamParams = 64;  // parameters per workgroup of mlp_adam_kernel
+
 __global__ __launch_bounds__(256) void mlp_adam_kernel(const float* __restrict__ partial, int chunks,
                                                        float* __restrict__ params, float* __restrict__ m,
                                                        float* __restrict__ v, float lr, float c1, float c2,
                                                        float* __restrict__ grad_out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kParams) return;
+    __shared__ float part[4][kAdamParams];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * kAdamParams + lane;
     float g = 0.0f;
-    for (int c = 0; c < chunks; ++c) g += partial[(size_t)c * kParams + i];
+    if (i < kParams) {
+        for (int c = wave; c < chunks; c += 4) g += partial[(size_t)c * kParams + i];
+    }
+    part[wave][lane] = g;
+    __syncthreads();
+    if (wave != 0 || i >= kParams) return;
+    g = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
     if (grad_out) {
         grad_out[i] = g;
         return;
@@ -515,13 +649,8 @@ __global__ __launch_bounds__(256) void surrogate_maps_kernel(hb_case cs, const f
 }
 
 int launch_train_grad(const hb_mlp* net, const float* x, const float* y, int32_t n, float* work, void* stream) {
-    hipStream_t st = (hipStream_t)stream;
-    const int chunks = (n + kChunk - 1) / kChunk;
-    // one wave per workgroup: a batch of 8,192 samples is 128 workgroups (the kernel runs at one wave per SIMD)
-    hipLaunchKernelGGL(mlp_fwd_bwd_kernel, dim3((n + 63) / 64), dim3(64), 0, st, net->params, net->norm, x, y, n,
-                       work);
-    float* partial = work + (size_t)(kIn + 1 + 2 * (kH + 1) + 2 * kH + kOut) * n;
-    hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(kTiles, chunks), dim3(64), 0, st, work, n, partial);
+    hipLaunchKernelGGL(mlp_grad_mfma_kernel, dim3((n + kGroup - 1) / kGroup), dim3(256), 0, (hipStream_t)stream,
+                       net->params, net->norm, x, y, n, work);
     return hipGetLastError();
 }
 
@@ -535,10 +664,15 @@ static int check_net(const hb_mlp* net) {
     return hipSuccess;
 }
 
+// The training kernel reads the parameters as float4.
+static int check_train_net(const hb_mlp* net) {
+    if (int e = check_net(net)) return e;
+    return reinterpret_cast<uintptr_t>(net->params) % 16 ? hipErrorInvalidValue : hipSuccess;
+}
+
 extern "C" int64_t hb_mlp_workspace(int32_t n) {
     if (n < 1) return 0;
-    const int64_t chunks = (n + kChunk - 1) / kChunk;
-    return (int64_t)(kIn + 1 + 2 * (kH + 1) + 2 * kH + kOut) * n + chunks * kParams;
+    return (int64_t)((n + kGroup - 1) / kGroup) * kParams;  // one gradient partial per workgroup of kGroup samples
 }
 
 extern "C" int hb_spinor_blocks(const double* table, const int32_t* bounds, const float* x, int32_t n, float* y,
@@ -552,25 +686,23 @@ extern "C" int hb_spinor_blocks(const double* table, const int32_t* bounds, cons
 
 extern "C" int hb_mlp_grad(const hb_mlp* net, const float* x, const float* y, int32_t n, float* work, float* grad,
                            void* stream) {
-    if (int e = check_net(net)) return e;
+    if (int e = check_train_net(net)) return e;
     if (!x || !y || !work || !grad || n < 1) return hipErrorInvalidValue;
     if (int e = launch_train_grad(net, x, y, n, work, stream)) return e;
-    const int chunks = (n + kChunk - 1) / kChunk;
-    float* partial = work + hb_mlp_workspace(n) - (int64_t)chunks * kParams;
-    hipLaunchKernelGGL(mlp_adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, (hipStream_t)stream, partial,
+    const int chunks = (n + kGroup - 1) / kGroup;
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3((kParams + kAdamParams - 1) / kAdamParams), dim3(256), 0, (hipStream_t)stream, work,
                        chunks, net->params, net->adam_m, net->adam_v, 0.0f, 0.0f, 0.0f, grad);
     return hipGetLastError();
 }
 
 extern "C" int hb_mlp_train_step(const hb_mlp* net, const float* x, const float* y, int32_t n, float* work,
                                  int32_t step, float lr, void* stream) {
-    if (int e = check_net(net)) return e;
+    if (int e = check_train_net(net)) return e;
     if (!net->adam_m || !net->adam_v || !x || !y || !work || n < 1 || step < 1) return hipErrorInvalidValue;
     if (int e = launch_train_grad(net, x, y, n, work, stream)) return e;
-    const int chunks = (n + kChunk - 1) / kChunk;
-    float* partial = work + hb_mlp_workspace(n) - (int64_t)chunks * kParams;
+    const int chunks = (n + kGroup - 1) / kGroup;
     const float c1 = (float)(1.0 / (1.0 - pow(0.9, step))), c2 = (float)(1.0 / (1.0 - pow(0.999, step)));
-    hipLaunchKernelGGL(mlp_adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, (hipStream_t)stream, partial,
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3((kParams + kAdamParams - 1) / kAdamParams), dim3(256), 0, (hipStream_t)stream, work,
                        chunks, net->params, net->adam_m, net->adam_v, lr, c1, c2, (float*)nullptr);
     return hipGetLastError();
 }
@@ -578,12 +710,12 @@ extern "C" int hb_mlp_train_step(const hb_mlp* net, const float* x, const float*
 extern "C" int hb_mlp_train_steps(const hb_mlp* net, const float* x, const float* y, int32_t batch, int32_t n_steps,
                                   float* work, int32_t step0, int32_t total_steps, float lr0, float lr1,
                                   void* stream) {
-    if (int e = check_net(net)) return e;
+    if (int e = check_train_net(net)) return e;
     if (!net->adam_m || !net->adam_v || !x || !y || !work || batch < 1 || n_steps < 0 || step0 < 0 ||
         total_steps < step0 + n_steps)
         return hipErrorInvalidValue;
-    const int chunks = (batch + kChunk - 1) / kChunk;
-    float* partial = work + hb_mlp_workspace(batch) - (int64_t)chunks * kParams;
+    const int chunks = (batch + kGroup - 1) / kGroup;
+    float* partial = work;
     for (int i = 0; i < n_steps; ++i) {
         const int t = step0 + i + 1;
         const double lr = lr1 + 0.5 * (lr0 - lr1) * (1.0 + cos(M_PI * (t - 1) / (double)total_steps));
@@ -591,7 +723,7 @@ extern "C" int hb_mlp_train_steps(const hb_mlp* net, const float* x, const float
         if (int e = launch_train_grad(net, x + (size_t)i * batch * kRaw, y + (size_t)i * batch * kOut, batch, work,
                                       stream))
             return e;
-        hipLaunchKernelGGL(mlp_adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, (hipStream_t)stream, partial,
+        hipLaunchKernelGGL(mlp_adam_kernel, dim3((kParams + kAdamParams - 1) / kAdamParams), dim3(256), 0, (hipStream_t)stream, partial,
                            chunks, net->params, net->adam_m, net->adam_v, (float)lr, c1, c2, (float*)nullptr);
     }
     return hipGetLastError();

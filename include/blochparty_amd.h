@@ -208,7 +208,7 @@ int hb_pulse_map_dlnb1(const hb_case* cs, const double* table, const int32_t* in
 #define HB_MLP_FIRST_GRID 2049
 
 /* A network. params: (HB_MLP_PARAMS,) W1 (64, HB_MLP_INPUTS), b1 (64), W2 (64, 64), b2 (64), W3 (n_out, 64),
- * b3 (n_out); adam_m, adam_v: Adam's moments (training only); norm: (HB_MLP_NORM_HEAD + 2 n_out,) c_f, h_f, c_b,
+ * b3 (n_out), 16-byte aligned for the training entry points (hb_mlp_grad, hb_mlp_train_step(s)); adam_m, adam_v: Adam's moments (training only); norm: (HB_MLP_NORM_HEAD + 2 n_out,) c_f, h_f, c_b,
  * h_b (the input box's centres and half-widths), S [s] (the features' base time), s_b (an input b times s_b is b in
  * Hz: 1 for the training inputs, the pulse's peak for a spin's B1 scale), then the outputs' means (n_out) and scales
  * (n_out): the network is trained on (y - mean) / scale; first: (HB_SUR_BLOCKS, n_grid, 8) the blocks' first-order
@@ -252,7 +252,8 @@ int hb_sur_first_quats(const hb_mlp* net, const float* x, int32_t n, float* q, v
 int64_t hb_mlp_workspace(int32_t n);
 
 /* grad (HB_MLP_PARAMS,): the gradient of the mean squared error between the network's outputs on the features of
- * x (n, 2) and (y - mean) / scale, over the n samples and n_out outputs; per-chunk sums added in a fixed order. */
+ * x (n, 2) and (y - mean) / scale, over the n samples and n_out outputs; the sums of 64-sample groups added in a fixed
+ * order. */
 int hb_mlp_grad(const hb_mlp* net, const float* x, const float* y, int32_t n, float* work, float* grad, void* stream);
 
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected for step = 1, 2, ...) on that gradient. */
