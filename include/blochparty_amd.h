@@ -48,6 +48,7 @@ extern "C" {
                              covers it): the identity, in tables with checkpoints (the adjoint's); pad[3] the
                              occurrence's first segment                                                       */
 #define HB_NUFFT_MAX_TERMS 8
+#define HB_NUFFT_WMAX 32 /* widest window of grid cells one workgroup of 256 spins may span (engines/nufft.py WMAX) */
 
 /* One segment of the event list, 128 bytes (the reference's segs row + flags + rows, driver.py:357-384). psi is the
  * receiver phase wrapped to [0, 2 pi) in float64 before the cast (driver.py:357). The float64 block carries the
