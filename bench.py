@@ -30,6 +30,9 @@ min(W, 1) warmup steps, the caps being --leg-steps and --adjoint-steps, and each
 - ``full_variant``: the full variant (a super-Lorentzian bound pool and a 0.55 T scanner's concomitant fields,
   workloads.mt_gre2d: 65,536 spins, 16,384 samples, 8 coils) forward, and forward + adjoint of L = sum |signal|^2
   w.r.t. the flip angle, the T1 map, the bound-pool fraction and the coil maps (float64 reverse sweep);
+- ``downstream``: either side of the path -- the sequence builder and the segment table on the host (wall time), and
+  the reconstruction of the workload's k-space on the GPU (grid, centred 3D inverse FFT, root-sum-of-squares)
+  against numpy in float64, with its HBM roofline;
 - ``end_to_end``: simulate() timed from the caller's Phantom / Sequence / coils to the CPU result, the way the
   reference's harness times it (benchmarks/bench.py:47-95; sequence and phantom built outside the timed region):
   the value's configuration forward, and forward + .backward() with the flip angle, the T1 map and the coils
@@ -57,6 +60,7 @@ import torch
 
 METRIC = "spin·samples/s fwd & fwd+adjoint, 3D GRE 64³; RF surrogate vs exact @1e-4"
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, published
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 FLIP_DEG = 10.0  # the workload's flip angle: rf = FLIP_DEG x (the pulse's shape per degree)
 
 # counting convention of the algorithmic work (per spin)
@@ -357,6 +361,65 @@ def full_variant_leg(W, driver, lib, steps: int, warmup: int, adj_steps: int, ad
     return out
 
 
+def recon_timing(run, n_coils: int, shape: tuple, steps: int) -> dict:
+    """GPU time of ``run`` (k-space (C, n_adc) complex64 on the device -> RSS image) on seeded synthetic k-space, its
+    error against numpy's centred inverse FFT and RSS in float64, and its HBM roofline."""
+    g = torch.Generator().manual_seed(5)
+    k = torch.randn(n_coils, int(np.prod(shape)), dtype=torch.complex64, generator=g)
+    kd = k.cuda()
+    for _ in range(2):
+        run(kd)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        img = run(kd)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    t0 = time.perf_counter()
+    axes = (1, 2, 3)
+    kk = k.numpy().astype(np.complex128).reshape(n_coils, *shape)
+    ref = np.sqrt((np.abs(np.fft.fftshift(np.fft.ifftn(np.fft.ifftshift(kk, axes=axes), axes=axes), axes=axes))
+                   ** 2).sum(axis=0))
+    t_cpu = time.perf_counter() - t0
+    err = float(np.abs(img.cpu().double().numpy() - ref).max() / np.abs(ref).max())
+    floor = k.numel() * 8 + ref.size * 4
+    gbs = floor / (ms * 1e-3) / 1e9
+    log(f"recon (grid, ifft3c, rss) of {n_coils} x {shape}: {ms:.3f} ms on the GPU ({gbs:.0f} GB/s of the floor), "
+        f"{t_cpu * 1e3:.0f} ms numpy f64, rel err {err:.2e}")
+    return {"shape": [n_coils, *shape], "ms": ms, "steps": steps, "cpu_numpy_f64_ms": t_cpu * 1e3,
+            "rel_err_vs_numpy_f64": err,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": floor}}
+
+
+def downstream_leg(W, driver, n: int, n_coils: int, steps: int) -> dict:
+    """Either side of the path: building the n^3 GRE with the sequence builder and its segment table on the host
+    (their wall time), and reconstructing its k-space on the GPU (Cartesian grid, centred 3D inverse FFT,
+    root-sum-of-squares over the coils; torch's FFT, i.e. rocFFT) against numpy's in float64 on the same seeded
+    synthetic k-space. The reconstruction's HBM floor: read the complex64 k-space once, write the float32 image."""
+    from blochparty_amd import recon
+
+    out = {}
+    t0 = time.perf_counter()
+    seq = W.gre3d_sequence(n)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rec = driver.segment_table(seq, [0])
+    t_table = time.perf_counter() - t0
+    out["sequence_build"] = {"s": t_build, "segments": len(seq), "segments_per_s": len(seq) / t_build}
+    out["segment_table"] = {"s": t_table, "records": len(rec), "records_per_s": len(rec) / t_table}
+    log(f"sequence builder {t_build:.2f} s for {len(seq)} segments; segment table {t_table:.2f} s")
+
+    shape = tuple(seq.meta["kspace_shape"])
+    out["recon_rss"] = recon_timing(lambda kd: recon.rss(recon.reconstruct(kd, seq)), n_coils, shape, steps)
+    # the largest grid of the workloads (the 128^3 GRE with 16 coils), where launches no longer dominate
+    big = (128, 128, 128)
+    out["recon_rss_128"] = recon_timing(lambda kd: recon.rss(recon.ifft3c(kd.reshape(16, *big))), 16, big, steps)
+    return out
+
+
 def end_to_end(bp, W, args, ph, seq, coils, work: int, steps: int, warmup: int) -> dict:
     """simulate() timed from the caller's objects to the CPU result (host packing, upload, the GPU work, readback),
     as the reference's harness times it; the grad case adds .backward() of sum |signal|^2 with the flip angle, the
@@ -434,6 +497,8 @@ def main() -> None:
                     help="the pulses of the non-exact forward + adjoint legs: the learned surrogate's maps, or the exact "
                          "reused maps (no network trained: for counter runs)")
     ap.add_argument("--no-full-variant", action="store_true", help="skip the full variant's legs")
+    ap.add_argument("--no-downstream", action="store_true",
+                    help="skip the sequence build / segment table / reconstruction timings")
     ap.add_argument("--full-variant-only", action="store_true",
                     help="only the full variant's legs (bound pool, concomitant fields), no JSON line")
     ap.add_argument("--cpu-trs", type=int, default=32)
@@ -445,9 +510,9 @@ def main() -> None:
         sys.exit(2)
     if args.only_value:
         args.no_cpu = args.no_adjoint = args.no_exact = args.no_e2e = args.no_multislice = True
-        args.no_full_variant = True
+        args.no_full_variant = args.no_downstream = True
     if args.adjoint_only:
-        args.no_cpu = args.no_e2e = args.no_multislice = args.no_full_variant = True
+        args.no_cpu = args.no_e2e = args.no_multislice = args.no_full_variant = args.no_downstream = True
     if args.no_surrogate:
         args.no_e2e = args.no_multislice = True
         args.adjoint_pulses = "reuse"
@@ -560,6 +625,11 @@ def main() -> None:
     if not args.no_full_variant:
         line["full_variant"] = full_variant_leg(W, driver, lib, leg_k, leg_w, adj_k, adj_w)
         torch.cuda.empty_cache()
+        emit()
+
+    # -- either side of the path: the sequence builder and segment table, the reconstruction
+    if not args.no_downstream:
+        line["downstream"] = downstream_leg(W, driver, args.n, args.coils, max(3, leg_k))
         emit()
 
     # -- forward + adjoint: the surrogate's maps with the value's readouts first, then exact RF

@@ -121,6 +121,8 @@ class GPUSimulate(torch.autograd.Function):
         hip.HIP.synchronize()
         GS = to_caller(gs.cpu()[:, :n])
         lam = to_caller(lam_d.cpu().double()[:, :n])
+        driver.check_output(case, GS, lam, *(t for t in (gc, seg) if t is not None),
+                            premise=bool(torch.isfinite(h).all()) and (g_m is None or bool(torch.isfinite(g_m).all())))
         out = [None] * 13
         if need[0]:
             out[0] = GS[0:3].T.contiguous()

@@ -393,6 +393,28 @@ def check_inputs(phantom: Phantom, coils) -> None:
         raise ValueError(f"at most {hip.MAX_COILS} coils on hip, got {coils.shape[1]}")
 
 
+def finite_inputs(phantom: Phantom, seq: Sequence, coils=None, m_init=None) -> bool:
+    """Every number of the case is finite: the premise of ``read_result``'s output guard."""
+    ts = [phantom.pos, phantom.pd, phantom.t1, phantom.t2, phantom.db0, phantom.b1, seq.dt, seq.rf, seq.grad,
+          seq.adc_phase, seq.b0_offset, seq.rf_freq, coils, m_init]
+    if phantom.bound is not None:
+        ts += [getattr(phantom.bound, k) for k in ("f", "k", "t1", "t2", "dw")]
+    for t in ts:
+        if isinstance(t, torch.Tensor):
+            if not bool(torch.isfinite(t.detach()).all()):
+                return False
+        elif isinstance(t, float) and not math.isfinite(t):
+            return False
+    return True
+
+
+def check_output(case: DeviceCase, *outs: torch.Tensor, premise: bool = True) -> None:
+    """Failure detection: a non-finite result from finite inputs (the case's, and ``premise``: the incoming
+    gradients' of a reverse sweep) is an engine fault and raises; otherwise the result is returned as computed."""
+    if premise and case.extra.get("finite_inputs") and not all(bool(torch.isfinite(o).all()) for o in outs):
+        raise RuntimeError("hip engine: non-finite result from finite inputs")
+
+
 def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, chunk: int = DEFAULT_CHUNK,
                  device=None, partial_bytes: int = PARTIAL_BYTES, aligned: bool = False,
                  reuse: bool = False, surrogate: bool = False, nufft=None, conc=None) -> DeviceCase:
@@ -508,6 +530,7 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
         sig=torch.zeros(max(n_adc, 1), n_coils, 2, dtype=torch.float64, device=dev),
         **pulse_buffers(seq, reused, n_pad, dev),
     )
+    case.extra["finite_inputs"] = finite_inputs(phantom, seq, coils, m_init)
     if plan is not None:
         attach_nufft(case, plan, order, seq, adc_np, starts, ends, dev)
     if full:
@@ -629,6 +652,7 @@ def read_result(case: DeviceCase) -> SimResult:
         inv = np.empty_like(case.perm)
         inv[case.perm] = np.arange(len(case.perm))
         m = m[torch.from_numpy(inv)].contiguous()
+    check_output(case, sig, st, *(() if m_bound is None else (m_bound,)))
     res = SimResult(signal, m, m_bound)
     if case.extra.get("surrogate") is not None:
         from . import surrogate

@@ -119,6 +119,8 @@ class FullSimulate(torch.autograd.Function):
         hip.HIP.synchronize()
         GS = gs.cpu()[:, :n]
         lam = lam_d.cpu().double()[:, :n]
+        driver.check_output(case, GS, lam, *(t for t in (gc, seg) if t is not None),
+                            premise=bool(torch.isfinite(h).all()) and (g_m is None or bool(torch.isfinite(g_m).all())))
         out = [None] * 16
         if need[0]:
             out[0] = GS[0:3].T.contiguous()

@@ -217,6 +217,34 @@ def test_api_refusals_without_gpu():
     capabilities.check("hip", frozenset({"b1", "m_init"}), grad=True)
 
 
+def test_output_guard_raises_on_non_finite_results_of_finite_inputs():
+    import dataclasses
+    import types
+
+    from blochparty_amd.exchange import BoundPool
+
+    ph, seq = random_phantom(10), busy_sequence()
+    coils = torch.ones(10, 2, dtype=torch.complex128)
+    assert driver.finite_inputs(ph, seq, coils)
+    t1 = ph.t1.clone()
+    t1[3] = float("nan")
+    assert not driver.finite_inputs(dataclasses.replace(ph, t1=t1), seq, coils)
+    rf = seq.rf.clone()
+    rf[0, 1] = float("inf")
+    assert not driver.finite_inputs(ph, dataclasses.replace(seq, rf=rf), coils)
+    assert not driver.finite_inputs(ph, seq, coils * float("nan"))
+    assert not driver.finite_inputs(dataclasses.replace(ph, bound=BoundPool(0.1, float("inf"), 1.0, 1e-5)), seq)
+    assert driver.finite_inputs(dataclasses.replace(ph, bound=BoundPool(0.1, 20.0, 1.0, 1e-5)), seq)
+
+    bad, good = torch.tensor([1.0, float("nan")]), torch.ones(3)
+    case = types.SimpleNamespace(extra={"finite_inputs": True})
+    with pytest.raises(RuntimeError, match="non-finite"):
+        driver.check_output(case, good, bad)
+    driver.check_output(case, good, good)
+    driver.check_output(case, bad, premise=False)  # a non-finite incoming gradient
+    driver.check_output(types.SimpleNamespace(extra={"finite_inputs": False}), bad)
+
+
 def test_default_checkpoint_interval():
     """About sqrt(segments) in multiples of 64; 4x that when reused pulses or NUFFT runs make most segments identity
     records (the 64^3 GRE: 1,110,016 segments -> 1,088 and 4,352)."""
