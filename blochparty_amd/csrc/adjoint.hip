@@ -252,9 +252,21 @@ __device__ __forceinline__ void nufft_reverse(const hb_segment& q, const hb_adjo
 
 // NU: the table has NUFFT runs (their reverse raises the register pressure of the unrolled block loop, so the other
 // tables take the variant without it).
+// The read-only inputs come in again as noalias parameters (the same pointers as in cs_in and ad_in): no store of the
+// kernel can change what they point to, so the loads at wave-uniform addresses (the records, h rows, NUFFT tables)
+// go through the scalar cache instead of one vector load per wave and a readfirstlane per value.
 template <int NC, bool SEG, bool NU>
 __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8 ? 2 : 1))) void adjoint_kernel(
-    hb_case cs, hb_adjoint ad, int s0, int s1) {
+    hb_case cs_in, hb_adjoint ad_in, int s0, int s1, const hb_segment* __restrict__ segs,
+    const float* __restrict__ spins, const float* __restrict__ pos_lo, const float* __restrict__ maps,
+    const float* __restrict__ ckpt, const float* __restrict__ h, const float* __restrict__ sens,
+    const float* __restrict__ maps_d, const float* __restrict__ nu_params, const int32_t* __restrict__ nu_win,
+    const float* __restrict__ nu_grid) {
+    hb_case cs = cs_in;
+    cs.segs = segs, cs.spins = spins, cs.pos_lo = pos_lo, cs.maps = maps;
+    hb_adjoint ad = ad_in;
+    ad.ckpt = ckpt, ad.h = h, ad.sens = sens, ad.maps_d = maps_d;
+    ad.nu_params = nu_params, ad.nu_win = nu_win, ad.nu_grid = nu_grid;
     __shared__ float segbuf[2][kWavesA][8][8];  // alternate buffers: one barrier per workgroup sum
     __shared__ double accs[(8 + 2 * NC) * kBlockA];  // float64 accumulators: slot a of lane tid at a * kBlockA + tid
     __shared__ double seeds_l[12 * kBlockA];          // a single reused pulse's seeds, same layout
@@ -543,14 +555,10 @@ template <int NC>
 static hipError_t launch_adjoint(const hb_case* cs, const hb_adjoint* ad, int s0, int s1, hipStream_t st) {
     const dim3 grid(cs->n_pad / kBlockA), block(kBlockA);
     const bool nu = ad->nu_params != nullptr;
-    if (ad->seg_grads && nu)
-        hipLaunchKernelGGL((adjoint_kernel<NC, true, true>), grid, block, 0, st, *cs, *ad, s0, s1);
-    else if (ad->seg_grads)
-        hipLaunchKernelGGL((adjoint_kernel<NC, true, false>), grid, block, 0, st, *cs, *ad, s0, s1);
-    else if (nu)
-        hipLaunchKernelGGL((adjoint_kernel<NC, false, true>), grid, block, 0, st, *cs, *ad, s0, s1);
-    else
-        hipLaunchKernelGGL((adjoint_kernel<NC, false, false>), grid, block, 0, st, *cs, *ad, s0, s1);
+    auto kernel = ad->seg_grads ? (nu ? adjoint_kernel<NC, true, true> : adjoint_kernel<NC, true, false>)
+                                : (nu ? adjoint_kernel<NC, false, true> : adjoint_kernel<NC, false, false>);
+    hipLaunchKernelGGL(kernel, grid, block, 0, st, *cs, *ad, s0, s1, cs->segs, cs->spins, cs->pos_lo, cs->maps,
+                       ad->ckpt, ad->h, ad->sens, ad->maps_d, ad->nu_params, ad->nu_win, ad->nu_grid);
     return hipGetLastError();
 }
 

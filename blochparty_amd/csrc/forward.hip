@@ -245,9 +245,19 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
 // NT: coil tiles (0: any, the sensitivities read from memory per flush); NUT: NUFFT runs with up to NUT decay terms
 // (0: none; the terms' weights are held in registers); RF: RF segments are stepped (false when every pulse of the
 // dispatch is applied as a map: the RF step is then left out, and its registers).
+// The read-only inputs and the NUFFT partials come in again as noalias parameters (the same pointers as in cs_in and
+// nu_in): no store of the kernel can change the records, so they load through the scalar cache instead of one vector
+// load per wave and a readfirstlane per value.
 template <int NT, int NUT, bool RF>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 ? 3 : 4))) void forward_kernel(
-    hb_case cs, int s0, int s1, float* __restrict__ partial, int rows, hb_nufft nu) {
+    hb_case cs_in, int s0, int s1, float* __restrict__ partial, int rows, hb_nufft nu_in,
+    const hb_segment* __restrict__ segs, const float* __restrict__ spins, const float* __restrict__ pos_lo,
+    const float* __restrict__ sens_a, const float* __restrict__ maps, const float* __restrict__ nu_params,
+    const int32_t* __restrict__ nu_win, float* __restrict__ nu_part) {
+    hb_case cs = cs_in;
+    cs.segs = segs, cs.spins = spins, cs.pos_lo = pos_lo, cs.sens_a = sens_a, cs.maps = maps;
+    hb_nufft nu = nu_in;
+    nu.params = nu_params, nu.win = nu_win, nu.part = nu_part;
     extern __shared__ float lds[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = blockIdx.x * kBlock + tid;
@@ -422,12 +432,11 @@ namespace hb {
 template <int NUT, bool RF>
 static void launch_nt(const hb_case& cs, int s0, int s1, float* partial, int rows, const hb_nufft& nu, dim3 grid,
                       dim3 block, size_t lds, hipStream_t st) {
-    if (cs.n_tiles == 1)
-        hipLaunchKernelGGL((forward_kernel<1, NUT, RF>), grid, block, lds, st, cs, s0, s1, partial, rows, nu);
-    else if (cs.n_tiles == 2)
-        hipLaunchKernelGGL((forward_kernel<2, NUT, RF>), grid, block, lds, st, cs, s0, s1, partial, rows, nu);
-    else
-        hipLaunchKernelGGL((forward_kernel<0, NUT, RF>), grid, block, lds, st, cs, s0, s1, partial, rows, nu);
+    auto kernel = cs.n_tiles == 1   ? forward_kernel<1, NUT, RF>
+                  : cs.n_tiles == 2 ? forward_kernel<2, NUT, RF>
+                                    : forward_kernel<0, NUT, RF>;
+    hipLaunchKernelGGL(kernel, grid, block, lds, st, cs, s0, s1, partial, rows, nu, cs.segs, cs.spins, cs.pos_lo,
+                       cs.sens_a, cs.maps, nu.params, nu.win, nu.part);
 }
 
 template <int NUT>

@@ -67,9 +67,21 @@ __device__ __forceinline__ full::Seg seg_of(const hb_segment& r, const hb_mt& mt
     return q;
 }
 
+// The read-only inputs come in again as noalias parameters (the same pointers as in the structs): no store of the
+// kernel can change what they point to, so the records and the h rows load through the scalar cache.
 template <bool SEG>
-__global__ __launch_bounds__(kBlockF) void adjoint_full_kernel(hb_case cs, hb_mt mt, hb_full_adjoint ad, int s0,
-                                                               int s1) {
+__global__ __launch_bounds__(kBlockF) void adjoint_full_kernel(
+    hb_case cs_in, hb_mt mt_in, hb_full_adjoint ad_in, int s0, int s1, const hb_segment* __restrict__ segs,
+    const float* __restrict__ spins, const float* __restrict__ pos_lo, const float* __restrict__ pool_p,
+    const float* __restrict__ rf_freq, const float* __restrict__ table, const float* __restrict__ conc,
+    const float* __restrict__ ckpt, const float* __restrict__ ckpt_mb, const float* __restrict__ h,
+    const float* __restrict__ sens) {
+    hb_case cs = cs_in;
+    cs.segs = segs, cs.spins = spins, cs.pos_lo = pos_lo;
+    hb_mt mt = mt_in;
+    mt.pool = pool_p, mt.rf_freq = rf_freq, mt.table = table, mt.conc = conc;
+    hb_full_adjoint ad = ad_in;
+    ad.ckpt = ckpt, ad.ckpt_mb = ckpt_mb, ad.h = h, ad.sens = sens;
     __shared__ double gcs[2 * kMaxCoilsF * kBlockF];       // coil gradients: slot a of lane tid at a * kBlockF + tid
     __shared__ double red[2][kWavesF][full::N_S];          // alternate buffers: one barrier per segment
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -190,9 +202,8 @@ extern "C" int hb_adjoint_full(const hb_case* cs, const hb_mt* mt, const hb_full
         return hipErrorInvalidValue;
     if (s1 == s0) return hipSuccess;
     const dim3 grid(cs->n_pad / kBlockF), block(kBlockF);
-    if (ad->seg_grads)
-        hipLaunchKernelGGL(adjoint_full_kernel<true>, grid, block, 0, (hipStream_t)stream, *cs, *mt, *ad, s0, s1);
-    else
-        hipLaunchKernelGGL(adjoint_full_kernel<false>, grid, block, 0, (hipStream_t)stream, *cs, *mt, *ad, s0, s1);
+    hipLaunchKernelGGL(ad->seg_grads ? adjoint_full_kernel<true> : adjoint_full_kernel<false>, grid, block, 0,
+                       (hipStream_t)stream, *cs, *mt, *ad, s0, s1, cs->segs, cs->spins, cs->pos_lo, mt->pool,
+                       mt->rf_freq, mt->table, mt->conc, ad->ckpt, ad->ckpt_mb, ad->h, ad->sens);
     return hipGetLastError();
 }
