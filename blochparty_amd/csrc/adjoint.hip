@@ -107,27 +107,34 @@ __device__ __forceinline__ int live_flags(int f, bool free_ok, bool rf_ok, int s
 // parameter gradients, in its frame (t = Rz(-psi) lambda_after, u = Rz(-psi) m_before): t u^T on D's entries and t
 // on v's, added in float64. With SEG: sg[G_RX] = t^T (Dd u + vd) (the map's derivative along the pulse's amplitude,
 // maps_d), sg[G_RY] = dL/dpsi = (lambda_after x m_after)_z - (lambda_before x m_before)_z (the frame's turn).
-// sl: with a single reused pulse, its seeds are held in LDS for the launch (slot a of lane tid at a * kBlockA + tid);
-// else NULL and they are added in HBM.
-template <bool SEG>
-__device__ __forceinline__ void pulse_reverse(const hb_segment& q, const hb_case& cs, const hb_adjoint& ad, int j,
-                                              const Stash& mb, const StashXY& ma, float& lx, float& ly, float& lz,
-                                              float (&sg)[N_SEG_GRADS], double* sl) {
-    const int np = cs.n_pad;
-    const float c = q.w_rf_x, sn = q.w_rf_y;
-    const size_t base = (size_t)q.pad[1] * 12 * np + j;
-    const float* U = cs.maps + base;
-    const float tx = c * lx + sn * ly, ty = c * ly - sn * lx, tz = lz;
-    const float ux = c * mb.x + sn * mb.y, uy = c * mb.y - sn * mb.x, uz = mb.z;
-    double* sd = sl ? sl + threadIdx.x : ad.seeds + base;
-    const size_t ss = sl ? (size_t)kBlockA : (size_t)np;  // the seeds' slot stride
-    const float t3[3] = {tx, ty, tz}, u3[3] = {ux, uy, uz};
+// in_lds: with a single reused pulse, its seeds are held in LDS for the launch (sl, the workgroup's array: slot a of
+// lane tid at a * kBlockA + tid); else they are added in HBM. The two are separate branches, so that each pointer
+// keeps its address space (LDS and global instructions, not flat ones that wait on both counters).
+__device__ __forceinline__ void seeds_add(double* __restrict__ sd, size_t ss, const float (&t3)[3],
+                                          const float (&u3)[3]) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) sd[(3 * i + k) * ss] += (double)t3[i] * (double)u3[k];
         sd[(9 + i) * ss] += (double)t3[i];
     }
+}
+
+template <bool SEG>
+__device__ __forceinline__ void pulse_reverse(const hb_segment& q, const hb_case& cs, const hb_adjoint& ad, int j,
+                                              const Stash& mb, const StashXY& ma, float& lx, float& ly, float& lz,
+                                              float (&sg)[N_SEG_GRADS], double* sl, bool in_lds) {
+    const int np = cs.n_pad;
+    const float c = q.w_rf_x, sn = q.w_rf_y;
+    const size_t base = (size_t)q.pad[1] * 12 * np + j;
+    const float* U = cs.maps + base;
+    const float tx = c * lx + sn * ly, ty = c * ly - sn * lx, tz = lz;
+    const float ux = c * mb.x + sn * mb.y, uy = c * mb.y - sn * mb.x, uz = mb.z;
+    const float t3[3] = {tx, ty, tz}, u3[3] = {ux, uy, uz};
+    if (in_lds)
+        seeds_add(sl + threadIdx.x, kBlockA, t3, u3);
+    else
+        seeds_add(ad.seeds + base, (size_t)np, t3, u3);
     const float bx = tx + U[0] * tx + U[3 * np] * ty + U[6 * np] * tz;
     const float by = ty + U[np] * tx + U[4 * np] * ty + U[7 * np] * tz;
     const float bz = tz + U[2 * np] * tx + U[5 * np] * ty + U[8 * np] * tz;
@@ -275,10 +282,10 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
     const int j = blockIdx.x * kBlockA + tid;
     const int np = cs.n_pad;
     const bool coil_grads = NC > 0 && ad.coil_grads;
-    double* const sl = (cs.n_pulses == 1 && ad.seeds) ? seeds_l : nullptr;  // uniform
-    if (sl) {
+    const bool seeds_in_lds = cs.n_pulses == 1 && ad.seeds;  // uniform
+    if (seeds_in_lds) {
 #pragma unroll
-        for (int a = 0; a < 12; ++a) sl[a * kBlockA + tid] = ad.seeds[(size_t)a * np + j];
+        for (int a = 0; a < 12; ++a) seeds_l[a * kBlockA + tid] = ad.seeds[(size_t)a * np + j];
     }
 #pragma unroll
     for (int a = 0; a < 8; ++a) accs[a * kBlockA + tid] = ad.gs[(size_t)a * np + j];
@@ -438,7 +445,7 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
                 }
                 if (q.flags & HB_PULSE) {
                     if (q.flags & HB_PULSE_SPOIL) lx = ly = 0.0f;
-                    pulse_reverse<SEG>(q, cs, ad, j, before[i], after[i], lx, ly, lz, sg, sl);
+                    pulse_reverse<SEG>(q, cs, ad, j, before[i], after[i], lx, ly, lz, sg, seeds_l, seeds_in_lds);
                     if (SEG) seg_add(segd, sg, i, lane);
                     seg_block = true;
                     continue;
@@ -518,9 +525,9 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
             ad.gc[(size_t)(2 * c + 1) * np + j] = accs[(9 + 2 * c) * kBlockA + tid] + (double)gi[c];
         }
     }
-    if (sl) {
+    if (seeds_in_lds) {
 #pragma unroll
-        for (int a = 0; a < 12; ++a) ad.seeds[(size_t)a * np + j] = sl[a * kBlockA + tid];
+        for (int a = 0; a < 12; ++a) ad.seeds[(size_t)a * np + j] = seeds_l[a * kBlockA + tid];
     }
     ad.lam[j] = lx, ad.lam[np + j] = ly, ad.lam[2 * np + j] = lz;
 }
