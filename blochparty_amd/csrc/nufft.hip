@@ -19,9 +19,11 @@ constexpr int kGridBlock = 256;
 constexpr int kBatch = 8;  // list entries per step of a lane's gather
 
 // One workgroup per (cell u, slot r): the cell's list entries (block b, local cell lc) are taken by G groups of
-// terms x vals lanes, group k the entries k, k + G, ... (each entry's terms x vals floats are contiguous in part:
-// one coalesced read), each lane summing in float64; then lane q adds the G groups' sums in order. Fixed order
-// throughout: the grid is the same bit for bit on every run.
+// terms x vals / 4 lanes, group k the entries k, k + G, ... (each entry's terms x vals floats are contiguous in part,
+// vals a multiple of 16: one 16-byte load per lane, the group's loads one coalesced read), each lane summing its four
+// values in float64; then the G groups' sums are added in order. With 12 lanes per entry (3 terms, 8 coils) 21 groups
+// share the cell's ~200 entries, so each lane's chain of dependent list and value loads is ~10 entries long. Fixed
+// order throughout: the grid is the same bit for bit on every run.
 // Workgroups go round robin over the 8 XCDs, so workgroup i works on item (i mod 8) * ~T/8 + i / 8 of the T (slot,
 // cell) items: each XCD takes one contiguous stretch of cells, whose entries share cache lines of part in its L2.
 __global__ __launch_bounds__(kGridBlock) void nufft_grid_kernel(const float* __restrict__ part, int n_blocks, int cols,
@@ -31,20 +33,21 @@ __global__ __launch_bounds__(kGridBlock) void nufft_grid_kernel(const float* __r
                                                                 const int* __restrict__ csr,
                                                                 const int* __restrict__ csr_base,
                                                                 double* __restrict__ grid) {
-    __shared__ double sums[kGridBlock];
+    __shared__ double sums[4 * kGridBlock];
     const int T = gridDim.x, xcd = blockIdx.x % 8, per = T / 8, extra = T % 8;
     const int w = xcd * per + min(xcd, extra) + blockIdx.x / 8;
     const int u = w % ng, r = w / ng, tid = threadIdx.x;
-    const int vt = terms * vals;
-    const int G = vt >= kGridBlock ? 1 : kGridBlock / vt;  // entry groups
+    const int vt4 = terms * vals / 4;                          // float4s per entry
+    const int span = vt4 >= kGridBlock ? kGridBlock : vt4;     // lanes per entry group
+    const int G = kGridBlock / span;                           // entry groups
     const int kind = kind_of[run_of[r]];
     const int* off = csr_off + (size_t)kind * (ng + 1);
     const int* ent = csr + 2 * (size_t)csr_base[kind];
     const int e0 = off[u], e1 = off[u + 1];
-    for (int q0 = 0; q0 < vt; q0 += kGridBlock) {  // value chunks (one unless terms x vals > 256)
-        const int k = tid / (G > 1 ? vt : kGridBlock), q = q0 + tid % (G > 1 ? vt : kGridBlock);
-        double acc = 0.0;
-        if (k < G && q < vt) {
+    for (int q0 = 0; q0 < vt4; q0 += span) {  // float4 chunks (one unless terms x vals > 1024)
+        const int k = tid / span, q = q0 + tid % span;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        if (k < G && q < vt4) {
             // kBatch entries at a time: their list reads, then their value reads, are all in flight together (the
             // sum still takes them in entry order; an entry past the end adds an exact 0)
             for (int e = e0 + k; e < e1; e += kBatch * G) {
@@ -54,23 +57,27 @@ __global__ __launch_bounds__(kGridBlock) void nufft_grid_kernel(const float* __r
                     const int ee = e + j * G;
                     be[j] = ee < e1 ? reinterpret_cast<const int2*>(ent)[ee] : make_int2(-1, 0);
                 }
-                float x[kBatch];
+                float4 x[kBatch];
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j)
-                    x[j] = be[j].x >= 0
-                               ? part[(((size_t)r * n_blocks + be[j].x) * cols + be[j].y * terms) * vals + q]
-                               : 0.0f;
+                    x[j] = be[j].x >= 0 ? reinterpret_cast<const float4*>(
+                                              part + ((size_t)r * n_blocks + be[j].x) * cols * vals +
+                                              (size_t)be[j].y * terms * vals)[q]
+                                        : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
-                for (int j = 0; j < kBatch; ++j) acc += (double)x[j];
+                for (int j = 0; j < kBatch; ++j) {
+                    a0 += (double)x[j].x, a1 += (double)x[j].y;
+                    a2 += (double)x[j].z, a3 += (double)x[j].w;
+                }
             }
         }
-        sums[tid] = acc;
+        sums[4 * tid] = a0, sums[4 * tid + 1] = a1, sums[4 * tid + 2] = a2, sums[4 * tid + 3] = a3;
         __syncthreads();
-        const int qq = q0 + tid;
-        if (tid < (G > 1 ? vt : kGridBlock) && qq < vt) {
-            double t = sums[tid];
-            for (int g = 1; g < G; ++g) t += sums[g * vt + tid];
-            const int m = qq / vals, v = qq % vals;
+        const int count = 4 * min(span, vt4 - q0);  // values of this chunk
+        for (int qq = tid; qq < count; qq += kGridBlock) {
+            double t = sums[qq];
+            for (int g = 1; g < G; ++g) t += sums[4 * g * span + qq];
+            const int qv = 4 * q0 + qq, m = qv / vals, v = qv % vals;
             grid[((size_t)(r * terms + m) * ng + u) * vals + v] = t;
         }
         __syncthreads();
