@@ -361,22 +361,31 @@ def full_variant_leg(W, driver, lib, steps: int, warmup: int, adj_steps: int, ad
     return out
 
 
-def recon_timing(run, n_coils: int, shape: tuple, steps: int) -> dict:
-    """GPU time of ``run`` (k-space (C, n_adc) complex64 on the device -> RSS image) on seeded synthetic k-space, its
-    error against numpy's centred inverse FFT and RSS in float64, and its HBM roofline."""
-    g = torch.Generator().manual_seed(5)
-    k = torch.randn(n_coils, int(np.prod(shape)), dtype=torch.complex64, generator=g)
-    kd = k.cuda()
+def gpu_ms(fn, steps: int):
+    """Mean time of fn() on the current stream (HIP events), after two untimed calls; and its last result."""
     for _ in range(2):
-        run(kd)
+        fn()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(steps):
-        img = run(kd)
+        out = fn()
     e1.record()
     e1.synchronize()
-    ms = e0.elapsed_time(e1) / steps
+    return e0.elapsed_time(e1) / steps, out
+
+
+def recon_timing(run, n_coils: int, shape: tuple, steps: int) -> dict:
+    """GPU time of ``run`` (k-space (C, n_adc) complex64 on the device -> RSS image; the HIP kernels) on seeded
+    synthetic k-space, its error against numpy's centred inverse FFT and RSS in float64, its HBM roofline, and the
+    same composition with torch's FFT (rocFFT) timed beside it."""
+    g = torch.Generator().manual_seed(5)
+    k = torch.randn(n_coils, int(np.prod(shape)), dtype=torch.complex64, generator=g)
+    kd = k.cuda()
+    ms, img = gpu_ms(lambda: run(kd), steps)
+    axes3 = tuple(range(-len(shape), 0))
+    torch_ms, _ = gpu_ms(lambda: torch.fft.fftshift(torch.fft.ifftn(torch.fft.ifftshift(
+        kd.reshape(n_coils, *shape), dim=axes3), dim=axes3), dim=axes3).abs().pow(2).sum(0).sqrt(), steps)
     t0 = time.perf_counter()
     axes = (1, 2, 3)
     kk = k.numpy().astype(np.complex128).reshape(n_coils, *shape)
@@ -386,10 +395,10 @@ def recon_timing(run, n_coils: int, shape: tuple, steps: int) -> dict:
     err = float(np.abs(img.cpu().double().numpy() - ref).max() / np.abs(ref).max())
     floor = k.numel() * 8 + ref.size * 4
     gbs = floor / (ms * 1e-3) / 1e9
-    log(f"recon (grid, ifft3c, rss) of {n_coils} x {shape}: {ms:.3f} ms on the GPU ({gbs:.0f} GB/s of the floor), "
-        f"{t_cpu * 1e3:.0f} ms numpy f64, rel err {err:.2e}")
-    return {"shape": [n_coils, *shape], "ms": ms, "steps": steps, "cpu_numpy_f64_ms": t_cpu * 1e3,
-            "rel_err_vs_numpy_f64": err,
+    log(f"recon (grid, ifft3c, rss) of {n_coils} x {shape}: {ms:.3f} ms on the GPU ({gbs:.0f} GB/s of the floor; "
+        f"torch's FFT {torch_ms:.3f} ms), {t_cpu * 1e3:.0f} ms numpy f64, rel err {err:.2e}")
+    return {"shape": [n_coils, *shape], "ms": ms, "steps": steps, "torch_fft_ms": torch_ms,
+            "cpu_numpy_f64_ms": t_cpu * 1e3, "rel_err_vs_numpy_f64": err,
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": floor}}
 

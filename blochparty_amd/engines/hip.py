@@ -15,7 +15,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent.parent / "libblochparty_amd.so"
-ABI_VERSION = 23
+ABI_VERSION = 24
 MAX_COILS = 128  # 16 tiles of 8 coils (the forward kernel's LDS budget)
 
 
@@ -104,6 +104,8 @@ SIGNATURES = {
     "hb_sur_first_quats": [_P, _P, _I, _P, _V],
     "hb_mlp_train_steps": [_P, _P, _P, _I, _I, _P, _I, _I, ctypes.c_float, ctypes.c_float, _V],
     "hb_surrogate_maps": [ctypes.POINTER(HbCase), _P, _P, _P, _P, _V],
+    "hb_ifftc_axis": [_P, _P, ctypes.c_int64, _I, ctypes.c_int64, _V],
+    "hb_rss": [_P, _I, ctypes.c_int64, _P, _V],
 }
 
 

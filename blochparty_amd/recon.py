@@ -3,9 +3,11 @@ combination, the k-space trajectory of a sequence, and the raw data as ISMRMRD a
 
 Reference interface: ``src/blochparty/recon.py:13-60`` (``kspace_grid``, ``ifftnc``, ``reconstruct``, ``rss``,
 ``coil_combine``, ``fft2c``), ``sequence.py:173-217`` (``flip_angle``, ``kspace_trajectory``) and
-``io/ismrmrd.py`` (one acquisition per readout, its trajectory and encode indices). The FFTs are torch's (rocFFT on
-the GPU): this is the step after the simulated path, not part of it. Writing an ISMRMRD HDF5 file needs ``h5py``
-(absent here, as in the reference without its ``io`` extra: ``ImportError``); the acquisitions are built without it.
+``io/ismrmrd.py`` (one acquisition per readout, its trajectory and encode indices). On the GPU, complex64 k-space with
+power-of-two axes takes the HIP kernels (``csrc/recon.hip``: one centred inverse DFT pass per axis, and the RSS);
+other shapes and dtypes, and CPU tensors, take torch's FFT as the reference does. Writing an ISMRMRD HDF5 file needs
+``h5py`` (absent here, as in the reference without its ``io`` extra: ``ImportError``); the acquisitions are built
+without it.
 """
 
 from __future__ import annotations
@@ -34,10 +36,30 @@ def kspace_grid(signal: Tensor, seq: Sequence) -> Tensor:
     raise ValueError("the sequence's meta names no Cartesian grid (kspace_shape, or matrix and kspace_index)")
 
 
+def _on_hip(t: Tensor, dtype: torch.dtype) -> bool:
+    return t.is_cuda and t.dtype == dtype
+
+
+def _pow2(n: int) -> bool:
+    return 2 <= n <= 4096 and n & (n - 1) == 0
+
+
 def ifftnc(k: Tensor, ndim: int) -> Tensor:
     """Centred inverse DFT over the last ``ndim`` axes, backward normalisation: a spin of magnetization m at a
-    voxel centre gives that voxel the value m."""
+    voxel centre gives that voxel the value m. complex64 on the GPU with power-of-two axes (2..4096): one HIP pass
+    per axis (``hb_ifftc_axis``), the first out of place; otherwise torch's FFT on the tensor's device."""
     axes = tuple(range(-ndim, 0))
+    if _on_hip(k, torch.complex64) and 1 <= ndim <= k.ndim and all(_pow2(k.shape[a]) for a in axes):
+        from .engines import hip
+
+        lib = hip.HIP.lib()
+        src = k.contiguous()
+        out = torch.empty_like(src)
+        shape = list(src.shape)
+        for i, a in enumerate(range(src.ndim - ndim, src.ndim)):
+            lib.ifftc_axis((src if i == 0 else out).data_ptr(), out.data_ptr(), math.prod(shape[:a]), shape[a],
+                           math.prod(shape[a + 1 :]))
+        return out
     return torch.fft.fftshift(torch.fft.ifftn(torch.fft.ifftshift(k, dim=axes), dim=axes), dim=axes)
 
 
@@ -67,7 +89,15 @@ def reconstruct(signal: Tensor, seq: Sequence) -> Tensor:
 
 
 def rss(images: Tensor, dim: int = 0) -> Tensor:
-    """Root-sum-of-squares over the coil axis."""
+    """Root-sum-of-squares over the coil axis (complex64 on the GPU over the leading axis: ``hb_rss``, one read of
+    the images)."""
+    if _on_hip(images, torch.complex64) and images.ndim >= 1 and dim in (0, -images.ndim) and images.numel():
+        from .engines import hip
+
+        src = images.contiguous()
+        out = torch.empty(src.shape[1:], dtype=torch.float32, device=src.device)
+        hip.HIP.lib().rss(src.data_ptr(), src.shape[0], out.numel(), out.data_ptr())
+        return out
     return (images.abs() ** 2).sum(dim=dim).sqrt()
 
 

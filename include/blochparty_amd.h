@@ -327,6 +327,16 @@ int64_t hb_seg_reduce_workspace(int32_t n_blocks, int32_t k_max);
 int hb_seg_reduce(const float* gseg, int32_t n_blocks, int32_t k_max, int32_t s0, int32_t len, double* work,
                   double* seg, void* stream);
 
+/* Downstream of the signal (replaces the torch FFT calls of recon.py:17-38 for complex64 data on the GPU).
+ * hb_ifftc_axis: src viewed as (outer, L, inner) complex64 values (re, im interleaved), into dst (which may be src)
+ * along the middle axis: y[m] = (-1)^(m + L/2) / L * sum_l (-1)^l x[l] e^(2 pi i l m / L) = fftshift(ifft(ifftshift(x))) for even L, the
+ * centred inverse DFT with the backward normalisation (recon.py:17-22 ifftnc, one call per axis). L a power of two,
+ * 2..4096; outer >= 0, inner >= 1.
+ * hb_rss: out[v] = sqrt(sum_c |x[c][v]|^2) for x (n_coils, per_coil) complex64, the coils summed in order in float32
+ * (recon.py:36-37 rss over dim 0). */
+int hb_ifftc_axis(const float* src, float* dst, int64_t outer, int32_t L, int64_t inner, void* stream);
+int hb_rss(const float* x, int32_t n_coils, int64_t per_coil, float* out, void* stream);
+
 /* Name of a return code. */
 const char* hb_error_string(int code);
 
