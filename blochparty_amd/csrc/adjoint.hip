@@ -383,10 +383,7 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
             const int head = qf.pad[3];
             const int b_min = head < s0 ? 0 : (head + 1 - s0 + 7) >> 3;
             const int lo = s0 + 8 * b_min;
-            if (SEG) {
-                float* gz = ad.gseg + ((size_t)blockIdx.x * ad.k_max + (lo - s0)) * 8;
-                for (int t = tid; t < 8 * (sb + nseg - lo); t += kBlockA) gz[t] = 0.0f;
-            }
+            // (their gseg rows are not written: hb_seg_reduce, given the records, sums bodies as 0)
             flush_point = (sb & ~63) >= lo;  // one of the skipped blocks starts at a multiple of 64
             b = b_min;
         } else {
@@ -534,18 +531,22 @@ __global__ __launch_bounds__(kBlockA) __attribute__((amdgpu_waves_per_eu(NC <= 8
 
 // seg[s0 + k][g] = sum over workgroups of gseg[b][k][g], float64, in a fixed order: the workgroups in chunks of
 // kSegChunk summed in order by one lane per (k, g) (consecutive lanes read consecutive floats), then the chunks in
-// order.
+// order. With the records (segs non-NULL), a body of a reused pulse's occurrence or of a NUFFT run (HB_PULSE_BODY:
+// the identity, whose gradients are 0 and whose gseg rows the adjoint kernel leaves unwritten) sums to 0 unread.
 constexpr int kSegChunk = 256;
 
 __global__ __launch_bounds__(256) void seg_chunks_kernel(const float* __restrict__ gseg, int n_blocks, int k_max,
-                                                         int len, double* __restrict__ work) {
+                                                         int len, const hb_segment* __restrict__ segs, int s0,
+                                                         double* __restrict__ work) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 8 * len) return;
-    const int b0 = blockIdx.y * kSegChunk, b1 = min(n_blocks, b0 + kSegChunk);
     const size_t stride = (size_t)k_max * 8;
     double acc = 0.0;
+    if (!segs || !(segs[s0 + (i >> 3)].flags & HB_PULSE_BODY)) {
+        const int b0 = blockIdx.y * kSegChunk, b1 = min(n_blocks, b0 + kSegChunk);
 #pragma unroll 16
-    for (int b = b0; b < b1; ++b) acc += (double)gseg[b * stride + i];
+        for (int b = b0; b < b1; ++b) acc += (double)gseg[b * stride + i];
+    }
     work[blockIdx.y * stride + i] = acc;
 }
 
@@ -595,13 +596,13 @@ extern "C" int64_t hb_seg_reduce_workspace(int32_t n_blocks, int32_t k_max) {
 }
 
 extern "C" int hb_seg_reduce(const float* gseg, int32_t n_blocks, int32_t k_max, int32_t s0, int32_t len,
-                             double* work, double* seg, void* stream) {
+                             double* work, double* seg, const hb_segment* segs, void* stream) {
     using namespace hb;
     if (n_blocks < 1 || len < 0 || len > k_max || s0 < 0 || !work) return hipErrorInvalidValue;
     if (len == 0) return hipSuccess;
     const int chunks = (n_blocks + kSegChunk - 1) / kSegChunk;
     hipLaunchKernelGGL(seg_chunks_kernel, dim3((8 * len + 255) / 256, chunks), dim3(256), 0, (hipStream_t)stream,
-                       gseg, n_blocks, k_max, len, work);
+                       gseg, n_blocks, k_max, len, segs, s0, work);
     hipLaunchKernelGGL(seg_final_kernel, dim3((8 * len + 255) / 256), dim3(256), 0, (hipStream_t)stream, work,
                        chunks, k_max, s0, len, seg);
     return hipGetLastError();

@@ -547,4 +547,4 @@ extern "C" int hb_signal_reduce_noalloc(const float* partial, int32_t n_blocks, 
 
 extern "C" const char* hb_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 
-extern "C" int hb_abi_version(void) { return 24; }
+extern "C" int hb_abi_version(void) { return 25; }

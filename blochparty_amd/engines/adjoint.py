@@ -245,7 +245,8 @@ def _sweep(case, cs, ckpts, every, h, lam, sens, nc, need_seg, need_coil, lib, s
                            ptr(grid), nc, int(need_seg), int(need_coil), every)
         lib.adjoint_interval(ctypes.byref(cs), ctypes.byref(ad), s0, s1)
         if need_seg:
-            lib.seg_reduce(gseg.data_ptr(), n_blocks, every, s0, s1 - s0, red.data_ptr(), seg.data_ptr())
+            lib.seg_reduce(gseg.data_ptr(), n_blocks, every, s0, s1 - s0, red.data_ptr(), seg.data_ptr(),
+                           case.segs.data_ptr())
 
 
 FD_TURN = 1e-4  # rad: how far a central-difference step turns a reused pulse's rotation

@@ -213,7 +213,7 @@ def _sweep(case, ckpts, every, h, lam, sens, nc, need_seg, need_coil, gs, gc, se
         if rc != 0:
             raise RuntimeError(f"HIP error {rc} in hb_adjoint_full: {lib.hb_error_string(rc).decode()}")
         if need_seg:
-            kern.seg_reduce(gseg.data_ptr(), n_blocks, every, s0, s1 - s0, red.data_ptr(), seg.data_ptr())
+            kern.seg_reduce(gseg.data_ptr(), n_blocks, every, s0, s1 - s0, red.data_ptr(), seg.data_ptr(), None)
 
 
 def simulate_grad(phantom: Phantom, seq: Sequence, coils, m_init, checkpoint_every, conc=None) -> SimResult:

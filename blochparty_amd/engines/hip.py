@@ -15,7 +15,7 @@ from pathlib import Path
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent.parent / "libblochparty_amd.so"
-ABI_VERSION = 24
+ABI_VERSION = 25
 MAX_COILS = 128  # 16 tiles of 8 coils (the forward kernel's LDS budget)
 
 
@@ -92,7 +92,7 @@ SIGNATURES = {
     "hb_nufft_signal": [_P, _I, _P, _P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _V],
     "hb_nufft_adjoint_grid": [_P, _I, _P, _I, _I, _I, _I, _P, _P, _P, _V],
     "hb_adjoint_interval": [ctypes.POINTER(HbCase), ctypes.POINTER(HbAdjoint), _I, _I, _V],
-    "hb_seg_reduce": [_P, _I, _I, _I, _I, _P, _P, _V],
+    "hb_seg_reduce": [_P, _I, _I, _I, _I, _P, _P, _P, _V],
     "hb_pulse_maps": [ctypes.POINTER(HbCase), _P, _P, _I, _P, _I, _P, _V],
     "hb_pulse_map_grads": [ctypes.POINTER(HbCase), _P, _P, _I, _P, _P, _P, _V],
     "hb_pulse_map_dlnb1": [ctypes.POINTER(HbCase), _P, _P, _I, ctypes.c_double, _P, _V],

@@ -322,10 +322,12 @@ int hb_adjoint_interval(const hb_case* cs, const hb_adjoint* ad, int32_t s0, int
 /* Fixed-order sum of an interval's per-workgroup segment gradients (replaces seg_reduce, cuda/seg_reduce.cu:3-11):
  * seg[(s0 + k) * 8 + g] = sum over b < n_blocks of gseg[(b * k_max + k) * 8 + g], float64, for k < len (the
  * workgroups in chunks of 256, each summed in order, then the chunks in order). work:
- * hb_seg_reduce_workspace(n_blocks, k_max) doubles. */
+ * hb_seg_reduce_workspace(n_blocks, k_max) doubles. segs: the case's records after hb_adjoint_interval (whose
+ * HB_PULSE_BODY segments sum to 0 without their gseg rows, which that kernel does not write), or NULL after
+ * hb_adjoint_full (every row written). */
 int64_t hb_seg_reduce_workspace(int32_t n_blocks, int32_t k_max);
 int hb_seg_reduce(const float* gseg, int32_t n_blocks, int32_t k_max, int32_t s0, int32_t len, double* work,
-                  double* seg, void* stream);
+                  double* seg, const hb_segment* segs, void* stream);
 
 /* Downstream of the signal (replaces the torch FFT calls of recon.py:17-38 for complex64 data on the GPU).
  * hb_ifftc_axis: src viewed as (outer, L, inner) complex64 values (re, im interleaved), into dst (which may be src)
