@@ -9,9 +9,9 @@
 // patterns). A 2D or 3D ifftnc is one call per axis (the sign patterns factor over the axes).
 // One workgroup per tile of TI lines along the axis (TI consecutive rows when inner = 1, TI consecutive inner indices
 // otherwise, so that every load and store is coalesced; TI a power of two): the tile is loaded into LDS with the input
-// signs, transformed by a radix-2 Stockham FFT (log2 L stages, each read into registers, a barrier, written to its new
-// places, a barrier; the twiddles from a table of L/2 computed once per workgroup), and stored with the output signs
-// and 1/L. All index arithmetic is shifts and masks.
+// signs, transformed by a Stockham FFT (radix-4 stages, and one radix-2 stage when log2 L is odd; each stage read
+// into registers, a barrier, written to its new places, a barrier; the twiddles from a table of L/2 computed once per
+// workgroup), and stored with the output signs and 1/L. All index arithmetic is shifts and masks.
 // hb_rss: out[v] = sqrt(sum_c |x[c][v]|^2), the coils summed in order in float32.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -67,9 +67,53 @@ __global__ __launch_bounds__(kFftThreads) void ifftc_axis_kernel(const float2* s
         buf[(l << ls) + (t << lt)] = v;
     }
     __syncthreads();
-    const int n_bf = half << lgT, lgh = lgL - 1;
-    for (int lgns = 0; lgns < lgL; ++lgns) {
-        const int ns = 1 << lgns;
+    // e^(2 pi i t / L) for t < L, from the half table
+    auto W = [&](int t) {
+        const float2 w = tw[t & (half - 1)];
+        return t < half ? w : make_float2(-w.x, -w.y);
+    };
+    // radix-4 stages while four fit (sub-transforms of ns values grow 4x), then one radix-2 stage if log2 L is odd
+    int lgns = 0;
+    for (; lgns + 2 <= lgL; lgns += 2) {
+        const int ns = 1 << lgns, lgq = lgL - 2, q4 = 1 << lgq, n_bf = q4 << lgT, sh = lgL - lgns - 2;
+        float2 v[kFftPerThread / 2][4];
+        int to[kFftPerThread / 2];
+#pragma unroll
+        for (int r = 0; r < kFftPerThread / 2; ++r) {
+            const int idx = tid + r * kFftThreads;
+            if (idx < n_bf) {
+                const int j = rows ? idx & (q4 - 1) : idx >> lgT, t = rows ? idx >> lgq : idx & (TI - 1);
+                const int k = j & (ns - 1);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) v[r][c] = buf[((j + (c << lgq)) << ls) + (t << lt)];
+#pragma unroll
+                for (int c = 1; c < 4; ++c) {  // v_c e^(2 pi i k c / (4 ns))
+                    const float2 w = W((k * c) << sh), x = v[r][c];
+                    v[r][c] = make_float2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
+                }
+                // the inverse 4-point DFT: V_s = sum_c v_c i^(c s)
+                const float2 a = v[r][0], b = v[r][1], c2 = v[r][2], d = v[r][3];
+                const float2 apc = make_float2(a.x + c2.x, a.y + c2.y), amc = make_float2(a.x - c2.x, a.y - c2.y);
+                const float2 bpd = make_float2(b.x + d.x, b.y + d.y), bmd = make_float2(b.x - d.x, b.y - d.y);
+                v[r][0] = make_float2(apc.x + bpd.x, apc.y + bpd.y);
+                v[r][1] = make_float2(amc.x - bmd.y, amc.y + bmd.x);  // a + i b - c - i d
+                v[r][2] = make_float2(apc.x - bpd.x, apc.y - bpd.y);
+                v[r][3] = make_float2(amc.x + bmd.y, amc.y - bmd.x);  // a - i b - c + i d
+                to[r] = ((((j - k) << 2) + k) << ls) + (t << lt);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kFftPerThread / 2; ++r) {
+            if (tid + r * kFftThreads < n_bf) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) buf[to[r] + ((c << lgns) << ls)] = v[r][c];
+            }
+        }
+        __syncthreads();
+    }
+    if (lgns < lgL) {  // the last stage, radix 2: ns = L / 2, so k = j and the twiddle is e^(2 pi i j / L)
+        const int n_bf = half << lgT, lgh = lgL - 1;
         float2 va[kFftPerThread], vb[kFftPerThread];
         int to[kFftPerThread];
 #pragma unroll
@@ -78,12 +122,11 @@ __global__ __launch_bounds__(kFftThreads) void ifftc_axis_kernel(const float2* s
             if (idx < n_bf) {
                 const int j = rows ? idx & (half - 1) : idx >> lgT, t = rows ? idx >> lgh : idx & (TI - 1);
                 const float2 a = buf[(j << ls) + (t << lt)], b = buf[((j + half) << ls) + (t << lt)];
-                const int k = j & (ns - 1);
-                const float2 w = tw[k << (lgh - lgns)];  // e^(i pi k / ns)
+                const float2 w = tw[j];
                 const float2 bw = make_float2(b.x * w.x - b.y * w.y, b.x * w.y + b.y * w.x);
                 va[r] = make_float2(a.x + bw.x, a.y + bw.y);
                 vb[r] = make_float2(a.x - bw.x, a.y - bw.y);
-                to[r] = ((((j - k) << 1) + k) << ls) + (t << lt);
+                to[r] = (j << ls) + (t << lt);
             }
         }
         __syncthreads();
@@ -91,7 +134,7 @@ __global__ __launch_bounds__(kFftThreads) void ifftc_axis_kernel(const float2* s
         for (int r = 0; r < kFftPerThread; ++r) {
             if (tid + r * kFftThreads < n_bf) {
                 buf[to[r]] = va[r];
-                buf[to[r] + (ns << ls)] = vb[r];
+                buf[to[r] + (half << ls)] = vb[r];
             }
         }
         __syncthreads();
