@@ -400,7 +400,9 @@ def recon_timing(run, n_coils: int, shape: tuple, steps: int) -> dict:
     return {"shape": [n_coils, *shape], "ms": ms, "steps": steps, "torch_fft_ms": torch_ms,
             "cpu_numpy_f64_ms": t_cpu * 1e3, "rel_err_vs_numpy_f64": err,
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": floor}}
+                         "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": floor,
+                         # what the HIP passes move: a read and a write of every value per axis, then the RSS
+                         "traffic": len(shape) * 16 * k.numel() + 8 * k.numel() + 4 * ref.size}}
 
 
 def downstream_leg(W, driver, n: int, n_coils: int, steps: int) -> dict:
