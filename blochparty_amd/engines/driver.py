@@ -17,6 +17,7 @@ work (the benchmark times ``run_forward`` on resident buffers).
 
 from __future__ import annotations
 
+import functools
 import math
 from dataclasses import dataclass, field
 
@@ -25,7 +26,7 @@ import torch
 
 from ..phantom import Phantom
 from ..result import SimResult
-from ..sequence import Sequence
+from ..sequence import Sequence, flag_array, flag_memo
 from . import hip, pulses
 
 BLOCK = 256
@@ -48,6 +49,16 @@ def _close(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     return np.abs(a - b) <= SAME_TOL * np.maximum(np.abs(a), np.abs(b))
 
 
+def _span_index(starts: np.ndarray, lengths: np.ndarray) -> np.ndarray:
+    """The indices of the ranges [starts[i], starts[i] + lengths[i]), concatenated in order."""
+    lengths = np.maximum(lengths, 0)
+    total = int(lengths.sum())
+    if total == 0:
+        return np.zeros(0, dtype=np.int64)
+    offsets = np.cumsum(lengths) - lengths  # where each range begins in the output
+    return np.arange(total, dtype=np.int64) + np.repeat(starts - offsets, lengths)
+
+
 def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = False, nufft=None) -> np.ndarray:
     """The (S,) records of ``seq``; ``starts`` are the dispatch starts (the factor caches restart there); ``pulses``:
     reused pulses (``pulses.reused_pulses``), whose occurrences are applied as maps and not stepped; ``runs``: mark
@@ -59,9 +70,7 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
     g = seq.grad.detach().double().cpu().numpy()
     b0 = np.zeros(S) if seq.b0_offset is None else seq.b0_offset.detach().double().cpu().numpy()
     psi = np.remainder(seq.adc_phase.detach().double().cpu().numpy(), 2 * math.pi)
-    is_rf = np.asarray(seq.is_rf, dtype=bool)
-    adc = np.asarray(seq.adc, dtype=bool)
-    spoil = np.asarray(seq.spoil, dtype=bool)
+    is_rf, adc, spoil = flag_array(seq.is_rf), flag_array(seq.adc), flag_array(seq.spoil)
     rec = np.zeros(S, dtype=SEG_DTYPE)
     packed = np.empty((S, 8), dtype=np.float32)
     for c, col in enumerate((dt, rf[:, 0], rf[:, 1], g[:, 0], g[:, 1], g[:, 2], b0, psi)):
@@ -80,24 +89,25 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
     chunk_id = np.cumsum(first) - 1
     stepped = np.ones(S, dtype=bool)  # segments the kernels step (not inside a reused pulse's occurrence)
     for pid, (_, k, occ) in enumerate(pulses):
-        for a, ph in occ:
-            stepped[a : a + k] = False
-            flags[a] |= PULSE
-            if spoil[a + k - 1]:
-                flags[a] |= PULSE_SPOIL
-            rec["pad"][a, 1], rec["pad"][a, 2] = pid, k
-            rec["f"][a, 7] = np.float32(ph % (2 * math.pi))
+        heads = np.fromiter((a for a, _ in occ), dtype=np.int64, count=len(occ))
+        phase = np.fromiter((ph for _, ph in occ), dtype=np.float64, count=len(occ))
+        stepped[_span_index(heads, np.full(len(occ), k))] = False
+        flags[heads] |= PULSE
+        flags[heads[spoil[heads + k - 1]]] |= PULSE_SPOIL
+        rec["pad"][heads, 1], rec["pad"][heads, 2] = pid, k
+        rec["f"][heads, 7] = np.remainder(phase, 2 * math.pi).astype(np.float32)
     if nufft is not None:  # NUFFT readout runs: slot in their dispatch, kind, length; their samples are not stepped
-        slot = {}
-        for (a, k), kind in zip(nufft.runs, nufft.kind):
-            c = int(chunk_id[a])
-            sl = slot.get(c, 0)
-            slot[c] = sl + 1
-            flags[a] |= NUFFT
-            if spoil[a + k - 1]:
-                flags[a] |= PULSE_SPOIL
-            rec["pad"][a, 1], rec["pad"][a, 2], rec["pad"][a, 3] = sl, kind, k
-            stepped[a : a + k] = False
+        heads = np.fromiter((a for a, _ in nufft.runs), dtype=np.int64, count=len(nufft.runs))
+        lens = np.fromiter((k for _, k in nufft.runs), dtype=np.int64, count=len(nufft.runs))
+        if len(heads):
+            if np.any(np.diff(heads) <= 0):
+                raise ValueError("NUFFT runs must be in segment order")
+            c = chunk_id[heads]
+            slot = np.arange(len(heads)) - np.searchsorted(c, c, side="left")  # the run's rank in its dispatch
+            flags[heads] |= NUFFT
+            flags[heads[spoil[heads + lens - 1]]] |= PULSE_SPOIL
+            rec["pad"][heads, 1], rec["pad"][heads, 2], rec["pad"][heads, 3] = slot, np.asarray(nufft.kind), lens
+            stepped[_span_index(heads, lens)] = False
     key = np.stack([dt, g[:, 0], g[:, 1], g[:, 2], b0], 1)
     zapped = np.zeros(S, dtype=bool)
     if runs:
@@ -138,9 +148,12 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
     mark_trains(flags, rec, is_rf, adc, spoil, chunk_id, stepped)
     # an occurrence's or a NUFFT run's other segments: the identity (the record is applied at its first segment)
     spans = [(a, k) for _, k, occ in pulses for a, _ in occ] + (list(nufft.runs) if nufft is not None else [])
-    for a, k in spans:
-        flags[a + 1 : a + k] = PULSE_BODY
-        rec["pad"][a + 1 : a + k, 3] = a
+    if spans:
+        sa = np.fromiter((a for a, _ in spans), dtype=np.int64, count=len(spans))
+        sk = np.fromiter((k for _, k in spans), dtype=np.int64, count=len(spans))
+        body = _span_index(sa + 1, sk - 1)
+        flags[body] = PULSE_BODY
+        rec["pad"][body, 3] = np.repeat(sa, sk - 1)
     if nufft is not None:
         for r, (a, _) in enumerate(nufft.runs):
             rec["pad"][a, 4] = r  # the run's index (the adjoint's grid)
@@ -151,8 +164,9 @@ def segment_table(seq: Sequence, starts: list[int], pulses=(), runs: bool = Fals
         packed[:, c] = col
     rec["w"] = packed
     for _, _, occ in pulses:  # a reused pulse's head is never stepped: its w_rf holds cos, sin of the RF phase
-        for a, ph in occ:
-            rec["w"][a, 1], rec["w"][a, 2] = math.cos(ph), math.sin(ph)
+        heads = np.fromiter((a for a, _ in occ), dtype=np.int64, count=len(occ))
+        rec["w"][heads, 1] = np.fromiter((math.cos(ph) for _, ph in occ), dtype=np.float64, count=len(occ))
+        rec["w"][heads, 2] = np.fromiter((math.sin(ph) for _, ph in occ), dtype=np.float64, count=len(occ))
     rec["pad"][:, 0] = np.where(adc, np.cumsum(adc) - 1, -1)  # ADC row
     ends_adc = np.flatnonzero((flags & MERGED != 0) & (flags & ADC != 0))  # merged runs sampled at their end
     rec["pad"][ends_adc, 0] = rec["pad"][ends_adc + rec["pad"][ends_adc, 3] - 1, 0]
@@ -288,7 +302,7 @@ def dispatch_starts(seq: Sequence, chunk: int, rows_cap: int, spans=(), rows_w=N
     """Dispatch boundaries: at most ``chunk`` segments, ``rows_cap`` partial rows (``rows_w``: the segments whose
     sample has a partial row; default every ADC segment) and ``runs_cap`` NUFFT runs (``heads``: their first
     segments) each, never inside one of ``spans`` ((first, length) of reused-pulse occurrences and NUFFT runs)."""
-    adc = np.asarray(seq.adc, dtype=np.int64) if rows_w is None else np.asarray(rows_w, dtype=np.int64)
+    adc = flag_array(seq.adc).astype(np.int64) if rows_w is None else np.asarray(rows_w, dtype=np.int64)
     S = len(seq)
     inside = np.zeros(S + 1, dtype=bool)  # a cut before segment e would split a span
     for a, k in spans:
@@ -415,6 +429,18 @@ def check_output(case: DeviceCase, *outs: torch.Tensor, premise: bool = True) ->
         raise RuntimeError("hip engine: non-finite result from finite inputs")
 
 
+def _with_flag_memo(fn):
+    """Run ``fn`` with each of the sequence's flag lists converted to an array once (``sequence.flag_memo``)."""
+
+    @functools.wraps(fn)
+    def call(*args, **kwargs):
+        with flag_memo():
+            return fn(*args, **kwargs)
+
+    return call
+
+
+@_with_flag_memo
 def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, chunk: int = DEFAULT_CHUNK,
                  device=None, partial_bytes: int = PARTIAL_BYTES, aligned: bool = False,
                  reuse: bool = False, surrogate: bool = False, nufft=None, conc=None) -> DeviceCase:
@@ -482,7 +508,7 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
     rows_cap = max(8, (partial_bytes // (4 * vals * n_blocks)) // 8 * 8)
     reused = pulses.reused_pulses(seq) if reuse else []
     spans = [(a, k) for _, k, occ in reused for a, _ in occ]
-    adc_np = np.asarray(seq.adc, dtype=np.int64)
+    adc_np = flag_array(seq.adc).astype(np.int64)
     rows_w = adc_np.copy()  # samples with a partial row (not in NUFFT runs)
     if plan is not None:
         for a, k in plan.runs:
@@ -502,14 +528,15 @@ def prepare_case(phantom: Phantom, seq: Sequence, coils=None, m_init=None, *, ch
     for _, k, occ in reused:
         for a, _ in occ:
             covered[a : a + k] = True
-    rf_cum = np.concatenate([[0], np.cumsum(np.asarray(seq.is_rf, dtype=bool) & ~covered)])
+    rf_cum = np.concatenate([[0], np.cumsum(flag_array(seq.is_rf) & ~covered)])
     # the coil contraction's scales: |M| <= max(PD, |m_init|) along any sequence (rotations keep |M|, relaxation
     # moves it toward (0, 0, PD))
     sens_a = sens_operand(coils, n, n_pad, n_tiles)
     sens_scale = pow2_scale(float(sens_a.abs().max()) if sens_a.numel() else 0.0)
     x_scale = pow2_scale(max(float(phantom.pd.detach().abs().max()) if n else 0.0,
-                             float(m64.norm(dim=0).max()) if n else 0.0))
-    psi_rows = np.remainder(seq.adc_phase.detach().double().cpu().numpy()[np.asarray(seq.adc, dtype=bool)],
+                             float((m64[0] * m64[0] + m64[1] * m64[1] + m64[2] * m64[2]).max().sqrt()) if n
+                             else 0.0))
+    psi_rows = np.remainder(seq.adc_phase.detach().double().cpu().numpy()[flag_array(seq.adc)],
                             2 * math.pi)
     max_rows = max(rows + [0])
     case = DeviceCase(

@@ -19,6 +19,7 @@ import numpy as np
 import torch
 
 from .. import exchange as X
+from ..sequence import flag_array
 from . import hip
 
 LINE = {"lorentzian": 1, "superlorentzian": 2}
@@ -116,7 +117,7 @@ def needed(phantom, seq, conc=None) -> bool:
 def mark_bound_only(rec, seq) -> None:
     """HB_BOUND_ONLY on the records of RF that only the bound pool feels (the water ignores it)."""
     if seq.bound_only is not None:
-        rec["flags"][np.asarray(seq.bound_only, dtype=bool)] |= BOUND_ONLY
+        rec["flags"][flag_array(seq.bound_only)] |= BOUND_ONLY
 
 
 __all__ = ["HbMt", "attach", "bind", "check_pool", "forward_dispatch", "mark_bound_only", "needed", "struct"]

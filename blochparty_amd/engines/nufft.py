@@ -40,7 +40,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from ..sequence import Sequence
+from ..sequence import Sequence, flag_array
 
 MIN_SAMPLES = 16  # readouts shorter than this keep the per-sample path
 WMAX = 32  # widest window of grid cells one workgroup may span
@@ -54,7 +54,7 @@ def readout_runs(seq: Sequence) -> list[tuple[int, int]]:
     RF in which every segment has the previous one's dt, gradient and b0_offset (to TOL) and no segment but the last
     is spoiled; of those, only the kind (samples, dt) that occurs most often (ties: the earliest). A readout whose
     first sample ends a half-dwell segment starts its run at the second sample, where the dt becomes constant."""
-    adc, rf, sp = (np.asarray(f, dtype=bool) for f in (seq.adc, seq.is_rf, seq.spoil))
+    adc, rf, sp = (flag_array(f) for f in (seq.adc, seq.is_rf, seq.spoil))
     S = len(adc)
     if S == 0:
         return []

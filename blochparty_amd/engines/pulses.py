@@ -22,7 +22,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from ..sequence import Sequence
+from ..sequence import Sequence, flag_array
 
 
 def rf_runs(is_rf: np.ndarray, adc: np.ndarray, spoil: np.ndarray) -> list[tuple[int, int]]:
@@ -49,7 +49,7 @@ def _within(diff_peak: np.ndarray, peak_a: np.ndarray, peak_b: np.ndarray, tol: 
 def reused_pulses(seq: Sequence, tol: float = 1e-6) -> list[tuple[int, int, list[tuple[int, float]]]]:
     """[(first segment, length, [(occurrence start, phase [rad]), ...])] of every pulse that occurs at least twice
     (the module's rules), in the order of first occurrence."""
-    is_rf, adc, spoil = (np.asarray(f, dtype=bool) for f in (seq.is_rf, seq.adc, seq.spoil))
+    is_rf, adc, spoil = (flag_array(f) for f in (seq.is_rf, seq.adc, seq.spoil))
     dt = seq.dt.detach().double().cpu().numpy()
     rf = seq.rf.detach().double().cpu().numpy()
     z_all = rf[:, 0] + 1j * rf[:, 1]

@@ -7,6 +7,8 @@ Units: RF in Hz (gamma_bar B1), gradients in Hz/m, so a gradient's time integral
 
 from __future__ import annotations
 
+import contextlib
+import contextvars
 from dataclasses import dataclass, field, replace
 
 import numpy as np
@@ -105,3 +107,37 @@ class Sequence:
             rf_freq,
             None if self.bound_only is None else np.asarray(self.bound_only, dtype=bool)[first].tolist(),
         )
+
+
+# Flag lists (is_rf, adc, spoil, bound_only) as bool arrays. Converting a Python list of a million bools takes tens of
+# milliseconds, and one case's preparation reads each list in several places; inside ``flag_memo()`` each list object
+# is converted once (the memo holds the list itself, so an id is never reused while it lives). Outside it, every call
+# converts, so lists changed in place between calls are always seen.
+_FLAG_MEMO: contextvars.ContextVar[dict | None] = contextvars.ContextVar("blochparty_amd_flag_memo", default=None)
+
+
+def flag_array(flags) -> np.ndarray:
+    """``flags`` as a read-only bool array."""
+    memo = _FLAG_MEMO.get()
+    if memo is not None:
+        hit = memo.get(id(flags))
+        if hit is not None and hit[0] is flags:
+            return hit[1]
+    arr = np.asarray(flags, dtype=bool)
+    if arr is flags:  # already a bool array: a read-only view, not the caller's array itself
+        arr = arr.view()
+    arr.flags.writeable = False
+    if memo is not None:
+        memo[id(flags)] = (flags, arr)
+    return arr
+
+
+@contextlib.contextmanager
+def flag_memo():
+    """Within the block, ``flag_array`` converts each list object once."""
+    token = _FLAG_MEMO.set({})
+    try:
+        yield
+    finally:
+        _FLAG_MEMO.reset(token)
+
