@@ -249,7 +249,7 @@ __device__ __forceinline__ void nufft_run(const hb_case& cs, const hb_nufft& nu,
 // nu_in): no store of the kernel can change the records, so they load through the scalar cache instead of one vector
 // load per wave and a readfirstlane per value.
 template <int NT, int NUT, bool RF>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 ? 3 : 4))) void forward_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT == 2 ? 3 : 5))) void forward_kernel(
     hb_case cs_in, int s0, int s1, float* __restrict__ partial, int rows, hb_nufft nu_in,
     const hb_segment* __restrict__ segs, const float* __restrict__ spins, const float* __restrict__ pos_lo,
     const float* __restrict__ sens_a, const float* __restrict__ maps, const float* __restrict__ nu_params,
